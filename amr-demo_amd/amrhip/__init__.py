@@ -1,0 +1,404 @@
+"""ctypes binding of libamrhip.so (the C ABI in include/amr_hip.h,
+include/amr_forest.h, include/amr_driver.h).
+
+The library is the product; this module only marshals numpy arrays for the
+tests and the benchmark.  Loading fails loudly when the shared library has
+not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libamrhip.so")
+
+_i = C.c_int
+_d = C.c_double
+_pi = C.POINTER(C.c_int)
+_pd = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+# every exported entry point of the three public headers, with its signature
+EXPORTS = {
+    # include/amr_hip.h
+    "hpc_amr_initialize_solver": (None, [_pi, _pi, _pd, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi]),
+    "physics_gpu_set_params": (None, [_vp]),
+    "physics_gpu_get_params": (None, [_vp]),
+    "physics_go_gpu": (_i, [_pi] * 15 + [_pd, _pd] + [_pi] * 5 + [_vp, _pd, _pd, _pd]),
+    "hpc_amr_evolve_solver": (_i, [_pi] * 15 + [_pd, _pd] + [_pi] * 5 + [_vp, _pd, _pd, _pd]),
+    "physics_rhs_structured_gpu": (_i, [_pi, _pi, _pi, _pi, _pi, _pd, _pi, _pi, _pi, _pd, _pd]),
+    "physics_time_step_gpu": (_i, [_pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd]),
+    "amrgpu_create": (_vp, []),
+    "amrgpu_destroy": (None, [_vp]),
+    "amrgpu_last_error": (C.c_char_p, []),
+    "amrgpu_setup": (_i, [_vp, _i, _i, _i, _vp, _i, _pi, _pi, _i, _pi, _pd, _pd]),
+    "amrgpu_upload_soln": (_i, [_vp, _pd]),
+    "amrgpu_download_soln": (_i, [_vp, _pd]),
+    "amrgpu_rhs": (_i, [_vp, _pd]),
+    "amrgpu_cfl": (_i, [_vp, _pd]),
+    "amrgpu_run": (_i, [_vp, _i, _i, _pd]),
+    "amrgpu_get_time": (_i, [_vp, _pd, _pd]),
+    "amrgpu_set_time": (_i, [_vp, _pd]),
+    "amrgpu_bench": (_i, [_vp, _i, _pd, _pd, _pi]),
+    "amrgpu_num_cells": (_i, [_vp]),
+    "amrgpu_num_quads": (_i, [_vp]),
+    "amrgpu_copy_probe": (_i, [C.c_longlong, _i, _pd]),
+    # include/amr_forest.h
+    "amr_forest_new": (_vp, [_i, _i, _i, _i, _d, _d, _d, _i, _i, _i]),
+    "amr_forest_destroy": (None, [_vp]),
+    "amr_forest_num_quads": (_i, [_vp]),
+    "amr_forest_nelem_subgrid": (_i, [_vp]),
+    "amr_forest_quads": (_vp, [_vp]),
+    "amr_forest_soln": (_pd, [_vp]),
+    "amr_forest_tags": (_pi, [_vp]),
+    "amr_forest_scale": (_d, [_vp]),
+    "amr_forest_quad_geom": (None, [_vp, _i, _pd]),
+    "amr_forest_build_external": (_i, [_vp, _pi, _pi, _pi, _pi, C.POINTER(_pi), C.POINTER(_pi),
+                                       C.POINTER(_pd), C.POINTER(_pd)]),
+    "amr_forest_refine": (_i, [_vp, _pi]),
+    "amr_forest_coarsen": (_i, [_vp]),
+    "amr_forest_balance": (_i, [_vp]),
+    "amr_forest_is_balanced": (_i, [_vp]),
+    "amr_forest_find_leaf": (_i, [_vp, _i, _i, _i]),
+    # include/amr_driver.h
+    "amr_ic_patch": (None, [_vp, _i, _pd, _pd]),
+    "amr_ic_apply": (None, [_vp, _vp]),
+    "amr_ic_vortex_density": (_d, [_vp, _d, _d, _d, _d, _d, _d, _d]),
+    "amr_indicator_gradient": (None, [_vp, _pd]),
+    "amr_regrid": (_i, [_vp, _d, _i]),
+    "amr_regrid_initial": (_i, [_vp, _vp, _d]),
+}
+
+
+class PhysicsParams(C.Structure):
+    """physics_gpu_params_t"""
+    _fields_ = [
+        ("dim", _i), ("nfields", _i), ("nelem_subgrid", _i),
+        ("extern_cell_size", _i), ("extern_face_size", _i),
+        ("time_scheme", _i), ("time_steps", _i), ("dt", _d),
+        ("checkpoint_interval", _i), ("visualization_interval", _i), ("regrid_interval", _i),
+        ("bc", _i * 6), ("cfl", _d), ("coarse_grid_h", _d), ("t_end", _d), ("verbose", _i),
+    ]
+
+
+class InstParams(C.Structure):
+    """amrgpu_inst_t"""
+    _fields_ = [("h_base", _d), ("bc", _i * 4), ("cfl", _d), ("dt_cap", _d), ("t_end", _d)]
+
+
+class Quad(C.Structure):
+    _fields_ = [("tree", _i), ("level", _i), ("x", _i), ("y", _i)]
+
+
+class IC(C.Structure):
+    """amr_ic_t"""
+    _fields_ = [
+        ("kind", _i),
+        ("mach", _d), ("alpha", _d), ("beta", _d), ("gamma", _d), ("density", _d), ("pressure", _d),
+        ("vortex_strength", _d), ("xo", _d), ("yo", _d),
+        ("xs", _d), ("ys", _d), ("state", (_d * 4) * 4),
+        ("dmr_x0", _d), ("dmr_angle", _d), ("dmr_mach", _d), ("dmr_rho", _d), ("dmr_p", _d),
+    ]
+
+
+IC_FREESTREAM, IC_VORTEX, IC_RIEMANN4, IC_DMR = 0, 1, 2, 3
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `make -C amr-demo_amd` "
+                               "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def ipt(a):
+    return _p(a, C.c_int)
+
+
+def dpt(a):
+    return _p(a, C.c_double)
+
+
+def last_error():
+    return lib().amrgpu_last_error().decode()
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+def check(rc, what=""):
+    if rc != 0:
+        raise GpuError(f"{what} failed rc={rc}: {last_error()}")
+
+
+# ---------------------------------------------------------------------------
+# host forest
+# ---------------------------------------------------------------------------
+class Forest:
+    def __init__(self, ntx, nty, periodic, xlo, ylo, scale, n, min_level, max_level):
+        self.L = lib()
+        self.h = self.L.amr_forest_new(ntx, nty, int(periodic[0]), int(periodic[1]), xlo, ylo,
+                                       scale, n, min_level, max_level)
+        if not self.h:
+            raise ValueError("amr_forest_new rejected the arguments")
+        self.n = n
+        self.scale = scale
+        self.ntx, self.nty, self.xlo, self.ylo = ntx, nty, xlo, ylo
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.amr_forest_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @property
+    def nquad(self):
+        return self.L.amr_forest_num_quads(self.h)
+
+    def soln(self):
+        """numpy view of the forest's solution (nquad, n, n, 4) -> reference Q(4,n,n)"""
+        nq, n = self.nquad, self.n
+        buf = np.ctypeslib.as_array(self.L.amr_forest_soln(self.h), shape=(nq * 4 * n * n,))
+        return buf.reshape(nq, n, n, 4)
+
+    def tags(self):
+        return np.ctypeslib.as_array(self.L.amr_forest_tags(self.h), shape=(self.nquad,))
+
+    def quads(self):
+        ptr = C.cast(self.L.amr_forest_quads(self.h), C.POINTER(Quad))
+        return [(ptr[k].tree, ptr[k].level, ptr[k].x, ptr[k].y) for k in range(self.nquad)]
+
+    def levels(self):
+        ptr = C.cast(self.L.amr_forest_quads(self.h), C.POINTER(Quad))
+        return np.array([ptr[k].level for k in range(self.nquad)], dtype=np.int32)
+
+    def external(self):
+        """(cell_info, face_info, geom, soln) copies in the reference format"""
+        ncell, ng, nface, dof = _i(), _i(), _i(), _i()
+        ci, fi, g, s = _pi(), _pi(), _pd(), _pd()
+        rc = self.L.amr_forest_build_external(self.h, C.byref(ncell), C.byref(ng), C.byref(nface),
+                                              C.byref(dof), C.byref(ci), C.byref(fi), C.byref(g),
+                                              C.byref(s))
+        if rc != 0:
+            raise ValueError("forest is not 2:1 balanced / inconsistent")
+        nc, nf, nd = ncell.value, nface.value, dof.value
+        cell_info = np.ctypeslib.as_array(ci, shape=(3 * nc,)).copy()
+        face_info = np.ctypeslib.as_array(fi, shape=(11 * nf,)).copy() if nf else np.zeros(0, np.int32)
+        geom = np.ctypeslib.as_array(g, shape=(12 * nc,)).copy()
+        soln = np.ctypeslib.as_array(s, shape=(nd,)).copy()
+        return cell_info.astype(np.int32), face_info.astype(np.int32), geom, soln
+
+    def set_soln(self, flat):
+        v = self.soln().reshape(-1)
+        v[:] = flat
+
+    def refine(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        return self.L.amr_forest_refine(self.h, ipt(f))
+
+    def coarsen(self):
+        return self.L.amr_forest_coarsen(self.h)
+
+    def balance(self):
+        return self.L.amr_forest_balance(self.h)
+
+    def is_balanced(self):
+        return bool(self.L.amr_forest_is_balanced(self.h))
+
+    def apply_ic(self, ic):
+        self.L.amr_ic_apply(self.h, C.byref(ic))
+
+    def indicator(self):
+        ind = np.zeros(self.nquad)
+        self.L.amr_indicator_gradient(self.h, dpt(ind))
+        return ind
+
+    def regrid(self, tol, allow_coarsening=True):
+        return self.L.amr_regrid(self.h, tol, int(allow_coarsening))
+
+    def regrid_initial(self, ic, tol):
+        return self.L.amr_regrid_initial(self.h, C.byref(ic), tol)
+
+
+def vortex_ic(strength=5.0, xo=5.0, yo=5.0, u=1.0, v=1.0, gamma=1.4, rho=1.0, p=1.0):
+    """isentropic vortex through the reference's free-stream inputs: u = v = 1
+    means |V| = sqrt(2) at 45 degrees, mach = |V| / sqrt(gamma p / rho)"""
+    ic = IC()
+    ic.kind = IC_VORTEX
+    speed = float(np.hypot(u, v))
+    ic.mach = speed / np.sqrt(gamma * p / rho)
+    ic.alpha = float(np.degrees(np.arctan2(v, u)))
+    ic.beta = 0.0
+    ic.gamma, ic.density, ic.pressure = gamma, rho, p
+    ic.vortex_strength, ic.xo, ic.yo = strength, xo, yo
+    return ic
+
+
+def riemann4_ic(seed, xs=0.5, ys=0.5, gamma=1.4):
+    """four-quadrant Riemann states from numpy.random.default_rng(seed):
+    rho log-uniform [0.1, 2], u, v uniform [-1, 1], p uniform [0.1, 2]"""
+    rng = np.random.default_rng(seed)
+    ic = IC()
+    ic.kind = IC_RIEMANN4
+    ic.gamma, ic.density, ic.pressure, ic.mach = gamma, 1.0, 1.0, 0.0
+    ic.xs, ic.ys = xs, ys
+    for k in range(4):
+        rho = float(np.exp(rng.uniform(np.log(0.1), np.log(2.0))))
+        u, v = (float(x) for x in rng.uniform(-1.0, 1.0, size=2))
+        p = float(rng.uniform(0.1, 2.0))
+        ic.state[k][0], ic.state[k][1], ic.state[k][2], ic.state[k][3] = rho, u, v, p
+    return ic
+
+
+def dmr_ic(gamma=1.4):
+    """double-Mach reflection: Mach-10 shock at 60 degrees through (1/6, 0),
+    pre-shock rho = 1.4, p = 1 at rest"""
+    ic = IC()
+    ic.kind = IC_DMR
+    ic.gamma, ic.density, ic.pressure, ic.mach = gamma, 1.4, 1.0, 0.0
+    ic.dmr_x0, ic.dmr_angle, ic.dmr_mach, ic.dmr_rho, ic.dmr_p = 1.0 / 6.0, 60.0, 10.0, 1.4, 1.0
+    return ic
+
+
+# ---------------------------------------------------------------------------
+# device context
+# ---------------------------------------------------------------------------
+def make_inst(h_base, bc=(0, 0, 0, 0), cfl=0.3, dt_cap=1.0, t_end=0.0):
+    ip = InstParams()
+    ip.h_base = h_base
+    for s in range(4):
+        ip.bc[s] = int(bc[s])
+    ip.cfl, ip.dt_cap, ip.t_end = cfl, dt_cap, t_end
+    return ip
+
+
+class Gpu:
+    """amrgpu_ctx"""
+
+    def __init__(self):
+        self.L = lib()
+        self.h = self.L.amrgpu_create()
+        if not self.h:
+            raise GpuError(f"amrgpu_create failed: {last_error()}")
+        self.ninst = 0
+
+    def close(self):
+        if self.h:
+            self.L.amrgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setup(self, n, scheme, insts, cell_info, face_info, geom, soln, cell_inst=None):
+        ninst = len(insts)
+        arr = (InstParams * ninst)(*insts)
+        ci = np.ascontiguousarray(cell_info, dtype=np.int32)
+        fi = np.ascontiguousarray(face_info, dtype=np.int32)
+        g = np.ascontiguousarray(geom, dtype=np.float64)
+        s = np.ascontiguousarray(soln, dtype=np.float64)
+        cinst = None if cell_inst is None else np.ascontiguousarray(cell_inst, dtype=np.int32)
+        ncell = ci.size // 3
+        nface = fi.size // 11
+        rc = self.L.amrgpu_setup(self.h, n, scheme, ninst, C.cast(arr, _vp), ncell, ipt(ci),
+                                 ipt(cinst) if cinst is not None else None, nface,
+                                 ipt(fi) if nface else ipt(np.zeros(11, np.int32)), dpt(g), dpt(s))
+        check(rc, "amrgpu_setup")
+        self.ninst = ninst
+        self.ndof = s.size
+
+    def rhs(self):
+        out = np.empty(self.ndof)
+        check(self.L.amrgpu_rhs(self.h, dpt(out)), "amrgpu_rhs")
+        return out
+
+    def cfl(self):
+        out = np.empty(self.ninst)
+        check(self.L.amrgpu_cfl(self.h, dpt(out)), "amrgpu_cfl")
+        return out
+
+    def run(self, nsteps, graph=True):
+        rn = _d()
+        rc = self.L.amrgpu_run(self.h, nsteps, int(graph), C.byref(rn))
+        if rc == 3:
+            raise FloatingPointError("nan detected in residual")
+        check(rc, "amrgpu_run")
+        return rn.value
+
+    def soln(self):
+        out = np.empty(self.ndof)
+        check(self.L.amrgpu_download_soln(self.h, dpt(out)), "amrgpu_download_soln")
+        return out
+
+    def upload(self, soln):
+        s = np.ascontiguousarray(soln, dtype=np.float64)
+        check(self.L.amrgpu_upload_soln(self.h, dpt(s)), "amrgpu_upload_soln")
+
+    def time(self):
+        t = np.empty(self.ninst)
+        dt = np.empty(self.ninst)
+        check(self.L.amrgpu_get_time(self.h, dpt(t), dpt(dt)), "amrgpu_get_time")
+        return t, dt
+
+    def set_time(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        check(self.L.amrgpu_set_time(self.h, dpt(t)), "amrgpu_set_time")
+
+    def bench(self, nsteps):
+        ms = _d()
+        st = (C.c_double * 8)()
+        ns = _i()
+        check(self.L.amrgpu_bench(self.h, nsteps, C.byref(ms), st, C.byref(ns)), "amrgpu_bench")
+        return ms.value, [st[k] for k in range(ns.value)]
+
+    @property
+    def ncells(self):
+        return self.L.amrgpu_num_cells(self.h)
+
+
+def set_params(**kw):
+    p = PhysicsParams()
+    lib().physics_gpu_get_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "bc":
+            for s in range(6):
+                p.bc[s] = int(v[s]) if s < len(v) else 1
+        else:
+            setattr(p, k, v)
+    lib().physics_gpu_set_params(C.byref(p))
+    return p
+
+
+def get_params():
+    p = PhysicsParams()
+    lib().physics_gpu_get_params(C.byref(p))
+    return p
+
+
+def copy_probe(nbytes, reps=20):
+    g = _d()
+    check(lib().amrgpu_copy_probe(int(nbytes), reps, C.byref(g)), "amrgpu_copy_probe")
+    return g.value

@@ -1,0 +1,251 @@
+/*
+ * amr_driver.c -- initial conditions, density-gradient indicator and the
+ * regrid sequence on the host forest.  See include/amr_driver.h.
+ */
+#include "amr_driver.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static void bilinear(const double g[12], double xi, double eta, double out[2])
+{
+    /* src/solver/src/bilinear_node_2d_module.F90:14-36 */
+    double oneMxi = 0.5 * (1.0 - xi), onePxi = 0.5 * (1.0 + xi);
+    double oneMeta = 0.5 * (1.0 - eta), onePeta = 0.5 * (1.0 + eta);
+    for (int d = 0; d < 2; ++d)
+        out[d] = g[0 + d] * oneMxi * oneMeta + g[3 + d] * onePxi * oneMeta +
+                 g[6 + d] * oneMxi * onePeta + g[9 + d] * onePxi * onePeta;
+}
+
+static void centre(const double g[12], int n, int i, int j, double out[2])
+{
+    /* sub-cell centres as initial_condition_2d_module.F90:100-111 */
+    double sf = 2.0 / (double)n;
+    double v1 = sf * (double)j - 1.0, v2 = sf * (double)(j + 1) - 1.0;
+    double w1 = sf * (double)i - 1.0, w2 = sf * (double)(i + 1) - 1.0;
+    bilinear(g, 0.5 * (w1 + w2), 0.5 * (v1 + v2), out);
+}
+
+static void prim_to_cons(double gamma, double rho, double u, double v, double p, double *q)
+{
+    q[0] = rho;
+    q[1] = rho * u;
+    q[2] = rho * v;
+    q[3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+}
+
+void amr_ic_patch(const amr_ic_t *ic, int n, const double geom[12], double *Q)
+{
+    const double pi = 3.141592653589793238462643383279502884197;
+    const double twopi = 6.283185307179586476925286766559005768394;
+    const double gamma = ic->gamma;
+    const double gm1 = gamma - 1.0, oneOgm1 = 1.0 / gm1;
+    /* free-stream quantities (initial_condition_2d_module.F90:51-67) */
+    double RT0 = ic->pressure / ic->density;
+    double V_0 = ic->mach * sqrt(gamma * RT0);
+    double u_in = V_0 * cos(ic->alpha * pi / 180.0) * cos(ic->beta * pi / 180.0);
+    double v_in = V_0 * sin(ic->alpha * pi / 180.0) * cos(ic->beta * pi / 180.0);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            double *q = &Q[4 * (i + n * j)];
+            double x[2];
+            centre(geom, n, i, j, x);
+            switch (ic->kind) {
+            case AMR_IC_VORTEX: {
+                /* initial_condition_2d_module.F90:83-129, strength/centre as inputs */
+                double a = 0.5, two_a = 2.0 * a, bl = ic->vortex_strength;
+                double dT1 = -gm1 * bl * bl / (16.0 * a * gamma * pi * pi);
+                double r2 = (x[0] - ic->xo) * (x[0] - ic->xo) + (x[1] - ic->yo) * (x[1] - ic->yo);
+                double dT = dT1 * exp(two_a * (1.0 - r2));
+                double rho = pow(1.0 + dT, oneOgm1);
+                double ff = bl / twopi * exp(a * (1.0 - r2));
+                double u = u_in - (x[1] - ic->yo) * ff;
+                double v = v_in + (x[0] - ic->xo) * ff;
+                double p = ic->pressure + pow(rho, gamma) - 1.0;
+                q[0] = rho;
+                q[1] = rho * u;
+                q[2] = rho * v;
+                q[3] = p * oneOgm1 + 0.5 * rho * (u * u + v * v);
+                break;
+            }
+            case AMR_IC_RIEMANN4: {
+                int k = (x[0] > ic->xs ? 1 : 0) + (x[1] > ic->ys ? 2 : 0);
+                const double *s = ic->state[k];
+                prim_to_cons(gamma, s[0], s[1], s[2], s[3], q);
+                break;
+            }
+            case AMR_IC_DMR: {
+                /* shock line through (x0, 0) making angle th with the x axis;
+                 * post-shock state behind it (x < x0 + y / tan th) */
+                double th = ic->dmr_angle * pi / 180.0;
+                double xs = ic->dmr_x0 + x[1] / tan(th);
+                double rho1 = ic->dmr_rho, p1 = ic->dmr_p;
+                double M = ic->dmr_mach;
+                double c1 = sqrt(gamma * p1 / rho1);
+                double w = M * c1;                          /* shock speed */
+                double rho2 = rho1 * (gamma + 1.0) * M * M / ((gamma - 1.0) * M * M + 2.0);
+                double p2 = p1 * (2.0 * gamma * M * M - (gamma - 1.0)) / (gamma + 1.0);
+                double un = w * (1.0 - rho1 / rho2);        /* post-shock normal speed */
+                if (x[0] < xs)
+                    prim_to_cons(gamma, rho2, un * sin(th), -un * cos(th), p2, q);
+                else
+                    prim_to_cons(gamma, rho1, 0.0, 0.0, p1, q);
+                break;
+            }
+            default:
+                q[0] = ic->density;
+                q[1] = ic->density * u_in;
+                q[2] = ic->density * v_in;
+                q[3] = (ic->pressure * oneOgm1) + 0.5 * ic->density * (u_in * u_in + v_in * v_in);
+                break;
+            }
+        }
+}
+
+void amr_ic_apply(amr_forest_t *f, const amr_ic_t *ic)
+{
+    int nq = amr_forest_num_quads(f), n = amr_forest_nelem_subgrid(f);
+    double *soln = amr_forest_soln(f);
+    double g[12];
+    for (int k = 0; k < nq; ++k) {
+        amr_forest_quad_geom(f, k, g);
+        amr_ic_patch(ic, n, g, &soln[(size_t)4 * n * n * k]);
+    }
+}
+
+double amr_ic_vortex_density(const amr_ic_t *ic, double x, double y, double t,
+                             double x0, double Lx, double y0, double Ly)
+{
+    const double pi = 3.141592653589793238462643383279502884197;
+    const double gamma = ic->gamma, gm1 = gamma - 1.0;
+    double RT0 = ic->pressure / ic->density;
+    double V_0 = ic->mach * sqrt(gamma * RT0);
+    double u_in = V_0 * cos(ic->alpha * pi / 180.0) * cos(ic->beta * pi / 180.0);
+    double v_in = V_0 * sin(ic->alpha * pi / 180.0) * cos(ic->beta * pi / 180.0);
+    double cx = ic->xo + u_in * t, cy = ic->yo + v_in * t;
+    double dx = x - cx, dy = y - cy;
+    dx -= Lx * floor((dx - x0 + x0) / Lx + 0.5);
+    dy -= Ly * floor((dy - y0 + y0) / Ly + 0.5);
+    double a = 0.5, bl = ic->vortex_strength;
+    double dT1 = -gm1 * bl * bl / (16.0 * a * gamma * pi * pi);
+    double dT = dT1 * exp(2.0 * a * (1.0 - (dx * dx + dy * dy)));
+    return pow(1.0 + dT, 1.0 / gm1);
+}
+
+static inline double jump(double a, double b)
+{
+    return fabs(a - b) / (0.5 * (a + b));
+}
+
+/* neighbour edge densities of side s of quadrant k, index-aligned */
+static void edge_rho(amr_forest_t *f, const int *fi_side, const double *soln, int n, int s,
+                     int nb, double *out)
+{
+    (void)fi_side;
+    const double *Q = &soln[(size_t)4 * n * n * nb];
+    for (int t = 0; t < n; ++t) {
+        int i, j;
+        switch (s) {
+        case 0: i = n - 1; j = t; break;
+        case 1: i = 0; j = t; break;
+        case 2: i = t; j = n - 1; break;
+        default: i = t; j = 0; break;
+        }
+        out[t] = Q[4 * (i + n * j)];
+    }
+    (void)f;
+}
+
+void amr_indicator_gradient(amr_forest_t *f, double *ind)
+{
+    int ncell, nghost, nface, dof;
+    int *cell_info, *face_info;
+    double *geom, *soln;
+    amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom, &soln);
+    int n = amr_forest_nelem_subgrid(f);
+    double *e1 = (double *)malloc(sizeof(double) * n);
+    /* interior pairs */
+    for (int k = 0; k < ncell; ++k) {
+        const double *Q = &soln[(size_t)4 * n * n * k];
+        double m = 0.0;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                double r = Q[4 * (i + n * j)];
+                if (i + 1 < n) { double v = jump(r, Q[4 * (i + 1 + n * j)]); if (v > m) m = v; }
+                if (j + 1 < n) { double v = jump(r, Q[4 * (i + n * (j + 1))]); if (v > m) m = v; }
+            }
+        ind[k] = m;
+    }
+    /* across faces: both sides see the same pairs */
+    for (int fc = 0; fc < nface; ++fc) {
+        const int *fi = &face_info[11 * fc];
+        if (fi[0] == 1) continue;
+        int s = fi[1], a = fi[2];
+        const double *Qa = &soln[(size_t)4 * n * n * a];
+        int nbs[2] = {fi[3], fi[0] == 3 ? fi[4] : -1};
+        for (int h = 0; h < 2; ++h) {
+            int b = nbs[h];
+            if (b < 0) continue;
+            edge_rho(f, fi, soln, n, s, b, e1);
+            double m = 0.0;
+            for (int t = 0; t < n; ++t) {
+                int i, j;
+                switch (s) {
+                case 0: i = 0; j = t; break;
+                case 1: i = n - 1; j = t; break;
+                case 2: i = t; j = 0; break;
+                default: i = t; j = n - 1; break;
+                }
+                double v = jump(Qa[4 * (i + n * j)], e1[t]);
+                if (v > m) m = v;
+            }
+            if (m > ind[a]) ind[a] = m;
+            if (m > ind[b]) ind[b] = m;
+        }
+    }
+    free(e1);
+}
+
+int amr_regrid(amr_forest_t *f, double tol, int allow_coarsening)
+{
+    int nq = amr_forest_num_quads(f);
+    double *ind = (double *)malloc(sizeof(double) * (size_t)(nq > 0 ? nq : 1));
+    int *tag = amr_forest_tags(f);
+    amr_indicator_gradient(f, ind);
+    /* tag_reset + regrid_feature */
+    for (int k = 0; k < nq; ++k) tag[k] = ind[k] > tol ? 1 : 0;
+    int *flags = (int *)malloc(sizeof(int) * (size_t)(nq > 0 ? nq : 1));
+    memcpy(flags, tag, sizeof(int) * (size_t)nq);
+    amr_forest_refine(f, flags);
+    free(flags);
+    free(ind);
+    if (allow_coarsening) amr_forest_coarsen(f);
+    amr_forest_balance(f);
+    return amr_forest_num_quads(f);
+}
+
+int amr_regrid_initial(amr_forest_t *f, const amr_ic_t *ic, double tol)
+{
+    int n = amr_forest_nelem_subgrid(f);
+    for (int pass = 0; pass < AMR_MAXLEVEL; ++pass) {
+        int nq = amr_forest_num_quads(f);
+        double *ind = (double *)malloc(sizeof(double) * (size_t)nq);
+        int *flags = (int *)malloc(sizeof(int) * (size_t)nq);
+        amr_indicator_gradient(f, ind);
+        for (int k = 0; k < nq; ++k) flags[k] = ind[k] > tol;
+        int r = amr_forest_refine(f, flags);
+        free(ind);
+        free(flags);
+        if (!r) break;
+        /* initialize_quadrant_data on every quadrant (new children get the IC) */
+        amr_ic_apply(f, ic);
+    }
+    amr_forest_balance(f);
+    amr_ic_apply(f, ic);
+    int nq = amr_forest_num_quads(f);
+    int *tag = amr_forest_tags(f);
+    for (int k = 0; k < nq; ++k) tag[k] = 0;
+    (void)n;
+    return nq;
+}

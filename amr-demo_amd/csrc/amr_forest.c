@@ -1,0 +1,407 @@
+/*
+ * amr_forest.c -- host quadtree forest (p4est stand-in) and the AMR->physics
+ * boundary builder.  See include/amr_forest.h.
+ *
+ * Quadrants live in one array sorted by (tree, Morton key of (x, y)); a
+ * complete linear quadtree per tree, like p4est's local quadrant arrays.
+ * Neighbour queries are binary searches for the leaf containing a point.
+ */
+#include "amr_forest.h"
+
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct amr_forest {
+    int ntx, nty, periodic[2];
+    double xlo, ylo, scale;
+    int n, min_level, max_level;
+    int nquad, cap;
+    amr_quad_t *q;
+    uint64_t *key;        /* Morton key per quadrant (lower-left corner) */
+    double *soln;         /* nquad * 4*n*n */
+    int *tag, *type;
+    /* boundary arrays (rebuilt by amr_forest_build_external) */
+    int *cell_info, *face_info;
+    double *geom;
+    int nface;
+};
+
+static inline int qlen(int level) { return 1 << (AMR_MAXLEVEL - level); }
+
+static inline uint64_t spread(uint32_t v)
+{
+    uint64_t x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+static inline uint64_t morton(int x, int y) { return spread((uint32_t)x) | (spread((uint32_t)y) << 1); }
+
+static int patch_size(const amr_forest_t *f) { return 4 * f->n * f->n; }
+
+static void reserve(amr_forest_t *f, int cap)
+{
+    if (cap <= f->cap) return;
+    int ps = patch_size(f);
+    f->q = (amr_quad_t *)realloc(f->q, sizeof(amr_quad_t) * (size_t)cap);
+    f->key = (uint64_t *)realloc(f->key, sizeof(uint64_t) * (size_t)cap);
+    f->soln = (double *)realloc(f->soln, sizeof(double) * (size_t)ps * (size_t)cap);
+    f->tag = (int *)realloc(f->tag, sizeof(int) * (size_t)cap);
+    f->type = (int *)realloc(f->type, sizeof(int) * (size_t)cap);
+    f->cap = cap;
+}
+
+amr_forest_t *amr_forest_new(int ntx, int nty, int periodic_x, int periodic_y,
+                             double xlo, double ylo, double scale, int n,
+                             int min_level, int max_level)
+{
+    if (ntx < 1 || nty < 1 || n < 1 || min_level < 0 || max_level < min_level ||
+        max_level > AMR_MAXLEVEL - 1)
+        return NULL;
+    amr_forest_t *f = (amr_forest_t *)calloc(1, sizeof(amr_forest_t));
+    f->ntx = ntx; f->nty = nty;
+    f->periodic[0] = periodic_x; f->periodic[1] = periodic_y;
+    f->xlo = xlo; f->ylo = ylo; f->scale = scale;
+    f->n = n; f->min_level = min_level; f->max_level = max_level;
+    long long per_tree = 1LL << (2 * min_level);
+    long long total = per_tree * ntx * nty;
+    if (total > (1LL << 30)) { free(f); return NULL; }
+    reserve(f, (int)total);
+    int k = 0, m = 1 << min_level, len = qlen(min_level);
+    for (int t = 0; t < ntx * nty; ++t) {
+        /* Morton order within the tree */
+        for (long long z = 0; z < per_tree; ++z) {
+            int x = 0, y = 0;
+            for (int b = 0; b < min_level; ++b) {
+                x |= (int)((z >> (2 * b)) & 1) << b;
+                y |= (int)((z >> (2 * b + 1)) & 1) << b;
+            }
+            (void)m;
+            f->q[k].tree = t; f->q[k].level = min_level;
+            f->q[k].x = x * len; f->q[k].y = y * len;
+            f->key[k] = morton(f->q[k].x, f->q[k].y);
+            f->tag[k] = 0; f->type[k] = 0;
+            ++k;
+        }
+    }
+    f->nquad = k;
+    memset(f->soln, 0, sizeof(double) * (size_t)patch_size(f) * (size_t)k);
+    return f;
+}
+
+void amr_forest_destroy(amr_forest_t *f)
+{
+    if (!f) return;
+    free(f->q); free(f->key); free(f->soln); free(f->tag); free(f->type);
+    free(f->cell_info); free(f->face_info); free(f->geom);
+    free(f);
+}
+
+int amr_forest_num_quads(const amr_forest_t *f) { return f->nquad; }
+int amr_forest_nelem_subgrid(const amr_forest_t *f) { return f->n; }
+const amr_quad_t *amr_forest_quads(const amr_forest_t *f) { return f->q; }
+double *amr_forest_soln(amr_forest_t *f) { return f->soln; }
+int *amr_forest_tags(amr_forest_t *f) { return f->tag; }
+double amr_forest_scale(const amr_forest_t *f) { return f->scale; }
+
+/* p4est_utilities_quad_coordinates (src/amr/src/amr_p4est_utilities.c:15-81)
+ * on brick vertices scaled as initialize_grid does
+ * (src/amr/src/amr_initialize.c:299-303). */
+void amr_forest_quad_geom(const amr_forest_t *f, int k, double xyz[12])
+{
+    const amr_quad_t *q = &f->q[k];
+    int tx = q->tree % f->ntx, ty = q->tree / f->ntx;
+    double v[4][3];
+    for (int c = 0; c < 4; ++c) {
+        int ix = tx + (c & 1), iy = ty + (c >> 1);
+        v[c][0] = (double)ix * f->scale + f->xlo;
+        v[c][1] = (double)iy * f->scale + f->ylo;
+        v[c][2] = 0.0 * f->scale + 0.0;
+    }
+    const double intsize = 1.0 / (double)AMR_ROOT_LEN;
+    double h2 = 0.5 * intsize * (double)qlen(q->level);
+    double eta_z = 0.0;
+    int kk = 0;
+    for (int yi = 0; yi < 2; ++yi) {
+        double eta_y = intsize * (double)q->y + h2 * (1.0 + (2 * yi - 1));
+        for (int xi = 0; xi < 2; ++xi) {
+            double eta_x = intsize * (double)q->x + h2 * (1.0 + (2 * xi - 1));
+            for (int j = 0; j < 3; ++j) {
+                xyz[3 * kk + j] =
+                    ((1. - eta_z) * ((1. - eta_y) * ((1. - eta_x) * v[0][j] + eta_x * v[1][j]) +
+                                     eta_y * ((1. - eta_x) * v[2][j] + eta_x * v[3][j])));
+            }
+            ++kk;
+        }
+    }
+}
+
+static int cmp_pos(const amr_forest_t *f, int k, int tree, uint64_t key)
+{
+    if (f->q[k].tree != tree) return f->q[k].tree < tree ? -1 : 1;
+    if (f->key[k] != key) return f->key[k] < key ? -1 : 1;
+    return 0;
+}
+
+int amr_forest_find_leaf(const amr_forest_t *f, int tree, int x, int y)
+{
+    uint64_t key = morton(x, y);
+    int lo = 0, hi = f->nquad - 1, best = -1;
+    while (lo <= hi) {
+        int mid = lo + (hi - lo) / 2;
+        if (cmp_pos(f, mid, tree, key) <= 0) { best = mid; lo = mid + 1; }
+        else hi = mid - 1;
+    }
+    if (best < 0) return -1;
+    const amr_quad_t *q = &f->q[best];
+    int len = qlen(q->level);
+    if (q->tree != tree || x < q->x || y < q->y || (long long)x >= (long long)q->x + len ||
+        (long long)y >= (long long)q->y + len)
+        return -1;
+    return best;
+}
+
+/* Same-size neighbour region of quadrant q in direction (dx, dy); returns 0
+ * if it lies outside a non-periodic domain. */
+static int neighbour_region(const amr_forest_t *f, const amr_quad_t *q, int dx, int dy,
+                            int *tree, int *nx, int *ny)
+{
+    long long len = qlen(q->level);
+    long long x = q->x + dx * len, y = q->y + dy * len;
+    int tx = q->tree % f->ntx, ty = q->tree / f->ntx;
+    if (x < 0) { x += AMR_ROOT_LEN; tx -= 1; }
+    else if (x >= AMR_ROOT_LEN) { x -= AMR_ROOT_LEN; tx += 1; }
+    if (y < 0) { y += AMR_ROOT_LEN; ty -= 1; }
+    else if (y >= AMR_ROOT_LEN) { y -= AMR_ROOT_LEN; ty += 1; }
+    if (tx < 0 || tx >= f->ntx) {
+        if (!f->periodic[0]) return 0;
+        tx = (tx + f->ntx) % f->ntx;
+    }
+    if (ty < 0 || ty >= f->nty) {
+        if (!f->periodic[1]) return 0;
+        ty = (ty + f->nty) % f->nty;
+    }
+    *tree = ty * f->ntx + tx;
+    *nx = (int)x; *ny = (int)y;
+    return 1;
+}
+
+static const int SIDE_DX[4] = {-1, 1, 0, 0};
+static const int SIDE_DY[4] = {0, 0, -1, 1};
+
+/* Classify side s of quadrant k: returns 0 boundary, 1 same-size neighbour
+ * (nb[0]), 2 coarser neighbour (nb[0]), 3 two finer neighbours (nb[0], nb[1])
+ * in order of increasing coordinate along the face, -1 on an inconsistency. */
+static int classify_side(const amr_forest_t *f, int k, int s, int nb[2])
+{
+    const amr_quad_t *q = &f->q[k];
+    int t, nx, ny;
+    if (!neighbour_region(f, q, SIDE_DX[s], SIDE_DY[s], &t, &nx, &ny)) return 0;
+    int L = amr_forest_find_leaf(f, t, nx, ny);
+    if (L < 0) return -1;
+    if (f->q[L].level == q->level) { nb[0] = L; return 1; }
+    if (f->q[L].level < q->level) { nb[0] = L; return 2; }
+    int h = qlen(q->level) / 2;
+    int ax = nx, ay = ny, bx = nx, by = ny;
+    switch (s) {
+    case 0: ax = nx + h; bx = nx + h; by = ny + h; break;  /* neighbour's xhi children */
+    case 1: by = ny + h; break;                             /* neighbour's xlo children */
+    case 2: ay = ny + h; by = ny + h; bx = nx + h; break;   /* neighbour's yhi children */
+    case 3: bx = nx + h; break;                             /* neighbour's ylo children */
+    }
+    nb[0] = amr_forest_find_leaf(f, t, ax, ay);
+    nb[1] = amr_forest_find_leaf(f, t, bx, by);
+    if (nb[0] < 0 || nb[1] < 0) return -1;
+    if (f->q[nb[0]].level != q->level + 1 || f->q[nb[1]].level != q->level + 1) return -1;
+    return 3;
+}
+
+int amr_forest_build_external(amr_forest_t *f, int *ncell, int *nghost, int *nface, int *dof,
+                              int **cell_info, int **face_info, double **geom, double **soln)
+{
+    int nq = f->nquad, ps = patch_size(f);
+    free(f->cell_info); free(f->face_info); free(f->geom);
+    f->cell_info = (int *)malloc(sizeof(int) * 3 * (size_t)nq);
+    f->face_info = (int *)malloc(sizeof(int) * 11 * 4 * (size_t)nq);
+    f->geom = (double *)malloc(sizeof(double) * 12 * (size_t)nq);
+    for (int k = 0; k < nq; ++k) {
+        f->cell_info[3 * k + 0] = f->type[k];
+        f->cell_info[3 * k + 1] = f->q[k].level;
+        f->cell_info[3 * k + 2] = ps * k;
+        amr_forest_quad_geom(f, k, &f->geom[12 * k]);
+    }
+    int nf = 0, rc = 0;
+    for (int k = 0; k < nq; ++k) {
+        for (int s = 0; s < 4; ++s) {
+            int nb[2] = {-1, -1};
+            int c = classify_side(f, k, s, nb);
+            int *fi = &f->face_info[11 * nf];
+            if (c < 0) { rc = -1; continue; }
+            if (c == 2) continue;                      /* emitted by the coarse side */
+            if (c == 1 && (s % 2) == 0) continue;      /* emitted from the odd side */
+            for (int e = 0; e < 11; ++e) fi[e] = -1;
+            fi[1] = s;
+            fi[2] = k;
+            if (c == 0) {
+                fi[0] = 1;
+            } else if (c == 1) {
+                fi[0] = 2;
+                fi[3] = nb[0]; fi[7] = s ^ 1;
+            } else {
+                fi[0] = 3;
+                fi[3] = nb[0]; fi[4] = nb[1];
+                fi[7] = s ^ 1; fi[8] = s ^ 1;
+            }
+            ++nf;
+        }
+    }
+    f->nface = nf;
+    *ncell = nq; *nghost = 0; *nface = nf; *dof = ps * nq;
+    *cell_info = f->cell_info; *face_info = f->face_info; *geom = f->geom; *soln = f->soln;
+    return rc;
+}
+
+/* Replace the quadrant array by applying refine flags (one level). */
+int amr_forest_refine(amr_forest_t *f, const int *flags)
+{
+    int nq = f->nquad, ps = patch_size(f), cnt = 0;
+    for (int k = 0; k < nq; ++k) if (flags[k] && f->q[k].level < f->max_level) ++cnt;
+    if (!cnt) return 0;
+    int nn = nq + 3 * cnt;
+    amr_quad_t *q = (amr_quad_t *)malloc(sizeof(amr_quad_t) * (size_t)nn);
+    uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)nn);
+    double *soln = (double *)malloc(sizeof(double) * (size_t)ps * (size_t)nn);
+    int *tag = (int *)malloc(sizeof(int) * (size_t)nn);
+    int *type = (int *)malloc(sizeof(int) * (size_t)nn);
+    int j = 0;
+    for (int k = 0; k < nq; ++k) {
+        if (flags[k] && f->q[k].level < f->max_level) {
+            int h = qlen(f->q[k].level + 1);
+            for (int c = 0; c < 4; ++c) {
+                q[j].tree = f->q[k].tree;
+                q[j].level = f->q[k].level + 1;
+                q[j].x = f->q[k].x + (c & 1) * h;
+                q[j].y = f->q[k].y + (c >> 1) * h;
+                key[j] = morton(q[j].x, q[j].y);
+                /* solver_refine_operator_2d.F90:49-52: children = parent */
+                memcpy(&soln[(size_t)ps * j], &f->soln[(size_t)ps * k], sizeof(double) * ps);
+                tag[j] = f->tag[k];
+                type[j] = f->type[k];
+                ++j;
+            }
+        } else {
+            q[j] = f->q[k]; key[j] = f->key[k];
+            memcpy(&soln[(size_t)ps * j], &f->soln[(size_t)ps * k], sizeof(double) * ps);
+            tag[j] = f->tag[k]; type[j] = f->type[k];
+            ++j;
+        }
+    }
+    free(f->q); free(f->key); free(f->soln); free(f->tag); free(f->type);
+    f->q = q; f->key = key; f->soln = soln; f->tag = tag; f->type = type;
+    f->nquad = nn; f->cap = nn;
+    return cnt;
+}
+
+static int is_family(const amr_forest_t *f, int k)
+{
+    if (k + 3 >= f->nquad) return 0;
+    const amr_quad_t *a = &f->q[k];
+    if (a->level == 0) return 0;
+    int h = qlen(a->level);
+    if ((a->x % (2 * h)) || (a->y % (2 * h))) return 0;
+    for (int c = 1; c < 4; ++c) {
+        const amr_quad_t *b = &f->q[k + c];
+        if (b->tree != a->tree || b->level != a->level) return 0;
+        if (b->x != a->x + (c & 1) * h || b->y != a->y + (c >> 1) * h) return 0;
+    }
+    return 1;
+}
+
+int amr_forest_coarsen(amr_forest_t *f)
+{
+    int nq = f->nquad, ps = patch_size(f), j = 0, cnt = 0;
+    for (int k = 0; k < nq;) {
+        int fam = is_family(f, k) && f->q[k].level > f->min_level;
+        if (fam) {
+            for (int c = 0; c < 4; ++c) if (f->tag[k + c] > 0) fam = 0;
+        }
+        if (fam) {
+            amr_quad_t p = f->q[k];
+            int h = qlen(p.level - 1);
+            p.level -= 1;
+            p.x = (p.x / h) * h; p.y = (p.y / h) * h;
+            double *dst = &f->soln[(size_t)ps * j];
+            const double *c1 = &f->soln[(size_t)ps * k], *c2 = c1 + ps, *c3 = c2 + ps, *c4 = c3 + ps;
+            /* solver_coarsen_operator_2d.F90:49-50 */
+            for (int e = 0; e < ps; ++e) {
+                double s = c1[e] + c2[e] + c3[e] + c4[e];
+                dst[e] = 0.25 * s;
+            }
+            f->q[j] = p; f->key[j] = morton(p.x, p.y);
+            f->tag[j] = 0;
+            f->type[j] = f->type[k + 3];  /* last child's type (amr_regrid.c:264-268) */
+            ++j; k += 4; ++cnt;
+        } else {
+            if (j != k) {
+                f->q[j] = f->q[k]; f->key[j] = f->key[k];
+                memmove(&f->soln[(size_t)ps * j], &f->soln[(size_t)ps * k], sizeof(double) * ps);
+                f->tag[j] = f->tag[k]; f->type[j] = f->type[k];
+            }
+            ++j; ++k;
+        }
+    }
+    f->nquad = j;
+    return cnt;
+}
+
+static int balance_flags(const amr_forest_t *f, int *flags)
+{
+    int any = 0;
+    memset(flags, 0, sizeof(int) * (size_t)f->nquad);
+    for (int k = 0; k < f->nquad; ++k) {
+        const amr_quad_t *q = &f->q[k];
+        if (q->level < 2) continue;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (!dx && !dy) continue;
+                int t, nx, ny;
+                if (!neighbour_region(f, q, dx, dy, &t, &nx, &ny)) continue;
+                int L = amr_forest_find_leaf(f, t, nx, ny);
+                if (L >= 0 && f->q[L].level < q->level - 1) { flags[L] = 1; any = 1; }
+            }
+    }
+    return any;
+}
+
+int amr_forest_balance(amr_forest_t *f)
+{
+    int total = 0;
+    for (;;) {
+        int *flags = (int *)malloc(sizeof(int) * (size_t)(f->nquad > 0 ? f->nquad : 1));
+        int any = balance_flags(f, flags);
+        if (any) {
+            /* balance may refine past max_level only if the forest already
+             * violated it; refine ignores max_level here */
+            int save = f->max_level;
+            f->max_level = AMR_MAXLEVEL - 1;
+            total += amr_forest_refine(f, flags);
+            f->max_level = save;
+        }
+        free(flags);
+        if (!any) break;
+    }
+    return total;
+}
+
+int amr_forest_is_balanced(const amr_forest_t *f)
+{
+    int *flags = (int *)malloc(sizeof(int) * (size_t)(f->nquad > 0 ? f->nquad : 1));
+    int any = balance_flags(f, flags);
+    free(flags);
+    return !any;
+}

@@ -1,0 +1,1275 @@
+// amr_gpu.hip -- CDNA4 (gfx950) kernels and the device-resident context for the
+// finite-volume Euler hot path of amr-demo: per-quadrant sub-grid patch
+// residual (first-order faces, HLLE-type "Roe" flux, conforming and 2:1
+// hanging faces, wall/copy boundaries), the RK stage update and the CFL
+// time-step reduction, fused into one kernel per RK stage.
+//
+// Reference arithmetic: src/solver/src/flux_2d_module.F90:63-182 (flux),
+// bc_2d_module.F90:14-75 + flux_2d_module.F90:13-60 (boundary flux),
+// solver_residual_subgrid_2d_module.F90:32-164 (interior faces, /V),
+// solver_residual_face_{full_structured_2d_module,full_hang_structured_2d,
+// hang_full_structured_2d}.F90 (patch faces), solver_time_step_2d_module.F90
+// :26-122 (CFL), src/physics/src/rk_time_integrators.c (RK schemes).
+//
+// HBM layout: state U[quad][field][j][i] (FP64 struct-of-arrays per patch,
+// 4*n*n doubles per quadrant, patches in Morton order); per-quadrant face
+// table nbr[quad][side][2]; per-quadrant level and instance id.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "amr_hip.h"
+
+namespace {
+
+constexpr int NFLD = 4;
+constexpr unsigned long long DBL_INF_BITS = 0x7ff0000000000000ull;
+
+struct InstDev {
+    double h_base;
+    int bc[4];
+    double cfl;
+    double dt_cap;
+    double t_end;
+};
+
+// Per-quadrant face table entry for side s (0 xlo, 1 xhi, 2 ylo, 3 yhi):
+//   a <  0          physical boundary (bc type from the instance)
+//   a >= 0, b < 0   one neighbour a whose opposite edge couples index-aligned
+//                   (same level, or a coarser quadrant across a hanging face)
+//   a >= 0, b >= 0  two finer neighbours (a, b) in hanging order
+struct MeshDev {
+    int nquad;
+    int ninst;
+    const int *nbr;      // nquad*8
+    const int *level;    // nquad
+    const int *inst;     // nquad
+    const InstDev *ip;   // ninst
+};
+
+enum { MODE_SO = 0, MODE_RK4_ACC = 1, MODE_RK4_FINAL = 2, MODE_LSERK = 3, MODE_RHS = 4 };
+enum { ROLE_DT = 1, ROLE_TIME = 2, ROLE_CFL = 4 };
+
+struct StageArgs {
+    MeshDev m;
+    const double *Us;    // state the residual is evaluated on
+    const double *U0;    // base state of the step (may alias Uout pointwise)
+    double *Uout;
+    double *Acc;         // RK4 accumulator / LSERK register
+    double c0, c1, c2;
+    int mode;
+    int first;           // first stage of an accumulating scheme
+    int role;
+    int use_cfl;         // stage-1 dt includes the CFL slot
+    double *dt_cur;
+    double *t_inst;
+    unsigned long long *slot;
+    double *norm_part;
+};
+
+__device__ __forceinline__ double bits2d(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ unsigned long long d2bits(double d) { return (unsigned long long)__double_as_longlong(d); }
+
+// Logical block index with each XCD receiving a contiguous range, so that
+// Morton-adjacent patches (which share halo cells) are processed by the same
+// XCD and its L2.  Workgroups are dispatched round-robin over the 8 XCDs.
+__device__ __forceinline__ int xcd_block(int b, int nblk)
+{
+    const int NX = 8;
+    int q = nblk / NX, r = nblk % NX;
+    int x = b % NX, k = b / NX;
+    return x * q + (x < r ? x : r) + k;
+}
+
+struct Prim {
+    double r, ru, rv, rE, u, v, p, H, sr;
+};
+
+__device__ __forceinline__ void make_prim(double r, double ru, double rv, double rE, Prim &P)
+{
+    // flux_2d_module.F90:93-99
+    double oneOr = 1.0 / r;
+    P.r = r; P.ru = ru; P.rv = rv; P.rE = rE;
+    P.u = ru * oneOr;
+    P.v = rv * oneOr;
+    double E = rE * oneOr;
+    P.p = (1.4 - 1.0) * (rE - 0.5 * r * (P.u * P.u + P.v * P.v));
+    P.H = E + P.p * oneOr;
+    P.sr = sqrt(r);
+}
+
+// HLLE-type flux of flux_2d_module.F90:63-182 on an axis-aligned face of
+// length h with unit normal +x (XDIR) or +y.  sqrt(rho_l*rho_r) is taken as
+// sqrt(rho_l)*sqrt(rho_r) and divisions by abar, (b1-b2) as products with
+// one reciprocal each: both stay within a few ulp of the reference.
+template <bool XDIR>
+__device__ __forceinline__ void flux_face(const Prim &L, const Prim &R, double h, double F[4])
+{
+    double unl = XDIR ? L.u : L.v;
+    double unr = XDIR ? R.u : R.v;
+    double inv_rf = 1.0 / (L.sr + R.sr);
+    double rhobar = L.sr * R.sr;
+    double ubar = (L.u * L.sr + R.u * R.sr) * inv_rf;
+    double vbar = (L.v * L.sr + R.v * R.sr) * inv_rf;
+    double Hbar = (L.H * L.sr + R.H * R.sr) * inv_rf;
+    double abar = sqrt(0.4 * (Hbar - 0.5 * (ubar * ubar + vbar * vbar)));
+    double unbar = XDIR ? ubar : vbar;
+    double inva = 1.0 / abar;
+    double Mn = unbar * inva;
+    double eigp = unbar + abar, eigm = unbar - abar;
+    double eigp_max = unr + abar, eigm_min = unl - abar;
+    double b1 = (eigp > 0.0 || eigp_max > 0.0) ? fmax(eigp, eigp_max) : 0.0;
+    double b2 = (eigm < 0.0 || eigm_min < 0.0) ? fmin(eigm, eigm_min) : 0.0;
+    double invb = 1.0 / (b1 - b2);
+    double bcoeff = b1 * b2 * invb;
+    double delta1 = (R.r - L.r) - (R.p - L.p) * (inva * inva);
+    double Mc = 1.0 / (1.0 + fabs(Mn));
+    double bM = bcoeff * Mc;
+    double dH = R.H - L.H;
+    double fl0, fr0, fl1, fr1, fl2, fr2, d1, d2;
+    if (XDIR) {
+        fl0 = L.ru;               fr0 = R.ru;
+        fl1 = L.ru * L.u + L.p;   fr1 = R.ru * R.u + R.p;
+        fl2 = L.ru * L.v;         fr2 = R.ru * R.v;
+        d1 = delta1 * ubar;                          // shear term vanishes for n = +x
+        d2 = delta1 * vbar + rhobar * (R.v - L.v);
+    } else {
+        fl0 = L.rv;               fr0 = R.rv;
+        fl1 = L.rv * L.u;         fr1 = R.rv * R.u;
+        fl2 = L.rv * L.v + L.p;   fr2 = R.rv * R.v + R.p;
+        d1 = delta1 * ubar + rhobar * (R.u - L.u);
+        d2 = delta1 * vbar;
+    }
+    double fl3 = (L.rE + L.p) * unl, fr3 = (R.rE + R.p) * unr;
+    F[0] = ((b1 * fl0 - b2 * fr0) * invb + (bcoeff * (R.r - L.r) - bM * delta1)) * h;
+    F[1] = ((b1 * fl1 - b2 * fr1) * invb + (bcoeff * (R.ru - L.ru) - bM * d1)) * h;
+    F[2] = ((b1 * fl2 - b2 * fr2) * invb + (bcoeff * (R.rv - L.rv) - bM * d2)) * h;
+    F[3] = ((b1 * fl3 - b2 * fr3) * invb +
+            (bcoeff * (R.r * R.H - L.r * L.H) - bM * (delta1 * Hbar + rhobar * dH))) * h;
+}
+
+// Physical boundary flux: bc_2d (bc_2d_module.F90:14-75) then flux_bc_2d
+// (flux_2d_module.F90:13-60) with outward normal nrm = (+-h, 0) or (0, +-h),
+// operation for operation.
+__device__ __forceinline__ void flux_boundary(int bc, const double nrm[2], const double q[4], double F[4])
+{
+    double rho = q[0];
+    double oneOrho = 1.0 / rho;
+    double u = q[1] * oneOrho;
+    double v = q[2] * oneOrho;
+    double qb[4];
+    if (bc == 1) {
+        double mag = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1]);
+        double nx = nrm[0] / mag, ny = nrm[1] / mag;
+        double Udn = u * nx + v * ny;
+        double ub = u - Udn * nx, vb = v - Udn * ny;
+        qb[0] = q[0]; qb[1] = q[0] * ub; qb[2] = q[0] * vb; qb[3] = q[3];
+    } else if (bc == 2) {
+        qb[0] = q[0]; qb[1] = 0.0; qb[2] = 0.0;
+        qb[3] = q[3] - 0.5 * rho * (u * u + v * v);
+    } else {
+        qb[0] = q[0]; qb[1] = q[1]; qb[2] = q[2]; qb[3] = q[3];
+    }
+    double r = qb[0];
+    double oneOr = 1.0 / r;
+    double ub = qb[1] * oneOr, vb = qb[2] * oneOr;
+    double p = (1.4 - 1.0) * (qb[3] - 0.5 * r * (ub * ub + vb * vb));
+    double fx[4] = {qb[1], qb[1] * ub + p, qb[1] * vb, ub * (qb[3] + p)};
+    double fy[4] = {qb[2], qb[2] * ub, qb[2] * vb + p, vb * (qb[3] + p)};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) F[k] = fx[k] * nrm[0] + fy[k] * nrm[1];
+}
+
+// Per-cell CFL candidate, solver_time_step_2d_module.F90:107-116, evaluated
+// without FMA contraction so that it reproduces the reference bit for bit.
+__device__ __attribute__((noinline)) double cfl_candidate(const double q[4], double cfl, double vol)
+{
+#pragma clang fp contract(off)
+    double oneOrho = 1.0 / q[0];
+    double u = q[1] * oneOrho;
+    double v = q[2] * oneOrho;
+    double V2 = u * u + v * v;
+    double pOverRho = 0.4 * (q[3] * oneOrho - 0.5 * V2);
+    double c = sqrt(1.4 * pOverRho);
+    double eig = sqrt(V2) + c;
+    return cfl * sqrt(vol) / eig;
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide min of the CFL candidates, then one 64-bit atomicMin per
+// instance (positive doubles order like their bit patterns).  Blocks that
+// straddle two instances fall back to per-thread atomics.
+template <int BLK>
+__device__ void cfl_reduce(double cand, int inst, bool active, unsigned long long *slot,
+                           double *red, int *flag)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) flag[0] = -1;
+    __syncthreads();
+    if (active) {
+        int cur = atomicCAS(&flag[0], -1, inst);
+        if (cur != -1 && cur != inst) flag[0] = -2;
+    }
+    __syncthreads();
+    int f = flag[0];
+    if (f == -2) {
+        if (active) atomicMin(&slot[inst], d2bits(cand));
+        return;
+    }
+    double v = active ? cand : INFINITY;
+    v = wave_min(v);
+    constexpr int NW = (BLK + 63) / 64;
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0 && f >= 0) {
+        double m = red[0];
+        for (int w = 1; w < NW; ++w) m = fmin(m, red[w]);
+        atomicMin(&slot[f], d2bits(m));
+    }
+}
+
+template <int BLK>
+__device__ void sum_reduce_store(double v, double *out, double *red)
+{
+    const int tid = threadIdx.x;
+    v = wave_sum(v);
+    constexpr int NW = (BLK + 63) / 64;
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        double s = red[0];
+        for (int w = 1; w < NW; ++w) s += red[w];
+        *out = s;
+    }
+}
+
+__device__ __forceinline__ double stage_dt(const StageArgs &a, int inst)
+{
+    const InstDev &ip = a.m.ip[inst];
+    double dt = ip.dt_cap;
+    if (a.use_cfl) {
+        double c = bits2d(a.slot[inst]);
+        if (c < dt) dt = c;
+    }
+    if (ip.t_end > 0.0) {
+        double rem = ip.t_end - a.t_inst[inst];
+        if (rem < 0.0) rem = 0.0;
+        if (rem < dt) dt = rem;
+    }
+    return dt;
+}
+
+// ---------------------------------------------------------------------------
+// Fused stage kernel: one workgroup = PPB patches of N x N cells, one thread
+// per cell.  LDS holds the patches' primitive variables with a one-cell ghost
+// ring gathered from the neighbours (two rings' worth on coarse sides of
+// hanging faces); every face flux of the patch is evaluated once per patch
+// into registers, staged to LDS and gathered per cell; the epilogue divides
+// by the volume, applies the RK combination, writes the new state
+// (coalesced) and, on the last stage, folds the CFL minimum and the residual
+// sum of squares into block reductions.
+// ---------------------------------------------------------------------------
+template <int N, int PPB>
+struct Tile {
+    static constexpr int N2 = N * N;
+    static constexpr int BLK = N2 * PPB;
+    static constexpr int T = N + 2;
+    static constexpr int TT = T * T;
+    static constexpr int NP = 9;                           // Prim fields
+    static constexpr int PRIM_PER_PATCH = NP * TT + 4 * N * NP;
+    static constexpr int FLUX_PER_PATCH = NFLD * 2 * N * (N + 1);
+    static constexpr int LDS_DOUBLES =
+        PPB * (PRIM_PER_PATCH > FLUX_PER_PATCH ? PRIM_PER_PATCH : FLUX_PER_PATCH);
+    static constexpr int NINT = 2 * N * (N - 1);           // interior faces per patch
+    static constexpr int NEDGE = 4 * N;                    // patch-edge faces per patch
+    static constexpr int NTASK = PPB * (NINT + NEDGE);
+    static constexpr int ROUNDS = (NTASK + BLK - 1) / BLK;
+};
+
+template <int N, int PPB>
+__device__ __forceinline__ void store_prim(double *lds, int p, int t, const Prim &P)
+{
+    using TL = Tile<N, PPB>;
+    double *b = lds + p * TL::PRIM_PER_PATCH;
+    b[0 * TL::TT + t] = P.r;  b[1 * TL::TT + t] = P.ru; b[2 * TL::TT + t] = P.rv;
+    b[3 * TL::TT + t] = P.rE; b[4 * TL::TT + t] = P.u;  b[5 * TL::TT + t] = P.v;
+    b[6 * TL::TT + t] = P.p;  b[7 * TL::TT + t] = P.H;  b[8 * TL::TT + t] = P.sr;
+}
+template <int N, int PPB>
+__device__ __forceinline__ void load_prim(const double *lds, int p, int t, Prim &P)
+{
+    using TL = Tile<N, PPB>;
+    const double *b = lds + p * TL::PRIM_PER_PATCH;
+    P.r = b[0 * TL::TT + t];  P.ru = b[1 * TL::TT + t]; P.rv = b[2 * TL::TT + t];
+    P.rE = b[3 * TL::TT + t]; P.u = b[4 * TL::TT + t];  P.v = b[5 * TL::TT + t];
+    P.p = b[6 * TL::TT + t];  P.H = b[7 * TL::TT + t];  P.sr = b[8 * TL::TT + t];
+}
+// second fine neighbour of side s, index k
+template <int N, int PPB>
+__device__ __forceinline__ double *h2_ptr(double *lds, int p, int s, int k)
+{
+    using TL = Tile<N, PPB>;
+    return lds + p * TL::PRIM_PER_PATCH + TL::NP * TL::TT + (s * N + k) * TL::NP;
+}
+
+template <int N, int PPB>
+__global__ __launch_bounds__((Tile<N, PPB>::BLK)) void stage_kernel(StageArgs a)
+{
+    using TL = Tile<N, PPB>;
+    constexpr int N2 = TL::N2, T = TL::T;
+    __shared__ double lds[TL::LDS_DOUBLES];
+    __shared__ double red[(TL::BLK + 63) / 64];
+    __shared__ int flag[1];
+    __shared__ int pquad[PPB], plevel[PPB], pinst[PPB];
+    __shared__ int pnbr[PPB][8];
+
+    const int tid = threadIdx.x;
+    const int nblk = gridDim.x;
+    const int lb = xcd_block(blockIdx.x, nblk);
+    const int p = tid / N2, c = tid % N2;
+    const int ci = c % N, cj = c / N;
+    const int quad = lb * PPB + p;
+    const bool active = quad < a.m.nquad;
+
+    if (tid < PPB) {
+        int qq = lb * PPB + tid;
+        bool act = qq < a.m.nquad;
+        pquad[tid] = act ? qq : -1;
+        plevel[tid] = act ? a.m.level[qq] : 0;
+        pinst[tid] = act ? a.m.inst[qq] : 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) pnbr[tid][k] = act ? a.m.nbr[qq * 8 + k] : -1;
+    }
+    // own state (coalesced: consecutive threads -> consecutive cells)
+    double U[4] = {1.0, 0.0, 0.0, 1.0};
+    if (active) {
+        const double *src = a.Us + (size_t)quad * NFLD * N2 + c;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) U[f] = src[f * N2];
+    }
+    __syncthreads();
+    {
+        Prim P;
+        make_prim(U[0], U[1], U[2], U[3], P);
+        if (active) store_prim<N, PPB>(lds, p, (cj + 1) * T + (ci + 1), P);
+    }
+    // ghost ring
+    for (int hh = c; hh < 4 * N; hh += N2) {
+        if (!active) break;
+        int s = hh / N, k = hh % N;
+        int na = pnbr[p][2 * s], nb = pnbr[p][2 * s + 1];
+        if (na < 0) continue;
+        int ec, tpos;
+        switch (s) {
+        case 0: ec = k * N + (N - 1); tpos = (k + 1) * T + 0; break;
+        case 1: ec = k * N + 0;       tpos = (k + 1) * T + (N + 1); break;
+        case 2: ec = (N - 1) * N + k; tpos = 0 * T + (k + 1); break;
+        default: ec = 0 * N + k;      tpos = (N + 1) * T + (k + 1); break;
+        }
+        const double *sa = a.Us + (size_t)na * NFLD * N2 + ec;
+        Prim P;
+        make_prim(sa[0], sa[N2], sa[2 * N2], sa[3 * N2], P);
+        store_prim<N, PPB>(lds, p, tpos, P);
+        if (nb >= 0) {
+            const double *sb = a.Us + (size_t)nb * NFLD * N2 + ec;
+            Prim Q;
+            make_prim(sb[0], sb[N2], sb[2 * N2], sb[3 * N2], Q);
+            double *d = h2_ptr<N, PPB>(lds, p, s, k);
+            d[0] = Q.r; d[1] = Q.ru; d[2] = Q.rv; d[3] = Q.rE; d[4] = Q.u;
+            d[5] = Q.v; d[6] = Q.p; d[7] = Q.H; d[8] = Q.sr;
+        }
+    }
+    __syncthreads();
+
+    // ---- face fluxes into registers --------------------------------------
+    double Fr[TL::ROUNDS][4];
+    int Fdst[TL::ROUNDS];     // packed destination: patch, x/y, index
+#pragma unroll
+    for (int r = 0; r < TL::ROUNDS; ++r) {
+        Fdst[r] = -1;
+        int task = tid + r * TL::BLK;
+        if (task >= TL::NTASK) continue;
+        int tp, isx, fi, fj;   // patch; x-face (fi in 0..N, row fj) or y-face (col fi, fj in 0..N)
+        bool edge;
+        int side = -1;
+        if (task < PPB * TL::NINT) {
+            tp = task / TL::NINT;
+            int t = task % TL::NINT;
+            edge = false;
+            if (t < N * (N - 1)) { isx = 1; fi = 1 + t % (N - 1); fj = t / (N - 1); }
+            else { t -= N * (N - 1); isx = 0; fi = t % N; fj = 1 + t / N; }
+        } else {
+            int t = task - PPB * TL::NINT;
+            tp = t / TL::NEDGE;
+            int e = t % TL::NEDGE;
+            edge = true;
+            side = e / N;
+            int k = e % N;
+            switch (side) {
+            case 0: isx = 1; fi = 0; fj = k; break;
+            case 1: isx = 1; fi = N; fj = k; break;
+            case 2: isx = 0; fi = k; fj = 0; break;
+            default: isx = 0; fi = k; fj = N; break;
+            }
+        }
+        if (pquad[tp] < 0) continue;
+        const double h = ldexp(a.m.ip[pinst[tp]].h_base, -plevel[tp]) / (double)N;
+        double F[4];
+        // tile coordinates of the low (L) and high (R) cell of the face
+        int tl, tr;
+        if (isx) { tl = (fj + 1) * T + fi; tr = tl + 1; }
+        else     { tl = fj * T + (fi + 1); tr = tl + T; }
+        if (!edge) {
+            Prim L, R;
+            load_prim<N, PPB>(lds, tp, tl, L);
+            load_prim<N, PPB>(lds, tp, tr, R);
+            if (isx) flux_face<true>(L, R, h, F); else flux_face<false>(L, R, h, F);
+        } else {
+            int na = pnbr[tp][2 * side], nb = pnbr[tp][2 * side + 1];
+            bool lowside = (side == 0 || side == 2);
+            int town = lowside ? tr : tl;
+            if (na < 0) {
+                // physical boundary: outward normal; the face value is stored
+                // with the +x/+y orientation of the interior fluxes
+                int bc = a.m.ip[pinst[tp]].bc[side];
+                double nrm[2] = {0.0, 0.0};
+                if (side == 0) nrm[0] = -h; else if (side == 1) nrm[0] = h;
+                else if (side == 2) nrm[1] = -h; else nrm[1] = h;
+                const double *b = lds + tp * TL::PRIM_PER_PATCH;
+                double q[4] = {b[0 * TL::TT + town], b[1 * TL::TT + town],
+                               b[2 * TL::TT + town], b[3 * TL::TT + town]};
+                flux_boundary(bc, nrm, q, F);
+                if (lowside) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; F[3] = -F[3]; }
+            } else if (nb < 0) {
+                Prim L, R;
+                load_prim<N, PPB>(lds, tp, tl, L);
+                load_prim<N, PPB>(lds, tp, tr, R);
+                if (isx) flux_face<true>(L, R, h, F); else flux_face<false>(L, R, h, F);
+            } else {
+                // coarse side of a hanging face: sub-face k couples to sub-face
+                // k of both fine neighbours, normal length of the fine level
+                // (solver_residual_face_full_hang_structured_2d.F90:86-122)
+                const double hf = 0.5 * h;
+                int k = isx ? fj : fi;
+                Prim own, f1, f2;
+                load_prim<N, PPB>(lds, tp, town, own);
+                load_prim<N, PPB>(lds, tp, lowside ? tl : tr, f1);
+                const double *d = h2_ptr<N, PPB>(lds, tp, side, k);
+                f2.r = d[0]; f2.ru = d[1]; f2.rv = d[2]; f2.rE = d[3]; f2.u = d[4];
+                f2.v = d[5]; f2.p = d[6]; f2.H = d[7]; f2.sr = d[8];
+                double G[4];
+                if (lowside) {
+                    if (isx) { flux_face<true>(f1, own, hf, F); flux_face<true>(f2, own, hf, G); }
+                    else     { flux_face<false>(f1, own, hf, F); flux_face<false>(f2, own, hf, G); }
+                } else {
+                    if (isx) { flux_face<true>(own, f1, hf, F); flux_face<true>(own, f2, hf, G); }
+                    else     { flux_face<false>(own, f1, hf, F); flux_face<false>(own, f2, hf, G); }
+                }
+                F[0] += G[0]; F[1] += G[1]; F[2] += G[2]; F[3] += G[3];
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f) Fr[r][f] = F[f];
+        // FX[p][f][fj][fi] (fi in 0..N) ; FY[p][f][fj][fi] (fj in 0..N)
+        Fdst[r] = tp * TL::FLUX_PER_PATCH + (isx ? (fj * (N + 1) + fi)
+                                                  : (NFLD * N * (N + 1) + fj * N + fi));
+    }
+    __syncthreads();   // all reads of the primitive tile are done
+#pragma unroll
+    for (int r = 0; r < TL::ROUNDS; ++r) {
+        if (Fdst[r] < 0) continue;
+        const int isy = (Fdst[r] % TL::FLUX_PER_PATCH) >= NFLD * N * (N + 1);
+        const int stride = N * (N + 1);
+        (void)isy;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) lds[Fdst[r] + f * stride] = Fr[r][f];
+    }
+    __syncthreads();
+
+    // ---- per-cell residual and stage update ------------------------------
+    double k4[4] = {0.0, 0.0, 0.0, 0.0};
+    double out[4] = {0.0, 0.0, 0.0, 0.0};
+    double cand = INFINITY;
+    double sq = 0.0;
+    if (active) {
+        const double h = ldexp(a.m.ip[pinst[p]].h_base, -plevel[p]) / (double)N;
+        const double vol = h * h;
+        const double ninvV = -1.0 / vol;
+        const double *FX = lds + p * TL::FLUX_PER_PATCH;
+        const double *FY = FX + NFLD * N * (N + 1);
+        const int stride = N * (N + 1);
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            double Rf = (FX[f * stride + cj * (N + 1) + ci + 1] - FX[f * stride + cj * (N + 1) + ci]) +
+                        (FY[f * stride + (cj + 1) * N + ci] - FY[f * stride + cj * N + ci]);
+            k4[f] = Rf * ninvV;
+            sq += k4[f] * k4[f];
+        }
+        const size_t base = (size_t)quad * NFLD * N2 + c;
+        const int inst = pinst[p];
+        double dt = 0.0;
+        if (a.mode != MODE_RHS) dt = (a.role & ROLE_DT) ? stage_dt(a, inst) : a.dt_cur[inst];
+        switch (a.mode) {
+        case MODE_SO:
+            if (a.U0) {
+#pragma unroll
+                for (int f = 0; f < 4; ++f)
+                    out[f] = a.c0 * a.U0[base + f * N2] + a.c1 * U[f] + (a.c2 * dt) * k4[f];
+            } else {
+#pragma unroll
+                for (int f = 0; f < 4; ++f) out[f] = a.c1 * U[f] + (a.c2 * dt) * k4[f];
+            }
+            break;
+        case MODE_RK4_ACC:
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                double acc = a.first ? k4[f] : a.Acc[base + f * N2] + 2.0 * k4[f];
+                a.Acc[base + f * N2] = acc;
+                out[f] = a.U0[base + f * N2] + (a.c2 * dt) * k4[f];
+            }
+            break;
+        case MODE_RK4_FINAL:
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+                out[f] = a.U0[base + f * N2] + (dt / 6.0) * (a.Acc[base + f * N2] + k4[f]);
+            break;
+        case MODE_LSERK:
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                double dq = a.first ? dt * k4[f] : a.Acc[base + f * N2] * a.c1 + dt * k4[f];
+                a.Acc[base + f * N2] = dq;
+                out[f] = U[f] + a.c2 * dq;
+            }
+            break;
+        default:   // MODE_RHS
+#pragma unroll
+            for (int f = 0; f < 4; ++f) out[f] = k4[f];
+            break;
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f) a.Uout[base + f * N2] = out[f];
+        if (a.role & ROLE_CFL) cand = cfl_candidate(out, a.m.ip[inst].cfl, vol);
+    }
+    // time bookkeeping: stage 2 advances t by this step's dt and re-arms the
+    // CFL slot that the last stage fills
+    if ((a.role & ROLE_TIME) && blockIdx.x == 0) {
+        for (int i = tid; i < a.m.ninst; i += TL::BLK) {
+            a.t_inst[i] += a.dt_cur[i];
+            a.slot[i] = DBL_INF_BITS;
+        }
+    }
+    if ((a.role & ROLE_DT) && blockIdx.x == 0) {
+        for (int i = tid; i < a.m.ninst; i += TL::BLK) a.dt_cur[i] = stage_dt(a, i);
+    }
+    if (a.role & ROLE_CFL) cfl_reduce<TL::BLK>(cand, active ? pinst[p] : 0, active, a.slot, red, flag);
+    if (a.norm_part) sum_reduce_store<TL::BLK>(sq, &a.norm_part[blockIdx.x], red);
+}
+
+// Standalone CFL minimum (physics_time_step): one thread per cell.
+template <int N, int PPB>
+__global__ __launch_bounds__((Tile<N, PPB>::BLK)) void cfl_kernel(MeshDev m, const double *U,
+                                                                unsigned long long *slot)
+{
+    using TL = Tile<N, PPB>;
+    constexpr int N2 = TL::N2;
+    __shared__ double red[(TL::BLK + 63) / 64];
+    __shared__ int flag[1];
+    const int tid = threadIdx.x;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int p = tid / N2, c = tid % N2;
+    const int quad = lb * PPB + p;
+    const bool active = quad < m.nquad;
+    double cand = INFINITY;
+    int inst = 0;
+    if (active) {
+        inst = m.inst[quad];
+        const double h = ldexp(m.ip[inst].h_base, -m.level[quad]) / (double)N;
+        const double *src = U + (size_t)quad * NFLD * N2 + c;
+        double q[4] = {src[0], src[N2], src[2 * N2], src[3 * N2]};
+        cand = cfl_candidate(q, m.ip[inst].cfl, h * h);
+    }
+    cfl_reduce<TL::BLK>(cand, inst, active, slot, red, flag);
+}
+
+__global__ void slot_reset_kernel(unsigned long long *slot, int ninst)
+{
+    for (int i = threadIdx.x; i < ninst; i += blockDim.x) slot[i] = DBL_INF_BITS;
+}
+
+// RK1 has a single stage: the step's dt and time update run in this tiny
+// kernel ahead of it.
+__global__ void dt_prep_kernel(MeshDev m, int use_cfl, double *dt_cur, double *t_inst,
+                               unsigned long long *slot)
+{
+    for (int i = threadIdx.x; i < m.ninst; i += blockDim.x) {
+        const InstDev &ip = m.ip[i];
+        double dt = ip.dt_cap;
+        if (use_cfl) {
+            double c = bits2d(slot[i]);
+            if (c < dt) dt = c;
+        }
+        if (ip.t_end > 0.0) {
+            double rem = ip.t_end - t_inst[i];
+            if (rem < 0.0) rem = 0.0;
+            if (rem < dt) dt = rem;
+        }
+        dt_cur[i] = dt;
+        t_inst[i] += dt;
+        slot[i] = DBL_INF_BITS;
+    }
+}
+
+// reference patch layout Q(4,n,n) (AoS) <-> U[quad][field][cell] (SoA)
+__global__ void aos_to_soa_kernel(const double *aos, double *soa, long long ncells, int n2)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncells) return;
+    long long q = t / n2, c = t % n2;
+    const double *s = aos + t * NFLD;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) soa[(q * NFLD + f) * n2 + c] = s[f];
+}
+__global__ void soa_to_aos_kernel(const double *soa, double *aos, long long ncells, int n2)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncells) return;
+    long long q = t / n2, c = t % n2;
+    double *d = aos + t * NFLD;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) d[f] = soa[(q * NFLD + f) * n2 + c];
+}
+
+__global__ void copy_kernel(const double4 *__restrict__ src, double4 *__restrict__ dst, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+thread_local char g_err[512];
+
+void set_err(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+#define HIPCHK(x)                                                                   \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess) {                                                     \
+            set_err("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+            return AMRGPU_ERR_HIP;                                                  \
+        }                                                                           \
+    } while (0)
+
+struct StageLaunch {
+    StageArgs a;
+};
+
+}  // namespace
+
+struct amrgpu_ctx {
+    int n = 0, nquad = 0, ninst = 0, scheme = 3;
+    hipStream_t stream = nullptr;
+    int *d_nbr = nullptr, *d_level = nullptr, *d_inst = nullptr;
+    InstDev *d_ip = nullptr;
+    double *U[3] = {nullptr, nullptr, nullptr};
+    double *Acc = nullptr;
+    double *d_dt_cur = nullptr, *d_t = nullptr;
+    unsigned long long *d_slot = nullptr;
+    double *d_norm = nullptr;
+    double *d_stage = nullptr;      // AoS staging for up/download
+    int nblocks = 0;
+    int cur = 0;
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    std::vector<InstDev> h_ip;
+};
+
+namespace {
+
+MeshDev mesh_of(const amrgpu_ctx *c)
+{
+    MeshDev m;
+    m.nquad = c->nquad; m.ninst = c->ninst;
+    m.nbr = c->d_nbr; m.level = c->d_level; m.inst = c->d_inst; m.ip = c->d_ip;
+    return m;
+}
+
+int ppb_of(int n)
+{
+    switch (n) {
+    case 2: return 16;
+    case 4: return 8;
+    case 8: return 4;
+    case 16: return 1;
+    default: return 0;
+    }
+}
+
+int launch_stage(amrgpu_ctx *c, const StageArgs &a)
+{
+    int ppb = ppb_of(c->n);
+    int nblk = (c->nquad + ppb - 1) / ppb;
+    switch (c->n) {
+    case 2: hipLaunchKernelGGL((stage_kernel<2, 16>), dim3(nblk), dim3(64), 0, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((stage_kernel<4, 8>), dim3(nblk), dim3(128), 0, c->stream, a); break;
+    case 8: hipLaunchKernelGGL((stage_kernel<8, 4>), dim3(nblk), dim3(256), 0, c->stream, a); break;
+    case 16: hipLaunchKernelGGL((stage_kernel<16, 1>), dim3(nblk), dim3(256), 0, c->stream, a); break;
+    default: set_err("unsupported nelem_subgrid %d", c->n); return AMRGPU_ERR_ARG;
+    }
+    HIPCHK(hipGetLastError());
+    return AMRGPU_OK;
+}
+
+int launch_cfl(amrgpu_ctx *c, const double *U)
+{
+    int ppb = ppb_of(c->n);
+    int nblk = (c->nquad + ppb - 1) / ppb;
+    MeshDev m = mesh_of(c);
+    switch (c->n) {
+    case 2: hipLaunchKernelGGL((cfl_kernel<2, 16>), dim3(nblk), dim3(64), 0, c->stream, m, U, c->d_slot); break;
+    case 4: hipLaunchKernelGGL((cfl_kernel<4, 8>), dim3(nblk), dim3(128), 0, c->stream, m, U, c->d_slot); break;
+    case 8: hipLaunchKernelGGL((cfl_kernel<8, 4>), dim3(nblk), dim3(256), 0, c->stream, m, U, c->d_slot); break;
+    case 16: hipLaunchKernelGGL((cfl_kernel<16, 1>), dim3(nblk), dim3(256), 0, c->stream, m, U, c->d_slot); break;
+    default: set_err("unsupported nelem_subgrid %d", c->n); return AMRGPU_ERR_ARG;
+    }
+    HIPCHK(hipGetLastError());
+    return AMRGPU_OK;
+}
+
+StageArgs base_args(amrgpu_ctx *c)
+{
+    StageArgs a;
+    memset(&a, 0, sizeof(a));
+    a.m = mesh_of(c);
+    a.dt_cur = c->d_dt_cur;
+    a.t_inst = c->d_t;
+    a.slot = c->d_slot;
+    a.use_cfl = (c->scheme != 2);
+    return a;
+}
+
+// Stage launches of one step starting from state buffer `cur`; returns the
+// buffer holding the new state.  Shu-Osher forms need the stage state and
+// the step's base state only (2 reads + 1 write per cell-stage).
+int build_step(amrgpu_ctx *c, int cur, std::vector<StageArgs> &st, bool with_norm)
+{
+    st.clear();
+    double *A = c->U[cur], *B = c->U[(cur + 1) % 3], *C = c->U[(cur + 2) % 3];
+    StageArgs a = base_args(c);
+    int cfl = (c->scheme != 2) ? ROLE_CFL : 0;
+    switch (c->scheme) {
+    case 1:
+        a.Us = A; a.U0 = nullptr; a.Uout = B; a.mode = MODE_SO; a.c0 = 0; a.c1 = 1; a.c2 = 1;
+        a.role = cfl;
+        if (with_norm) a.norm_part = c->d_norm;
+        st.push_back(a);
+        return (cur + 1) % 3;
+    case 2:
+        a.Us = A; a.U0 = nullptr; a.Uout = B; a.mode = MODE_SO; a.c0 = 0; a.c1 = 1; a.c2 = 1;
+        a.role = ROLE_DT; st.push_back(a);
+        a.Us = B; a.U0 = A; a.Uout = A; a.c0 = 0.5; a.c1 = 0.5; a.c2 = 0.5;
+        a.role = ROLE_TIME; if (with_norm) a.norm_part = c->d_norm; st.push_back(a);
+        return cur;
+    case 3:
+        a.Us = A; a.U0 = nullptr; a.Uout = B; a.mode = MODE_SO; a.c0 = 0; a.c1 = 1; a.c2 = 1;
+        a.role = ROLE_DT; st.push_back(a);
+        a.Us = B; a.U0 = A; a.Uout = C; a.c0 = 0.75; a.c1 = 0.25; a.c2 = 0.25;
+        a.role = ROLE_TIME; st.push_back(a);
+        a.Us = C; a.U0 = A; a.Uout = A; a.c0 = 1.0 / 3.0; a.c1 = 2.0 / 3.0; a.c2 = 2.0 / 3.0;
+        a.role = cfl; if (with_norm) a.norm_part = c->d_norm; st.push_back(a);
+        return cur;
+    case 4:
+        a.Acc = c->Acc;
+        a.Us = A; a.U0 = A; a.Uout = B; a.mode = MODE_RK4_ACC; a.first = 1; a.c2 = 0.5;
+        a.role = ROLE_DT; st.push_back(a);
+        a.Us = B; a.Uout = C; a.first = 0; a.c2 = 0.5; a.role = ROLE_TIME; st.push_back(a);
+        a.Us = C; a.Uout = B; a.c2 = 1.0; a.role = 0; st.push_back(a);
+        a.Us = B; a.Uout = A; a.mode = MODE_RK4_FINAL; a.role = cfl;
+        if (with_norm) a.norm_part = c->d_norm; st.push_back(a);
+        return cur;
+    case 5: {
+        static const double ra[5] = {0.0,
+            -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+            -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+        static const double rb[5] = {1432997174477.0 / 9575080441755.0,
+            5161836677717.0 / 13612068292357.0, 1720146321549.0 / 2090206949498.0,
+            3134564353537.0 / 4481467310338.0, 2277821191437.0 / 14882151754819.0};
+        a.Acc = c->Acc;
+        a.mode = MODE_LSERK;
+        double *X = A, *Y = B;
+        for (int s = 0; s < 5; ++s) {
+            a.Us = X; a.U0 = nullptr; a.Uout = Y; a.first = (s == 0); a.c1 = ra[s]; a.c2 = rb[s];
+            a.role = (s == 0 ? ROLE_DT : 0) | (s == 1 ? ROLE_TIME : 0) | (s == 4 ? cfl : 0);
+            a.norm_part = (s == 4 && with_norm) ? c->d_norm : nullptr;
+            st.push_back(a);
+            double *tmp = X; X = Y; Y = tmp;
+        }
+        return (cur + 1) % 3;   // five ping-pong stages end in B
+    }
+    default:
+        return -1;
+    }
+}
+
+int enqueue_step(amrgpu_ctx *c, int cur, bool with_norm, int *next)
+{
+    std::vector<StageArgs> st;
+    int nx = build_step(c, cur, st, with_norm);
+    if (nx < 0) { set_err("bad time scheme %d", c->scheme); return AMRGPU_ERR_ARG; }
+    if (c->scheme == 1) {
+        hipLaunchKernelGGL(dt_prep_kernel, dim3(1), dim3(64), 0, c->stream, mesh_of(c), 1,
+                           c->d_dt_cur, c->d_t, c->d_slot);
+        HIPCHK(hipGetLastError());
+    }
+    for (auto &a : st) {
+        int rc = launch_stage(c, a);
+        if (rc) return rc;
+    }
+    *next = nx;
+    return AMRGPU_OK;
+}
+
+void free_graphs(amrgpu_ctx *c)
+{
+    for (int i = 0; i < 3; ++i)
+        if (c->graph[i]) { (void)hipGraphExecDestroy(c->graph[i]); c->graph[i] = nullptr; }
+}
+
+void free_mesh(amrgpu_ctx *c)
+{
+    free_graphs(c);
+    (void)hipFree(c->d_nbr); (void)hipFree(c->d_level); (void)hipFree(c->d_inst); (void)hipFree(c->d_ip);
+    for (int i = 0; i < 3; ++i) (void)hipFree(c->U[i]);
+    (void)hipFree(c->Acc); (void)hipFree(c->d_dt_cur); (void)hipFree(c->d_t); (void)hipFree(c->d_slot);
+    (void)hipFree(c->d_norm); (void)hipFree(c->d_stage);
+    c->d_nbr = c->d_level = c->d_inst = nullptr; c->d_ip = nullptr;
+    c->U[0] = c->U[1] = c->U[2] = nullptr; c->Acc = nullptr;
+    c->d_dt_cur = c->d_t = nullptr; c->d_slot = nullptr; c->d_norm = nullptr;
+    c->d_stage = nullptr;
+    c->nquad = 0;
+}
+
+// face_info (reference format) -> per-quadrant face table
+int build_nbr(int ncell, int nface, const int *face_info, std::vector<int> &nbr)
+{
+    nbr.assign((size_t)ncell * 8, -2);
+    auto setside = [&](int q, int s, int a, int b) -> int {
+        if (q < 0 || q >= ncell || s < 0 || s > 3) return -1;
+        int *e = &nbr[(size_t)q * 8 + 2 * s];
+        if (e[0] != -2) return -1;   // side already described
+        e[0] = a; e[1] = b;
+        return 0;
+    };
+    for (int f = 0; f < nface; ++f) {
+        const int *fi = &face_info[11 * f];
+        int type = fi[0], side = fi[1], cell = fi[2];
+        int rc = 0;
+        if (type == 1) {
+            rc |= setside(cell, side, -1, -1);
+        } else if (type == 2) {
+            int o = fi[3];
+            if (o < 0 || o >= ncell) { set_err("face %d: bad neighbour %d", f, o); return AMRGPU_ERR_MESH; }
+            rc |= setside(cell, side, o, -1);
+            if (o != cell || (side ^ 1) != side) rc |= setside(o, side ^ 1, cell, -1);
+        } else if (type == 3) {
+            int f1 = fi[3], f2 = fi[4];
+            if (f1 < 0 || f1 >= ncell || f2 < 0 || f2 >= ncell) {
+                set_err("face %d: bad hanging neighbours %d %d", f, f1, f2);
+                return AMRGPU_ERR_MESH;
+            }
+            rc |= setside(cell, side, f1, f2);
+            rc |= setside(f1, side ^ 1, cell, -1);
+            rc |= setside(f2, side ^ 1, cell, -1);
+        } else {
+            set_err("face %d: unsupported face type %d", f, type);
+            return AMRGPU_ERR_MESH;
+        }
+        if (rc) { set_err("face %d: inconsistent face table", f); return AMRGPU_ERR_MESH; }
+    }
+    for (int q = 0; q < ncell; ++q)
+        for (int s = 0; s < 4; ++s)
+            if (nbr[(size_t)q * 8 + 2 * s] == -2) {
+                set_err("quadrant %d side %d has no face", q, s);
+                return AMRGPU_ERR_MESH;
+            }
+    return AMRGPU_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *amrgpu_last_error(void) { return g_err; }
+
+amrgpu_ctx *amrgpu_create(void)
+{
+    amrgpu_ctx *c = new amrgpu_ctx();
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_err("hipStreamCreate failed");
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+void amrgpu_destroy(amrgpu_ctx *c)
+{
+    if (!c) return;
+    (void)hipStreamSynchronize(c->stream);
+    free_mesh(c);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int amrgpu_num_cells(const amrgpu_ctx *c) { return c ? c->nquad * c->n * c->n : 0; }
+int amrgpu_num_quads(const amrgpu_ctx *c) { return c ? c->nquad : 0; }
+
+int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
+                 const amrgpu_inst_t *inst, int ncell, const int *cell_info,
+                 const int *cell_inst, int nface, const int *face_info, const double *geom,
+                 const double *soln)
+{
+    (void)geom;
+    if (!c || !inst || !cell_info || !face_info || ncell <= 0 || ninst <= 0 || nface < 0) {
+        set_err("amrgpu_setup: bad arguments");
+        return AMRGPU_ERR_ARG;
+    }
+    if (ppb_of(nelem_subgrid) == 0) {
+        set_err("amrgpu_setup: nelem_subgrid %d not in {2,4,8,16}", nelem_subgrid);
+        return AMRGPU_ERR_ARG;
+    }
+    if (time_scheme < 1 || time_scheme > 5) {
+        set_err("amrgpu_setup: time_scheme %d", time_scheme);
+        return AMRGPU_ERR_ARG;
+    }
+    const int n2 = nelem_subgrid * nelem_subgrid;
+    for (int q = 0; q < ncell; ++q) {
+        if (cell_info[3 * q + 2] != 4 * n2 * q) {
+            set_err("amrgpu_setup: cell %d soln offset %d is not patch-contiguous", q, cell_info[3 * q + 2]);
+            return AMRGPU_ERR_MESH;
+        }
+        int lv = cell_info[3 * q + 1];
+        if (lv < 0 || lv > 29) { set_err("amrgpu_setup: cell %d level %d", q, lv); return AMRGPU_ERR_MESH; }
+        if (cell_inst && (cell_inst[q] < 0 || cell_inst[q] >= ninst)) {
+            set_err("amrgpu_setup: cell %d instance %d", q, cell_inst[q]);
+            return AMRGPU_ERR_ARG;
+        }
+    }
+    std::vector<int> nbr;
+    int rc = build_nbr(ncell, nface, face_info, nbr);
+    if (rc) return rc;
+
+    HIPCHK(hipStreamSynchronize(c->stream));
+    free_mesh(c);
+    c->n = nelem_subgrid; c->nquad = ncell; c->ninst = ninst; c->scheme = time_scheme;
+    c->cur = 0;
+    std::vector<int> level(ncell), qi(ncell, 0);
+    for (int q = 0; q < ncell; ++q) {
+        level[q] = cell_info[3 * q + 1];
+        if (cell_inst) qi[q] = cell_inst[q];
+    }
+    c->h_ip.resize(ninst);
+    for (int i = 0; i < ninst; ++i) {
+        c->h_ip[i].h_base = inst[i].h_base;
+        for (int s = 0; s < 4; ++s) c->h_ip[i].bc[s] = inst[i].bc[s];
+        c->h_ip[i].cfl = inst[i].cfl;
+        c->h_ip[i].dt_cap = inst[i].dt_cap;
+        c->h_ip[i].t_end = inst[i].t_end;
+    }
+    const size_t nstate = (size_t)ncell * 4 * n2;
+    int ppb = ppb_of(c->n);
+    c->nblocks = (ncell + ppb - 1) / ppb;
+    HIPCHK(hipMalloc(&c->d_nbr, sizeof(int) * nbr.size()));
+    HIPCHK(hipMalloc(&c->d_level, sizeof(int) * ncell));
+    HIPCHK(hipMalloc(&c->d_inst, sizeof(int) * ncell));
+    HIPCHK(hipMalloc(&c->d_ip, sizeof(InstDev) * ninst));
+    for (int i = 0; i < 3; ++i) HIPCHK(hipMalloc(&c->U[i], sizeof(double) * nstate));
+    if (time_scheme == 4 || time_scheme == 5) HIPCHK(hipMalloc(&c->Acc, sizeof(double) * nstate));
+    HIPCHK(hipMalloc(&c->d_dt_cur, sizeof(double) * ninst));
+    HIPCHK(hipMalloc(&c->d_t, sizeof(double) * ninst));
+    HIPCHK(hipMalloc(&c->d_slot, sizeof(unsigned long long) * ninst));
+    HIPCHK(hipMalloc(&c->d_norm, sizeof(double) * c->nblocks));
+    HIPCHK(hipMalloc(&c->d_stage, sizeof(double) * nstate));
+    HIPCHK(hipMemcpyAsync(c->d_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_level, level.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_inst, qi.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> zeros(ninst, 0.0);
+    HIPCHK(hipMemcpyAsync(c->d_t, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_dt_cur, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (soln) return amrgpu_upload_soln(c, soln);
+    return AMRGPU_OK;
+}
+
+int amrgpu_upload_soln(amrgpu_ctx *c, const double *soln)
+{
+    if (!c || !soln || !c->nquad) { set_err("amrgpu_upload_soln: no mesh"); return AMRGPU_ERR_ARG; }
+    const int n2 = c->n * c->n;
+    const long long ncells = (long long)c->nquad * n2;
+    HIPCHK(hipMemcpyAsync(c->d_stage, soln, sizeof(double) * 4 * ncells, hipMemcpyHostToDevice, c->stream));
+    c->cur = 0;
+    hipLaunchKernelGGL(aos_to_soa_kernel, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, c->stream,
+                       c->d_stage, c->U[0], ncells, n2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMRGPU_OK;
+}
+
+int amrgpu_download_soln(amrgpu_ctx *c, double *soln)
+{
+    if (!c || !soln || !c->nquad) { set_err("amrgpu_download_soln: no mesh"); return AMRGPU_ERR_ARG; }
+    const int n2 = c->n * c->n;
+    const long long ncells = (long long)c->nquad * n2;
+    hipLaunchKernelGGL(soa_to_aos_kernel, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, c->stream,
+                       c->U[c->cur], c->d_stage, ncells, n2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(soln, c->d_stage, sizeof(double) * 4 * ncells, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMRGPU_OK;
+}
+
+int amrgpu_rhs(amrgpu_ctx *c, double *rhs)
+{
+    if (!c || !rhs || !c->nquad) { set_err("amrgpu_rhs: no mesh"); return AMRGPU_ERR_ARG; }
+    StageArgs a = base_args(c);
+    int other = (c->cur + 1) % 3;
+    a.Us = c->U[c->cur]; a.Uout = c->U[other]; a.mode = MODE_RHS; a.role = 0;
+    int rc = launch_stage(c, a);
+    if (rc) return rc;
+    const int n2 = c->n * c->n;
+    const long long ncells = (long long)c->nquad * n2;
+    hipLaunchKernelGGL(soa_to_aos_kernel, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, c->stream,
+                       c->U[other], c->d_stage, ncells, n2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(rhs, c->d_stage, sizeof(double) * 4 * ncells, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMRGPU_OK;
+}
+
+int amrgpu_cfl(amrgpu_ctx *c, double *dt_inst)
+{
+    if (!c || !dt_inst || !c->nquad) { set_err("amrgpu_cfl: no mesh"); return AMRGPU_ERR_ARG; }
+    hipLaunchKernelGGL(slot_reset_kernel, dim3(1), dim3(64), 0, c->stream, c->d_slot, c->ninst);
+    HIPCHK(hipGetLastError());
+    int rc = launch_cfl(c, c->U[c->cur]);
+    if (rc) return rc;
+    std::vector<unsigned long long> b(c->ninst);
+    HIPCHK(hipMemcpyAsync(b.data(), c->d_slot, sizeof(unsigned long long) * c->ninst, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->ninst; ++i) memcpy(&dt_inst[i], &b[i], sizeof(double));
+    return AMRGPU_OK;
+}
+
+static int prime_run(amrgpu_ctx *c)
+{
+    // CFL slot of the current state for the first step's dt
+    hipLaunchKernelGGL(slot_reset_kernel, dim3(1), dim3(64), 0, c->stream, c->d_slot, c->ninst);
+    HIPCHK(hipGetLastError());
+    if (c->scheme != 2) return launch_cfl(c, c->U[c->cur]);
+    return AMRGPU_OK;
+}
+
+static int graph_for(amrgpu_ctx *c, int cur, hipGraphExec_t *out)
+{
+    if (c->graph[cur]) { *out = c->graph[cur]; return AMRGPU_OK; }
+    hipGraph_t g;
+    int nx;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(c, cur, true, &nx);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) { set_err("graph capture: %s", hipGetErrorString(e)); return AMRGPU_ERR_HIP; }
+    HIPCHK(hipGraphInstantiate(&c->graph[cur], g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    *out = c->graph[cur];
+    return AMRGPU_OK;
+}
+
+static int step_next(amrgpu_ctx *c, int cur)
+{
+    std::vector<StageArgs> st;
+    return build_step(c, cur, st, false);
+}
+
+static int finish_norm(amrgpu_ctx *c, double *resnorm)
+{
+    std::vector<double> part(c->nblocks);
+    HIPCHK(hipMemcpyAsync(part.data(), c->d_norm, sizeof(double) * c->nblocks, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (double v : part) s += v;
+    double rn = sqrt(s);
+    if (resnorm) *resnorm = rn;
+    if (std::isnan(rn)) { set_err("nan detected in residual"); return AMRGPU_ERR_NAN; }
+    return AMRGPU_OK;
+}
+
+int amrgpu_run(amrgpu_ctx *c, int nsteps, int use_graph, double *resnorm)
+{
+    if (!c || !c->nquad || nsteps < 0) { set_err("amrgpu_run: no mesh"); return AMRGPU_ERR_ARG; }
+    if (nsteps == 0) return AMRGPU_OK;
+    int rc = prime_run(c);
+    if (rc) return rc;
+    for (int s = 0; s < nsteps; ++s) {
+        int nx;
+        if (use_graph) {
+            hipGraphExec_t g;
+            rc = graph_for(c, c->cur, &g);
+            if (rc) return rc;
+            HIPCHK(hipGraphLaunch(g, c->stream));
+            nx = step_next(c, c->cur);
+        } else {
+            rc = enqueue_step(c, c->cur, true, &nx);
+            if (rc) return rc;
+        }
+        c->cur = nx;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return finish_norm(c, resnorm);
+}
+
+int amrgpu_get_time(amrgpu_ctx *c, double *t_inst, double *dt_inst)
+{
+    if (!c || !c->nquad) { set_err("amrgpu_get_time: no mesh"); return AMRGPU_ERR_ARG; }
+    if (t_inst) HIPCHK(hipMemcpyAsync(t_inst, c->d_t, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->stream));
+    if (dt_inst) HIPCHK(hipMemcpyAsync(dt_inst, c->d_dt_cur, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_time(amrgpu_ctx *c, const double *t_inst)
+{
+    if (!c || !c->nquad || !t_inst) { set_err("amrgpu_set_time: no mesh"); return AMRGPU_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(c->d_t, t_inst, sizeof(double) * c->ninst, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMRGPU_OK;
+}
+
+int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_ms, int *nstages)
+{
+    if (!c || !c->nquad || nsteps <= 0) { set_err("amrgpu_bench: no mesh"); return AMRGPU_ERR_ARG; }
+    int rc = prime_run(c);
+    if (rc) return rc;
+    // capture graphs for every starting buffer reached before timing
+    {
+        int cur = c->cur;
+        for (int s = 0; s < 2; ++s) {
+            hipGraphExec_t g;
+            rc = graph_for(c, cur, &g);
+            if (rc) return rc;
+            cur = step_next(c, cur);
+        }
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        hipGraphExec_t g;
+        rc = graph_for(c, c->cur, &g);
+        if (rc) return rc;
+        HIPCHK(hipGraphLaunch(g, c->stream));
+        c->cur = step_next(c, c->cur);
+    }
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_total) *ms_total = ms;
+
+    // per-stage kernel durations: direct launches bracketed by events
+    std::vector<StageArgs> st;
+    build_step(c, c->cur, st, true);
+    const int ns = (int)st.size();
+    std::vector<hipEvent_t> ev((size_t)nsteps * (ns + 1));
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    for (int s = 0; s < nsteps; ++s) {
+        int nx = build_step(c, c->cur, st, true);
+        if (c->scheme == 1) {
+            hipLaunchKernelGGL(dt_prep_kernel, dim3(1), dim3(64), 0, c->stream, mesh_of(c), 1,
+                               c->d_dt_cur, c->d_t, c->d_slot);
+        }
+        HIPCHK(hipEventRecord(ev[(size_t)s * (ns + 1)], c->stream));
+        for (int k = 0; k < ns; ++k) {
+            rc = launch_stage(c, st[k]);
+            if (rc) return rc;
+            HIPCHK(hipEventRecord(ev[(size_t)s * (ns + 1) + k + 1], c->stream));
+        }
+        c->cur = nx;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < ns; ++k) {
+        double tot = 0.0;
+        for (int s = 0; s < nsteps; ++s) {
+            float m = 0.f;
+            HIPCHK(hipEventElapsedTime(&m, ev[(size_t)s * (ns + 1) + k], ev[(size_t)s * (ns + 1) + k + 1]));
+            tot += m;
+        }
+        if (avg_stage_ms) avg_stage_ms[k] = tot / nsteps;
+    }
+    if (nstages) *nstages = ns;
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return finish_norm(c, nullptr);
+}
+
+int amrgpu_copy_probe(long long nbytes, int reps, double *gbps)
+{
+    long long n4 = nbytes / 32;
+    if (n4 <= 0 || reps <= 0) { set_err("amrgpu_copy_probe: bad size"); return AMRGPU_ERR_ARG; }
+    double4 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, sizeof(double4) * n4));
+    HIPCHK(hipMalloc(&b, sizeof(double4) * n4));
+    HIPCHK(hipMemset(a, 0, sizeof(double4) * n4));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int blocks = 256 * 8 * 4;
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, 0, a, b, n4);
+    HIPCHK(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, 0, a, b, n4);
+    HIPCHK(hipEventRecord(e1, 0));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (gbps) *gbps = 2.0 * (double)n4 * 32.0 * reps / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return AMRGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+/*
+ * amr_driver.h -- host AMR driver pieces around the GPU physics path:
+ * initial conditions, the density-gradient indicator and the regrid
+ * sequence of src/amr/src/amr_regrid.c:15-81 on the host forest.
+ *
+ * Initial conditions: free stream and the isentropic vortex follow
+ * src/solver/src/initial_condition_2d_module.F90:12-131 (vortex strength
+ * and centre as parameters); the four-quadrant Riemann problem and the
+ * double-Mach reflection are project additions the configs need.
+ *
+ * The density-gradient indicator is project-defined (the reference tags by
+ * density thresholds only, src/solver/src/tagging_methods_module.F90:14-71):
+ * a quadrant is tagged when max |rho_a - rho_b| / (0.5 (rho_a + rho_b)) over
+ * adjacent sub-cell pairs of its patch and across its four faces (the same
+ * index-aligned pairing the residual uses) exceeds the tolerance.
+ */
+#ifndef AMR_DRIVER_H
+#define AMR_DRIVER_H
+
+#include "amr_forest.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { AMR_IC_FREESTREAM = 0, AMR_IC_VORTEX = 1, AMR_IC_RIEMANN4 = 2, AMR_IC_DMR = 3 };
+
+typedef struct {
+    int kind;
+    /* free stream / vortex (reference inputs) */
+    double mach, alpha, beta, gamma, density, pressure;
+    double vortex_strength, xo, yo;
+    /* four-quadrant Riemann: split point and primitive states (rho,u,v,p)
+     * of quadrants 0:(x<xs,y<ys) 1:(x>xs,y<ys) 2:(x<xs,y>ys) 3:(x>xs,y>ys) */
+    double xs, ys;
+    double state[4][4];
+    /* double-Mach reflection: shock through (x0, 0) at angle (deg) to the
+     * x axis, post-shock state left of it, pre-shock (rho_pre, p_pre at rest)
+     * right of it, shock Mach number */
+    double dmr_x0, dmr_angle, dmr_mach, dmr_rho, dmr_p;
+} amr_ic_t;
+
+/* Fill one patch Q(4,n,n) from the corner geometry (bilinear centres). */
+void amr_ic_patch(const amr_ic_t *ic, int n, const double geom[12], double *Q);
+/* Fill every quadrant of the forest. */
+void amr_ic_apply(amr_forest_t *f, const amr_ic_t *ic);
+/* Exact vortex density at (x, y, t) on a periodic box [x0,x0+Lx] x [y0,y0+Ly]
+ * (nearest periodic image of the advected centre). */
+double amr_ic_vortex_density(const amr_ic_t *ic, double x, double y, double t,
+                             double x0, double Lx, double y0, double Ly);
+
+/* Project-defined density-gradient indicator per quadrant (see above);
+ * ind[k] receives the indicator value. */
+void amr_indicator_gradient(amr_forest_t *f, double *ind);
+
+/* Regrid in the order of regrid_solution (src/amr/src/amr_regrid.c:15-81):
+ * reset tags, tag by indicator > tol and refine tagged quadrants one level
+ * (below max_level), coarsen untagged families (unless initial), 2:1 balance.
+ * Returns the new number of quadrants. */
+int amr_regrid(amr_forest_t *f, double tol, int allow_coarsening);
+/* Initial refinement (regrid_feature with initial = 1): repeatedly tag and
+ * refine, re-evaluating the initial condition on new quadrants
+ * (initialize_quadrant_data), then balance. */
+int amr_regrid_initial(amr_forest_t *f, const amr_ic_t *ic, double tol);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,84 @@
+/*
+ * amr_forest.h -- host quadtree forest that stands in for p4est on the AMR
+ * side of the boundary (p4est is not available in this image).
+ *
+ * It owns the mesh (brick of trees, linear Morton-ordered quadrants, 2:1
+ * face+corner balance, refine/coarsen with the reference's data operators)
+ * and produces the exact arrays the reference AMR module hands to the physics
+ * module through hpc_amr_evolve_solver (src/amr/src/amr_external.c:211-508):
+ *
+ *   cell_info[3*c + {0,1,2}] = {type, level, soln offset = 4*n*n*c}
+ *   face_info[11*f + k]      = {facetype (1 bc, 2 full, 3 hanging 2D), side of the
+ *                               full quad, its cell id, neighbour ids [3..6],
+ *                               neighbour sides [7..10]}, unused entries -1
+ *   geom[12*c]               = 4 corners x (x,y,z), z-order, as
+ *                               p4est_utilities_quad_coordinates
+ *                               (src/amr/src/amr_p4est_utilities.c:15-81)
+ *   soln                     = per quadrant patch Q(4,n,n), patch-contiguous
+ *
+ * One process owns the whole forest: no ghosts, nsend = nrecv = 0.
+ */
+#ifndef AMR_FOREST_H
+#define AMR_FOREST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMR_MAXLEVEL 30                 /* P4EST_MAXLEVEL in 2D */
+#define AMR_ROOT_LEN (1 << AMR_MAXLEVEL)
+
+typedef struct {
+    int tree;       /* brick tree id */
+    int level;
+    int x, y;       /* integer coordinates in [0, AMR_ROOT_LEN) */
+} amr_quad_t;
+
+typedef struct amr_forest amr_forest_t;
+
+/* Brick of ntx x nty trees of edge `scale`, lower corner (xlo, ylo), periodic
+ * flags per direction; n x n sub-grid patch per quadrant; uniform refinement
+ * to min_level.  Mirrors initialize_grid + p4est_new_ext
+ * (src/amr/src/amr_initialize.c:213-314, 199-207). */
+amr_forest_t *amr_forest_new(int ntx, int nty, int periodic_x, int periodic_y,
+                             double xlo, double ylo, double scale, int n,
+                             int min_level, int max_level);
+void amr_forest_destroy(amr_forest_t *f);
+
+int amr_forest_num_quads(const amr_forest_t *f);
+int amr_forest_nelem_subgrid(const amr_forest_t *f);
+const amr_quad_t *amr_forest_quads(const amr_forest_t *f);
+double *amr_forest_soln(amr_forest_t *f);     /* nquad * 4*n*n, reference layout */
+int *amr_forest_tags(amr_forest_t *f);        /* nquad */
+double amr_forest_scale(const amr_forest_t *f);
+
+/* Corner coordinates of quadrant k (12 doubles), reference formula. */
+void amr_forest_quad_geom(const amr_forest_t *f, int k, double xyz[12]);
+
+/* Build the boundary arrays (memory owned by the forest, valid until the next
+ * call or destroy).  Returns 0 on success. */
+int amr_forest_build_external(amr_forest_t *f, int *ncell, int *nghost, int *nface, int *dof,
+                              int **cell_info, int **face_info, double **geom, double **soln);
+
+/* Refine every quadrant with flags[k] != 0 (and level < max_level) by one
+ * level; children copy the parent patch (solver_refine_operator_2d) and
+ * inherit its tag.  Returns the number of quadrants refined. */
+int amr_forest_refine(amr_forest_t *f, const int *flags);
+/* Coarsen every complete family whose four members have tag == 0 and level >
+ * min_level (tag_coarsen, src/amr/src/amr_tag.c:202-220); the parent patch is
+ * the element-wise mean (solver_coarsen_operator_2d).  Returns families
+ * coarsened. */
+int amr_forest_coarsen(amr_forest_t *f);
+/* 2:1 balance across faces and corners (P4EST_CONNECT_FULL); refined
+ * quadrants copy their parent patch.  Returns quadrants refined. */
+int amr_forest_balance(amr_forest_t *f);
+/* 1 if the forest is 2:1 balanced across faces and corners. */
+int amr_forest_is_balanced(const amr_forest_t *f);
+
+/* Leaf index containing the point (tree, x, y) or -1. */
+int amr_forest_find_leaf(const amr_forest_t *f, int tree, int x, int y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

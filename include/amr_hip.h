@@ -1,0 +1,158 @@
+/*
+ * amr_hip.h -- C ABI of the MI355X implementation of the demo's finite-volume
+ * Euler hot path (patch residual, RK stage update, CFL time step).
+ *
+ * Two layers:
+ *
+ * 1. Drop-in physics entry points.  Same names, argument lists and meaning
+ *    as the reference physics module, so an AMR driver written against
+ *    src/amr/src/amr_external.h binds them unchanged.  The only deviation is
+ *    the communicator: this image has no MPI, and one GPU owns the whole
+ *    forest, so the MPI_Comm argument is an opaque pointer that is ignored.
+ *
+ * 2. A device-resident context (amrgpu_*), with no reference counterpart,
+ *    that keeps the state in HBM across steps, runs whole evolve intervals as
+ *    captured HIP graphs and exposes the timing hooks the benchmark needs.
+ *    It also carries the batched mode: several independent forests
+ *    ("instances") advanced by the same launches.
+ *
+ * All arrays are host pointers unless stated otherwise.  Return codes: 0 ok,
+ * negative on argument/HIP/mesh errors, AMRGPU_ERR_NAN (3, the reference's
+ * MPI_Abort code) when the residual norm is NaN.
+ */
+#ifndef AMR_HIP_H
+#define AMR_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMRGPU_OK        0
+#define AMRGPU_ERR_ARG  -1
+#define AMRGPU_ERR_HIP  -2
+#define AMRGPU_ERR_MESH -3
+#define AMRGPU_ERR_NAN   3
+
+/* Physics settings: the reference keeps these in header globals filled from
+ * input.physics (src/physics/src/physics_initialize.c:61-197, 312-316). */
+typedef struct {
+    int    dim, nfields, nelem_subgrid;
+    int    extern_cell_size, extern_face_size;
+    int    time_scheme;            /* 1 RK1, 2 SSP-RK2, 3 SSP-RK3, 4 RK4, 5 LSERK45 */
+    int    time_steps;
+    double dt;                     /* input dt (cap) */
+    int    checkpoint_interval, visualization_interval, regrid_interval;
+    int    bc[6];
+    double cfl;                    /* effective CFL = 0.25*input*time_scheme */
+    double coarse_grid_h;          /* mesh_scale */
+    double t_end;                  /* physical-time stop (project addition; <= 0: off) */
+    int    verbose;
+} physics_gpu_params_t;
+
+/* Replaces hpc_amr_initialize_solver (src/physics/src/hpc_amr_initialize_solver.c:16-28
+ * -> physics_initialize, src/physics/src/physics_initialize.c:16-58): reads
+ * input.physics from the working directory with the same keywords, defaults,
+ * CFL rescale and interval fix-up. */
+void hpc_amr_initialize_solver(int *rank, int *dim, double *mesh_scale, int *nlevels,
+                               int *nfields, int *nfringe, int *nelem_subgrid,
+                               int *extern_cell_size, int *extern_face_size,
+                               int *extern_edge_size, int *extern_node_size,
+                               int *extern_mpi_comm_size);
+
+/* Programmatic access to the same settings (no reference counterpart; the
+ * reference writes the globals directly). */
+void physics_gpu_set_params(const physics_gpu_params_t *p);
+void physics_gpu_get_params(physics_gpu_params_t *p);
+
+/* The GPU seam: physics_go_gpu has the argument list of physics_go
+ * (src/physics/src/physics_go.h:59-84), as physics_evolve_interface calls it
+ * under _GPU_ (src/physics/src/physics_evolve_interface.c:42-46).  Uploads the
+ * interval's arrays, runs steps until time_steps or an interval flag
+ * (src/physics/src/physics_runge_kutta_interface.c:195-230) on the GPU, and
+ * writes soln back in place.  Returns AMRGPU_ERR_NAN where the reference calls
+ * MPI_Abort(comm, 3). */
+int physics_go_gpu(int *dim, int *unstructured, int *nfields, int *nelem_subgrid,
+                   int *ncell, int *nghost, int *nface, int *dof,
+                   int *cell_info, int *face_info, int *nsend, int *nrecv,
+                   int *send, int *recv, int *sizes, double *geom, double *soln,
+                   int *istep, int *regrid, int *visualize, int *checkpoint,
+                   int *evolve_solution, void *mpicomm_ext,
+                   double *total_time, double *compute_time, double *residual_time);
+
+/* Replaces hpc_amr_evolve_solver (src/physics/src/hpc_amr_evolve_solver.c:16-49,
+ * declared at src/amr/src/amr_external.h:124-130); forwards to physics_go_gpu. */
+int hpc_amr_evolve_solver(int *dim, int *unstructured, int *nfields, int *nelem_subgrid,
+                          int *ncell, int *nghost, int *nface, int *dof,
+                          int *cell_info, int *face_info, int *nsend, int *nrecv,
+                          int *send, int *recv, int *sizes, double *geom, double *soln,
+                          int *istep, int *regrid, int *visualize, int *checkpoint,
+                          int *evolve_solution, void *mpicomm_ext,
+                          double *total_time, double *compute_time, double *residual_time);
+
+/* Replaces physics_rhs_structured (src/physics/src/physics_rhs_structured.c:19-22):
+ * rhs[0:dof) = -R/V of soln, evaluated on the GPU (dim 2, nfields 4). */
+int physics_rhs_structured_gpu(int *dim, int *nfields, int *nelem_subgrid, int *ncell,
+                               int *cell_info, double *geom, int *nface, int *face_info,
+                               int *dof, double *soln, double *rhs);
+
+/* Replaces physics_time_step (src/physics/src/physics_time_step.c:16-19):
+ * *dt = min(*dt, min over cells of cfl*sqrt(V)/(|u|+c)), on the GPU. */
+int physics_time_step_gpu(int *dim, int *nfields, int *nelem_subgrid, int *ncell,
+                          int *cell_info, double *geom, double *soln, double *dt);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident context                                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct amrgpu_ctx amrgpu_ctx;
+
+/* Per-instance settings (one forest = one instance). */
+typedef struct {
+    double h_base;      /* coarse_grid_h of this forest */
+    int    bc[4];       /* xlo, xhi, ylo, yhi */
+    double cfl;         /* effective CFL */
+    double dt_cap;      /* input dt */
+    double t_end;       /* physical-time stop, <= 0: none */
+} amrgpu_inst_t;
+
+amrgpu_ctx *amrgpu_create(void);
+void amrgpu_destroy(amrgpu_ctx *c);
+const char *amrgpu_last_error(void);
+
+/* Upload a mesh + state.  cell_info/face_info/soln use the reference layout
+ * (cell ids and soln offsets global over all instances); cell_inst[ncell]
+ * gives each quadrant's instance (NULL: all instance 0).  geom is accepted for
+ * interface symmetry: the structured path derives sub-cell sizes from the
+ * level, as the reference face kernels do. */
+int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme,
+                 int ninst, const amrgpu_inst_t *inst,
+                 int ncell, const int *cell_info, const int *cell_inst,
+                 int nface, const int *face_info, const double *geom, const double *soln);
+int amrgpu_upload_soln(amrgpu_ctx *c, const double *soln);
+int amrgpu_download_soln(amrgpu_ctx *c, double *soln);
+/* Residual rhs = -R/V of the current state (reference layout, host out). */
+int amrgpu_rhs(amrgpu_ctx *c, double *rhs);
+/* CFL minimum of the current state per instance (no dt cap). */
+int amrgpu_cfl(amrgpu_ctx *c, double *dt_inst);
+/* Advance nsteps steps.  use_graph: replay one captured step.  On return
+ * resnorm (if non-NULL) holds sqrt(sum rhs^2) of the last stage. */
+int amrgpu_run(amrgpu_ctx *c, int nsteps, int use_graph, double *resnorm);
+/* Per-instance physical time and last dt. */
+int amrgpu_get_time(amrgpu_ctx *c, double *t_inst, double *dt_inst);
+int amrgpu_set_time(amrgpu_ctx *c, const double *t_inst);
+
+/* Benchmark hooks: run nsteps steps, timing the whole region with HIP events
+ * (graph replay) -> *ms_total; then nsteps more steps launched directly with
+ * events around every stage kernel -> avg_stage_ms[s] for s < nstages. */
+int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_ms,
+                 int *nstages);
+int amrgpu_num_cells(const amrgpu_ctx *c);   /* sub-grid cells */
+int amrgpu_num_quads(const amrgpu_ctx *c);
+
+/* Device-side helpers for the HBM roofline probe: copy nbytes between two
+ * device buffers allocated by the library, timed with events. */
+int amrgpu_copy_probe(long long nbytes, int reps, double *gbps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,142 @@
+"""Host forest (p4est stand-in) invariants: p4est is absent, so topology,
+balance and the boundary arrays are pinned by construction checks."""
+import numpy as np
+import pytest
+
+import amrhip
+
+ROOT_LEN = 1 << 30
+
+
+def corners(geom, k):
+    g = geom[12 * k:12 * k + 12].reshape(4, 3)
+    return g[0, 0], g[3, 0], g[0, 1], g[3, 1]   # xlo, xhi, ylo, yhi
+
+
+def random_forest(seed, n=4, ntx=2, nty=1, periodic=(1, 0), lmin=1, lmax=5, rounds=3):
+    rng = np.random.default_rng(seed)
+    f = amrhip.Forest(ntx, nty, periodic, -1.0, 0.5, 0.75, n, lmin, lmax)
+    for _ in range(rounds):
+        f.refine((rng.uniform(size=f.nquad) < 0.3).astype(np.int32))
+        f.balance()
+    return f
+
+
+def test_uniform_counts():
+    f = amrhip.Forest(3, 2, (1, 1), 0.0, 0.0, 1.0, 8, 3, 3)
+    assert f.nquad == 6 * 64
+    ci, fi, geom, soln = f.external()
+    fi = fi.reshape(-1, 11)
+    assert len(fi) == 2 * f.nquad and np.all(fi[:, 0] == 2)
+    assert np.array_equal(ci.reshape(-1, 3)[:, 2], 4 * 64 * np.arange(f.nquad))
+
+
+def test_geometry_closed_form():
+    f = amrhip.Forest(2, 3, (0, 0), -1.0, 0.5, 0.75, 2, 2, 2)
+    _, _, geom, _ = f.external()
+    for k, (tree, level, x, y) in enumerate(f.quads()):
+        tx, ty = tree % 2, tree // 2
+        size = 0.75 / 2 ** level
+        x0 = -1.0 + 0.75 * tx + 0.75 * x / ROOT_LEN
+        y0 = 0.5 + 0.75 * ty + 0.75 * y / ROOT_LEN
+        g = geom[12 * k:12 * k + 12].reshape(4, 3)
+        exp = np.array([[x0, y0], [x0 + size, y0], [x0, y0 + size], [x0 + size, y0 + size]])
+        assert np.allclose(g[:, :2], exp, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_balance_and_face_table(seed):
+    f = random_forest(seed)
+    assert f.is_balanced()
+    ci, fi, geom, _ = f.external()
+    ci = ci.reshape(-1, 3)
+    fi = fi.reshape(-1, 11)
+    lv = ci[:, 1]
+    seen = np.zeros((f.nquad, 4), int)
+    Lx, Ly = 2 * 0.75, 1 * 0.75
+    for face in fi:
+        t, s, c = face[0], face[1], face[2]
+        seen[c, s] += 1
+        if t == 1:
+            xlo, xhi, ylo, yhi = corners(geom, c)
+            # boundary faces only on non-periodic y sides
+            assert s in (2, 3)
+            assert np.isclose(ylo if s == 2 else yhi, 0.5 if s == 2 else 0.5 + Ly)
+            continue
+        nbs = [face[3]] if t == 2 else [face[3], face[4]]
+        assert np.all(face[7:7 + len(nbs)] == s ^ 1)
+        assert np.all(face[3 + len(nbs):7] == -1)
+        for j, d in enumerate(nbs):
+            seen[d, s ^ 1] += 1
+            assert lv[d] == lv[c] + (t == 3)
+            a = corners(geom, c)
+            b = corners(geom, d)
+            if s in (0, 1):
+                xa = a[1] if s == 1 else a[0]
+                xb = b[0] if s == 1 else b[1]
+                assert np.isclose((xa - xb) % Lx, 0) or np.isclose((xa - xb) % Lx, Lx)
+                assert a[2] - 1e-12 <= b[2] and b[3] <= a[3] + 1e-12
+            else:
+                ya = a[3] if s == 3 else a[2]
+                yb = b[2] if s == 3 else b[3]
+                assert np.isclose(ya, yb)
+                assert a[0] - 1e-12 <= b[0] and b[1] <= a[1] + 1e-12
+        if t == 3:
+            # hanging order: increasing coordinate along the face
+            b0, b1 = corners(geom, nbs[0]), corners(geom, nbs[1])
+            if s in (0, 1):
+                assert b0[2] < b1[2]
+            else:
+                assert b0[0] < b1[0]
+    assert np.all(seen == 1), "every quadrant side is described exactly once"
+
+
+def test_refine_copies_and_coarsen_averages(oracle):
+    import pyoracle
+    f = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 1.0, 4, 1, 3)
+    rng = np.random.default_rng(4)
+    s = f.soln()
+    s[:] = rng.uniform(0.5, 1.5, s.shape)
+    parent = s[2].copy()
+    flags = np.zeros(f.nquad, np.int32)
+    flags[2] = 1
+    f.refine(flags)
+    s = f.soln()
+    for c in range(4):
+        assert np.array_equal(s[2 + c], parent)
+    # perturb children, then coarsen the family (tags are 0)
+    s[2:6] = rng.uniform(0.5, 1.5, s[2:6].shape)
+    ch = [np.ascontiguousarray(s[2 + c].reshape(-1)) for c in range(4)]
+    exp = np.zeros(ch[0].size)
+    oracle.L.orc_coarsen_2d(4, *[pyoracle.dpt(c) for c in ch], pyoracle.dpt(exp))
+    tags = f.tags()
+    tags[:] = 1
+    tags[2:6] = 0
+    assert f.coarsen() == 1
+    assert f.nquad == 4
+    assert np.array_equal(f.soln()[2].reshape(-1), exp)
+
+
+def test_find_leaf_and_levels():
+    f = random_forest(9)
+    L = amrhip.lib()
+    for k, (tree, level, x, y) in enumerate(f.quads()):
+        assert L.amr_forest_find_leaf(f.h, tree, x, y) == k
+        size = ROOT_LEN >> level
+        assert L.amr_forest_find_leaf(f.h, tree, x + size - 1, y + size - 1) == k
+
+
+def test_regrid_refines_discontinuity():
+    ic = amrhip.riemann4_ic(0)
+    f = amrhip.Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, 8, 2, 5)
+    f.apply_ic(ic)
+    f.regrid_initial(ic, 0.1)
+    assert f.is_balanced()
+    lv = f.levels()
+    assert lv.max() == 5 and lv.min() >= 2
+    ci, fi, geom, _ = f.external()
+    # the finest quadrants touch the discontinuity lines x = 0.5 or y = 0.5
+    for k in np.nonzero(lv == 5)[0]:
+        xlo, xhi, ylo, yhi = corners(geom, k)
+        near = min(abs(xlo - 0.5), abs(xhi - 0.5), abs(ylo - 0.5), abs(yhi - 0.5))
+        assert near < 2 * (xhi - xlo) + 1e-12

@@ -1,0 +1,209 @@
+"""GPU parity: the HIP path through the C ABI against the reference's golden
+outputs and against the CPU oracle (which reproduces the reference bit for
+bit).  Bars: residual per cell and field within 1e-12 relative to
+max(|r_ref|, ||r_ref_field||_inf); CFL time step bit-exact; RK trajectories
+within 1e-11 of the oracle's k-form (the GPU uses the algebraically identical
+Shu-Osher / low-storage forms); conservation to round-off."""
+import os
+
+import numpy as np
+import pytest
+
+import amrhip
+from conftest import golden_meshes, parity_err
+
+pytestmark = pytest.mark.gpu
+
+RHS_TOL = 1e-12
+TRAJ_TOL = 1e-11
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    g = amrhip.Gpu()
+    yield g
+    g.close()
+
+
+def setup(g, n, scheme, h_base, bc, cfl, dt_cap, ci, fi, geom, soln, t_end=0.0):
+    g.setup(n, scheme, [amrhip.make_inst(h_base, bc, cfl, dt_cap, t_end)], ci, fi, geom, soln)
+
+
+@pytest.mark.parametrize("path", golden_meshes(), ids=lambda p: os.path.basename(p)[5:-4])
+def test_rhs_against_reference_golden(gpu, path):
+    m = np.load(path)
+    n = int(m["n"])
+    log(f"golden {os.path.basename(path)} n={n}")
+    setup(gpu, n, 3, float(m["h_base"]), tuple(int(b) for b in m["bc"]), float(m["cfl"]), 1.0,
+          m["cell_info"], m["face_info"], m["geom"], m["soln"])
+    err = parity_err(gpu.rhs(), m["rhs"])
+    assert err <= RHS_TOL, err
+    assert gpu.cfl()[0] == float(m["dt"])     # bit-exact CFL minimum
+
+
+def riemann_forest(n, seed, lmin=2, lmax=5, tol=0.1):
+    ic = amrhip.riemann4_ic(seed)
+    f = amrhip.Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, n, lmin, lmax)
+    f.apply_ic(ic)
+    f.regrid_initial(ic, tol)
+    return f
+
+
+def vortex_forest(n, level, extra=True):
+    f = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 10.0, n, level, level + 1)
+    if extra:
+        flags = np.zeros(f.nquad, np.int32)
+        flags[f.nquad // 3] = 1
+        flags[f.nquad // 2] = 1
+        f.refine(flags)
+        f.balance()
+    f.apply_ic(amrhip.vortex_ic(5.0, 5.0, 5.0))
+    return f
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16])
+@pytest.mark.parametrize("bc", [(0, 0, 0, 0), (1, 2, 1, 0)])
+def test_rhs_adaptive_forest(gpu, oracle, n, bc):
+    log(f"adaptive riemann n={n} bc={bc}")
+    f = riemann_forest(n, seed=n, lmax=4 if n == 16 else 5)
+    ci, fi, geom, soln = f.external()
+    assert np.any(fi.reshape(-1, 11)[:, 0] == 3)
+    setup(gpu, n, 3, 1.0, bc, 0.3, 1.0, ci, fi, geom, soln)
+    ref = oracle.rhs(oracle.params(n, bc=bc, h_base=1.0), ci, fi, geom, soln)
+    assert parity_err(gpu.rhs(), ref) <= RHS_TOL
+
+
+def test_rhs_config1_vortex(gpu, oracle):
+    """the isentropic vortex on the uniform level-4 forest with 8x8 patches"""
+    log("vortex level 4 n=8")
+    f = vortex_forest(8, 4, extra=False)
+    ci, fi, geom, soln = f.external()
+    assert f.nquad == 256
+    setup(gpu, 8, 3, 10.0, (0, 0, 0, 0), 0.3, 1.0, ci, fi, geom, soln)
+    ref = oracle.rhs(oracle.params(8, h_base=10.0), ci, fi, geom, soln)
+    assert parity_err(gpu.rhs(), ref) <= RHS_TOL
+    p = oracle.params(8, cfl=0.3, h_base=10.0)
+    assert gpu.cfl()[0] == oracle.time_step(p, ci, geom, soln, np.inf)
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4, 5])
+def test_rk_trajectory(gpu, oracle, scheme):
+    log(f"trajectory scheme {scheme}")
+    n = 8
+    f = vortex_forest(n, 2)
+    ci, fi, geom, soln = f.external()
+    cfl = 0.25 * 0.4 * scheme
+    dt_cap = 1.0 if scheme != 2 else 0.01
+    setup(gpu, n, scheme, 10.0, (0, 0, 0, 0), cfl, dt_cap, ci, fi, geom, soln)
+    nsteps = 10
+    gpu.run(nsteps)
+    q_gpu = gpu.soln()
+    p = oracle.params(n, cfl=cfl, dt=dt_cap, h_base=10.0, scheme=scheme)
+    q_ref, rn_ref, dt_ref = oracle.run(p, ci, fi, geom, soln, nsteps)
+    err = parity_err(q_gpu, q_ref)
+    assert err <= TRAJ_TOL, err
+    t, dt = gpu.time()
+    assert dt[0] == pytest.approx(dt_ref, rel=1e-12)
+
+
+def test_graph_and_direct_launch_identical(gpu):
+    f = riemann_forest(8, seed=3, lmax=4)
+    ci, fi, geom, soln = f.external()
+    setup(gpu, 8, 3, 1.0, (0, 1, 2, 0), 0.3, 1.0, ci, fi, geom, soln)
+    gpu.run(7, graph=True)
+    a = gpu.soln()
+    gpu.upload(soln)
+    gpu.set_time([0.0])
+    gpu.run(7, graph=False)
+    b = gpu.soln()
+    assert np.array_equal(a, b)
+
+
+def test_conservation_periodic(gpu):
+    n = 8
+    f = vortex_forest(n, 3)
+    ci, fi, geom, soln = f.external()
+    setup(gpu, n, 3, 10.0, (0, 0, 0, 0), 0.3, 1.0, ci, fi, geom, soln)
+    vol = ((10.0 / 2.0 ** f.levels()) / n) ** 2
+    def totals(s):
+        return np.einsum("q,qcf->f", vol, s.reshape(f.nquad, n * n, 4))
+    m0 = totals(soln)
+    gpu.run(50)
+    m1 = totals(gpu.soln())
+    assert np.all(np.abs(m1 - m0) <= 1e-12 * np.abs(m0))
+
+
+def test_batched_instances_match_single(gpu):
+    """two independent forests in one context give each forest's own result"""
+    ga = []
+    parts = []
+    for seed, bc in ((1, (0, 1, 2, 0)), (2, (1, 1, 1, 1))):
+        f = riemann_forest(8, seed=seed, lmax=4)
+        parts.append((f.external(), bc))
+    for (ci, fi, geom, soln), bc in parts:
+        g = amrhip.Gpu()
+        setup(g, 8, 3, 1.0, bc, 0.3, 1.0, ci, fi, geom, soln, t_end=0.01)
+        g.run(5)
+        ga.append((g.soln(), g.time()[0][0]))
+        g.close()
+    # concatenate
+    cis, fis, gs, ss, inst = [], [], [], [], []
+    off_c, off_d = 0, 0
+    for k, ((ci, fi, geom, soln), bc) in enumerate(parts):
+        c = ci.reshape(-1, 3).copy()
+        c[:, 2] += off_d
+        fr = fi.reshape(-1, 11).copy()
+        for col in range(2, 7):
+            fr[:, col] = np.where(fr[:, col] >= 0, fr[:, col] + off_c, -1)
+        cis.append(c.reshape(-1)); fis.append(fr.reshape(-1)); gs.append(geom); ss.append(soln)
+        inst.append(np.full(len(c), k, np.int32))
+        off_c += len(c)
+        off_d += soln.size
+    g = amrhip.Gpu()
+    g.setup(8, 3, [amrhip.make_inst(1.0, bc, 0.3, 1.0, 0.01) for (_, bc) in parts],
+            np.concatenate(cis), np.concatenate(fis), np.concatenate(gs), np.concatenate(ss),
+            cell_inst=np.concatenate(inst))
+    g.run(5)
+    out = g.soln()
+    t, _ = g.time()
+    g.close()
+    a = ga[0][0].size
+    assert np.array_equal(out[:a], ga[0][0])
+    assert np.array_equal(out[a:], ga[1][0])
+    assert t[0] == ga[0][1] and t[1] == ga[1][1]
+
+
+def test_physics_go_gpu_interval(oracle):
+    """the drop-in entry point: interval semantics and state vs the oracle"""
+    import ctypes as C
+    n = 8
+    f = vortex_forest(n, 2)
+    ci, fi, geom, soln = f.external()
+    amrhip.set_params(nelem_subgrid=n, time_scheme=3, time_steps=12, dt=1.0, cfl=0.3,
+                      coarse_grid_h=10.0, regrid_interval=5, checkpoint_interval=2 ** 31 - 1,
+                      visualization_interval=2 ** 31 - 1, bc=(0, 0, 0, 0, 1, 1), t_end=0.0)
+    L = amrhip.lib()
+    q = soln.copy()
+    ci32 = np.ascontiguousarray(ci, np.int32)
+    fi32 = np.ascontiguousarray(fi, np.int32)
+    ints = {k: C.c_int(v) for k, v in dict(dim=2, uns=0, nf=4, n=n, ncell=f.nquad, ngh=0,
+                                           nface=fi.size // 11, dof=q.size, ns=0, nr=0).items()}
+    istep, rg, vis, chk, ev = (C.c_int(0) for _ in range(5))
+    ev.value = 1
+    tt, ct, rt = C.c_double(0), C.c_double(0), C.c_double(0)
+    dummy = np.zeros(4, np.int32)
+    rc = L.physics_go_gpu(*(C.byref(ints[k]) for k in ("dim", "uns", "nf", "n", "ncell", "ngh", "nface",
+                                                       "dof")),
+                          amrhip.ipt(ci32), amrhip.ipt(fi32), C.byref(ints["ns"]), C.byref(ints["nr"]),
+                          amrhip.ipt(dummy), amrhip.ipt(dummy), amrhip.ipt(dummy),
+                          amrhip.dpt(geom), amrhip.dpt(q), C.byref(istep), C.byref(rg), C.byref(vis),
+                          C.byref(chk), C.byref(ev), None, C.byref(tt), C.byref(ct), C.byref(rt))
+    assert rc == 0
+    assert istep.value == 5 and rg.value == 1 and ev.value == 1
+    p = oracle.params(n, cfl=0.3, dt=1.0, h_base=10.0, scheme=3)
+    q_ref, _, _ = oracle.run(p, ci, fi, geom, soln, 5)
+    assert np.max(np.abs(q - q_ref) / np.maximum(np.abs(q_ref), 1.0)) <= TRAJ_TOL
