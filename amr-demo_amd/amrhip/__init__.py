@@ -46,6 +46,10 @@ EXPORTS = {
     "amrgpu_num_cells": (_i, [_vp]),
     "amrgpu_num_quads": (_i, [_vp]),
     "amrgpu_copy_probe": (_i, [C.c_longlong, _i, _pd]),
+    "amrgpu_set_tile": (_i, [_vp, _i, _i, _i]),
+    "amrgpu_get_tile": (_i, [_vp, _pi, _pi, _pi]),
+    "amrgpu_tile_variants": (_i, [_i, _pi, _pi, _pi, _i]),
+    "amrgpu_autotune": (_i, [_vp, _i, _pi, _pi, _pi, _pi, _pd]),
     # include/amr_forest.h
     "amr_forest_new": (_vp, [_i, _i, _i, _i, _d, _d, _d, _i, _i, _i]),
     "amr_forest_destroy": (None, [_vp]),
@@ -374,6 +378,20 @@ class Gpu:
         check(self.L.amrgpu_bench(self.h, nsteps, C.byref(ms), st, C.byref(ns)), "amrgpu_bench")
         return ms.value, [st[k] for k in range(ns.value)]
 
+    def set_tile(self, ppb, minw=1, design=0):
+        check(self.L.amrgpu_set_tile(self.h, ppb, minw, design), "amrgpu_set_tile")
+
+    def tile(self):
+        p, m, d = _i(), _i(), _i()
+        check(self.L.amrgpu_get_tile(self.h, C.byref(p), C.byref(m), C.byref(d)), "amrgpu_get_tile")
+        return p.value, m.value, d.value
+
+    def autotune(self, nsteps=20):
+        nv = _i()
+        ppb, mw, ds, ms = (_i * 32)(), (_i * 32)(), (_i * 32)(), (_d * 32)()
+        check(self.L.amrgpu_autotune(self.h, nsteps, C.byref(nv), ppb, mw, ds, ms), "amrgpu_autotune")
+        return [(ppb[k], mw[k], ds[k], ms[k]) for k in range(nv.value)]
+
     @property
     def ncells(self):
         return self.L.amrgpu_num_cells(self.h)
@@ -396,6 +414,12 @@ def get_params():
     p = PhysicsParams()
     lib().physics_gpu_get_params(C.byref(p))
     return p
+
+
+def tile_variants(n):
+    a, b, c = (_i * 32)(), (_i * 32)(), (_i * 32)()
+    k = lib().amrgpu_tile_variants(n, a, b, c, 32)
+    return [(a[i], b[i], c[i]) for i in range(min(k, 32))]
 
 
 def copy_probe(nbytes, reps=20):

@@ -2,7 +2,11 @@
  * amr_driver.c -- initial conditions, density-gradient indicator and the
  * regrid sequence on the host forest.  See include/amr_driver.h.
  */
+#define _POSIX_C_SOURCE 200809L
 #include "amr_driver.h"
+#include "amr_hip.h"
+
+#include <time.h>
 
 #include <math.h>
 #include <stdlib.h>
@@ -248,4 +252,60 @@ int amr_regrid_initial(amr_forest_t *f, const amr_ic_t *ic, double tol)
     for (int k = 0; k < nq; ++k) tag[k] = 0;
     (void)n;
     return nq;
+}
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int amr_evolve(amr_forest_t *f, double tol, int allow_coarsening, amr_evolve_stats_t *st)
+{
+    physics_gpu_params_t p;
+    physics_gpu_get_params(&p);
+    static const int stages[6] = {0, 1, 2, 3, 4, 5};
+    int istep = 0, evolve = 1, rc = 0;
+    double total = 0.0, comp = 0.0, res = 0.0;
+    amr_evolve_stats_t s;
+    memset(&s, 0, sizeof(s));
+    s.peak_quads = amr_forest_num_quads(f);
+    while (evolve) {
+        int regrid = 0, visualize = 0, checkpoint = 0;
+        int ncell, nghost, nface, dof, dim = 2, uns = 0, nfields = 4, n = amr_forest_nelem_subgrid(f);
+        int nsend = 0, nrecv = 0, dummy = 0;
+        int *cell_info, *face_info;
+        double *geom, *soln;
+        double t0 = now_s();
+        if (amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info,
+                                      &geom, &soln))
+            return AMRGPU_ERR_MESH;
+        double t1 = now_s();
+        int before = istep;
+        /* soln is the forest's own patch array: updated in place
+         * (external_p4est_to_soln / external_soln_to_p4est) */
+        rc = hpc_amr_evolve_solver(&dim, &uns, &nfields, &n, &ncell, &nghost, &nface, &dof,
+                                   cell_info, face_info, &nsend, &nrecv, &dummy, &dummy, &dummy,
+                                   geom, soln, &istep, &regrid, &visualize, &checkpoint, &evolve,
+                                   NULL, &total, &comp, &res);
+        double t2 = now_s();
+        s.wall_gpu += t2 - t1;
+        s.wall_regrid += t1 - t0;
+        if (rc) break;
+        int nst = istep - before;
+        s.steps += nst;
+        s.cell_stages += (double)nst * ncell * n * n * stages[p.time_scheme];
+        if (regrid && evolve) {
+            double t3 = now_s();
+            amr_regrid(f, tol, allow_coarsening);
+            s.wall_regrid += now_s() - t3;
+            s.regrids++;
+            if (amr_forest_num_quads(f) > s.peak_quads) s.peak_quads = amr_forest_num_quads(f);
+        }
+    }
+    physics_gpu_get_params(&p);
+    s.t = physics_gpu_time();
+    if (st) *st = s;
+    return rc;
 }

@@ -16,6 +16,7 @@
 // table nbr[quad][side][2]; per-quadrant level and instance id.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -30,6 +31,10 @@ namespace {
 
 constexpr int NFLD = 4;
 constexpr unsigned long long DBL_INF_BITS = 0x7ff0000000000000ull;
+// The CFL minimum of an instance is spread over NSLOT sub-slots (workgroup b
+// folds into sub-slot b % NSLOT): a single address would serialise one 64-bit
+// atomic per workgroup; the consumer reduces the sub-slots with one wave.
+constexpr int NSLOT = 64;
 
 struct InstDev {
     double h_base;
@@ -91,45 +96,79 @@ struct Prim {
     double r, ru, rv, rE, u, v, p, H, sr;
 };
 
+// 1/x from the hardware estimate (v_rcp_f64) refined by two Newton steps in
+// FMA form: within ~1 ulp of the correctly rounded quotient the reference
+// computes, at half the instructions of an IEEE divide.
+__device__ __forceinline__ double frcp(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// sqrt(x) and 1/sqrt(x) together from the hardware estimate (v_rsq_f64)
+// refined by two coupled Newton (Goldschmidt) steps and a final residual
+// correction of the root; both within ~1 ulp.  Arguments are positive
+// normal numbers (densities, squared sound speeds).
+__device__ __forceinline__ void fsqrt_rsq(double x, double &s, double &rs)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    s = fma(d, h, g);
+    rs = 2.0 * h;
+}
+
 __device__ __forceinline__ void make_prim(double r, double ru, double rv, double rE, Prim &P)
 {
-    // flux_2d_module.F90:93-99
-    double oneOr = 1.0 / r;
+    // flux_2d_module.F90:93-99, with sqrt(rho) and 1/rho from one rsq
+    double sr, rsr;
+    fsqrt_rsq(r, sr, rsr);
+    double oneOr = rsr * rsr;
     P.r = r; P.ru = ru; P.rv = rv; P.rE = rE;
     P.u = ru * oneOr;
     P.v = rv * oneOr;
     double E = rE * oneOr;
     P.p = (1.4 - 1.0) * (rE - 0.5 * r * (P.u * P.u + P.v * P.v));
     P.H = E + P.p * oneOr;
-    P.sr = sqrt(r);
+    P.sr = sr;
 }
 
 // HLLE-type flux of flux_2d_module.F90:63-182 on an axis-aligned face of
 // length h with unit normal +x (XDIR) or +y.  sqrt(rho_l*rho_r) is taken as
-// sqrt(rho_l)*sqrt(rho_r) and divisions by abar, (b1-b2) as products with
-// one reciprocal each: both stay within a few ulp of the reference.
+// sqrt(rho_l)*sqrt(rho_r), abar and 1/abar come from one rsq, and the
+// divisions become products with refined reciprocals: all within a few ulp
+// of the reference's correctly rounded operations.
 template <bool XDIR>
 __device__ __forceinline__ void flux_face(const Prim &L, const Prim &R, double h, double F[4])
 {
     double unl = XDIR ? L.u : L.v;
     double unr = XDIR ? R.u : R.v;
-    double inv_rf = 1.0 / (L.sr + R.sr);
+    double inv_rf = frcp(L.sr + R.sr);
     double rhobar = L.sr * R.sr;
     double ubar = (L.u * L.sr + R.u * R.sr) * inv_rf;
     double vbar = (L.v * L.sr + R.v * R.sr) * inv_rf;
     double Hbar = (L.H * L.sr + R.H * R.sr) * inv_rf;
-    double abar = sqrt(0.4 * (Hbar - 0.5 * (ubar * ubar + vbar * vbar)));
+    double abar, inva;
+    fsqrt_rsq(0.4 * (Hbar - 0.5 * (ubar * ubar + vbar * vbar)), abar, inva);
     double unbar = XDIR ? ubar : vbar;
-    double inva = 1.0 / abar;
     double Mn = unbar * inva;
     double eigp = unbar + abar, eigm = unbar - abar;
     double eigp_max = unr + abar, eigm_min = unl - abar;
     double b1 = (eigp > 0.0 || eigp_max > 0.0) ? fmax(eigp, eigp_max) : 0.0;
     double b2 = (eigm < 0.0 || eigm_min < 0.0) ? fmin(eigm, eigm_min) : 0.0;
-    double invb = 1.0 / (b1 - b2);
+    double invb = frcp(b1 - b2);
     double bcoeff = b1 * b2 * invb;
     double delta1 = (R.r - L.r) - (R.p - L.p) * (inva * inva);
-    double Mc = 1.0 / (1.0 + fabs(Mn));
+    double Mc = frcp(1.0 + fabs(Mn));
     double bM = bcoeff * Mc;
     double dH = R.H - L.H;
     double fl0, fr0, fl1, fr1, fl2, fr2, d1, d2;
@@ -188,7 +227,7 @@ __device__ __forceinline__ void flux_boundary(int bc, const double nrm[2], const
 
 // Per-cell CFL candidate, solver_time_step_2d_module.F90:107-116, evaluated
 // without FMA contraction so that it reproduces the reference bit for bit.
-__device__ __attribute__((noinline)) double cfl_candidate(const double q[4], double cfl, double vol)
+__device__ __forceinline__ double cfl_candidate(const double q[4], double cfl, double vol)
 {
 #pragma clang fp contract(off)
     double oneOrho = 1.0 / q[0];
@@ -215,34 +254,32 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // Block-wide min of the CFL candidates, then one 64-bit atomicMin per
-// instance (positive doubles order like their bit patterns).  Blocks that
-// straddle two instances fall back to per-thread atomics.
+// instance (positive doubles order like their bit patterns).  uinst is the
+// block's common instance, or -1 when the block straddles two instances
+// (then every thread issues its own atomic).
 template <int BLK>
-__device__ void cfl_reduce(double cand, int inst, bool active, unsigned long long *slot,
-                           double *red, int *flag)
+__device__ void cfl_reduce(double cand, int inst, bool active, int uinst,
+                           unsigned long long *slot, double *red)
 {
     const int tid = threadIdx.x;
-    if (tid == 0) flag[0] = -1;
-    __syncthreads();
-    if (active) {
-        int cur = atomicCAS(&flag[0], -1, inst);
-        if (cur != -1 && cur != inst) flag[0] = -2;
-    }
-    __syncthreads();
-    int f = flag[0];
-    if (f == -2) {
-        if (active) atomicMin(&slot[inst], d2bits(cand));
+    if (uinst < 0) {
+        if (active) atomicMin(&slot[inst * NSLOT + (blockIdx.x & (NSLOT - 1))], d2bits(cand));
         return;
     }
     double v = active ? cand : INFINITY;
     v = wave_min(v);
     constexpr int NW = (BLK + 63) / 64;
+    if (NW == 1) {
+        if (tid == 0) atomicMin(&slot[uinst * NSLOT + (blockIdx.x & (NSLOT - 1))], d2bits(v));
+        return;
+    }
+    __syncthreads();
     if ((tid & 63) == 0) red[tid >> 6] = v;
     __syncthreads();
-    if (tid == 0 && f >= 0) {
+    if (tid == 0) {
         double m = red[0];
         for (int w = 1; w < NW; ++w) m = fmin(m, red[w]);
-        atomicMin(&slot[f], d2bits(m));
+        atomicMin(&slot[uinst * NSLOT + (blockIdx.x & (NSLOT - 1))], d2bits(m));
     }
 }
 
@@ -262,16 +299,31 @@ __device__ void sum_reduce_store(double v, double *out, double *red)
     }
 }
 
-__device__ __forceinline__ double stage_dt(const StageArgs &a, int inst)
+// CFL minimum of instance inst over its sub-slots; called by a whole wave
+// (every lane returns the result).
+__device__ __forceinline__ double slot_min(const unsigned long long *slot, int inst)
 {
-    const InstDev &ip = a.m.ip[inst];
+    const int lane = threadIdx.x & 63;
+    const unsigned long long *s = slot + (size_t)inst * NSLOT;
+    double m = INFINITY;
+#pragma unroll
+    for (int k = 0; k < (NSLOT + 63) / 64; ++k)
+        if (lane + 64 * k < NSLOT) m = fmin(m, bits2d(s[lane + 64 * k]));
+    return wave_min(m);
+}
+
+// dt of the step for instance inst: min(input dt, CFL minimum, time left);
+// called by a whole wave.
+__device__ __forceinline__ double step_dt(const InstDev &ip, const unsigned long long *slot,
+                                          const double *t_inst, int inst, int use_cfl)
+{
     double dt = ip.dt_cap;
-    if (a.use_cfl) {
-        double c = bits2d(a.slot[inst]);
+    if (use_cfl) {
+        double c = slot_min(slot, inst);
         if (c < dt) dt = c;
     }
     if (ip.t_end > 0.0) {
-        double rem = ip.t_end - a.t_inst[inst];
+        double rem = ip.t_end - t_inst[inst];
         if (rem < 0.0) rem = 0.0;
         if (rem < dt) dt = rem;
     }
@@ -294,8 +346,12 @@ struct Tile {
     static constexpr int BLK = N2 * PPB;
     static constexpr int T = N + 2;
     static constexpr int TT = T * T;
-    static constexpr int NP = 9;                           // Prim fields
-    static constexpr int PRIM_PER_PATCH = NP * TT + 4 * N * NP;
+    // LDS record per tile position: r, ru, rv, rE, p, 1/r, sqrt(r).  Tile
+    // positions [0, TT) are the patch with its ghost ring, [TT, TT + 4N) the
+    // second fine neighbour of each coarse side of a hanging face.
+    static constexpr int NP = 7;
+    static constexpr int NPOS = TT + 4 * N;
+    static constexpr int PRIM_PER_PATCH = NP * NPOS;
     static constexpr int FLUX_PER_PATCH = NFLD * 2 * N * (N + 1);
     static constexpr int LDS_DOUBLES =
         PPB * (PRIM_PER_PATCH > FLUX_PER_PATCH ? PRIM_PER_PATCH : FLUX_PER_PATCH);
@@ -306,41 +362,45 @@ struct Tile {
 };
 
 template <int N, int PPB>
-__device__ __forceinline__ void store_prim(double *lds, int p, int t, const Prim &P)
+__device__ __forceinline__ void store_rec(double *lds, int p, int t, double r, double ru, double rv,
+                                          double rE)
 {
     using TL = Tile<N, PPB>;
-    double *b = lds + p * TL::PRIM_PER_PATCH;
-    b[0 * TL::TT + t] = P.r;  b[1 * TL::TT + t] = P.ru; b[2 * TL::TT + t] = P.rv;
-    b[3 * TL::TT + t] = P.rE; b[4 * TL::TT + t] = P.u;  b[5 * TL::TT + t] = P.v;
-    b[6 * TL::TT + t] = P.p;  b[7 * TL::TT + t] = P.H;  b[8 * TL::TT + t] = P.sr;
+    double sr, rsr;
+    fsqrt_rsq(r, sr, rsr);
+    const double ir = rsr * rsr;
+    const double u = ru * ir, v = rv * ir;
+    // flux_2d_module.F90:98 pressure
+    const double pr = (1.4 - 1.0) * (rE - 0.5 * r * (u * u + v * v));
+    double *b = lds + p * TL::PRIM_PER_PATCH + t;
+    b[0 * TL::NPOS] = r;  b[1 * TL::NPOS] = ru; b[2 * TL::NPOS] = rv; b[3 * TL::NPOS] = rE;
+    b[4 * TL::NPOS] = pr; b[5 * TL::NPOS] = ir; b[6 * TL::NPOS] = sr;
 }
+
 template <int N, int PPB>
 __device__ __forceinline__ void load_prim(const double *lds, int p, int t, Prim &P)
 {
     using TL = Tile<N, PPB>;
-    const double *b = lds + p * TL::PRIM_PER_PATCH;
-    P.r = b[0 * TL::TT + t];  P.ru = b[1 * TL::TT + t]; P.rv = b[2 * TL::TT + t];
-    P.rE = b[3 * TL::TT + t]; P.u = b[4 * TL::TT + t];  P.v = b[5 * TL::TT + t];
-    P.p = b[6 * TL::TT + t];  P.H = b[7 * TL::TT + t];  P.sr = b[8 * TL::TT + t];
-}
-// second fine neighbour of side s, index k
-template <int N, int PPB>
-__device__ __forceinline__ double *h2_ptr(double *lds, int p, int s, int k)
-{
-    using TL = Tile<N, PPB>;
-    return lds + p * TL::PRIM_PER_PATCH + TL::NP * TL::TT + (s * N + k) * TL::NP;
+    const double *b = lds + p * TL::PRIM_PER_PATCH + t;
+    P.r = b[0 * TL::NPOS]; P.ru = b[1 * TL::NPOS]; P.rv = b[2 * TL::NPOS]; P.rE = b[3 * TL::NPOS];
+    P.p = b[4 * TL::NPOS];
+    const double ir = b[5 * TL::NPOS];
+    P.sr = b[6 * TL::NPOS];
+    P.u = P.ru * ir;
+    P.v = P.rv * ir;
+    P.H = (P.rE + P.p) * ir;     // E + p/rho
 }
 
-template <int N, int PPB>
-__global__ __launch_bounds__((Tile<N, PPB>::BLK)) void stage_kernel(StageArgs a)
+template <int N, int PPB, int MINW, int DESIGN>
+__global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(StageArgs a)
 {
     using TL = Tile<N, PPB>;
     constexpr int N2 = TL::N2, T = TL::T;
     __shared__ double lds[TL::LDS_DOUBLES];
     __shared__ double red[(TL::BLK + 63) / 64];
-    __shared__ int flag[1];
     __shared__ int pquad[PPB], plevel[PPB], pinst[PPB];
     __shared__ int pnbr[PPB][8];
+    __shared__ double pdt[PPB];
 
     const int tid = threadIdx.x;
     const int nblk = gridDim.x;
@@ -349,7 +409,9 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK)) void stage_kernel(StageArgs a)
     const int ci = c % N, cj = c / N;
     const int quad = lb * PPB + p;
     const bool active = quad < a.m.nquad;
+    const size_t base = (size_t)quad * NFLD * N2 + c;
 
+    // ---- prologue: every independent global load is issued here ----------
     if (tid < PPB) {
         int qq = lb * PPB + tid;
         bool act = qq < a.m.nquad;
@@ -359,151 +421,262 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK)) void stage_kernel(StageArgs a)
 #pragma unroll
         for (int k = 0; k < 8; ++k) pnbr[tid][k] = act ? a.m.nbr[qq * 8 + k] : -1;
     }
-    // own state (coalesced: consecutive threads -> consecutive cells)
+    if ((a.role & ROLE_DT) && tid < 64) {
+        // first stage of a step: wave 0 derives the dt of the block's
+        // instances from the CFL sub-slots and publishes it
+        int last_inst = -1;
+        double last_dt = 0.0;
+        for (int q = 0; q < PPB; ++q) {
+            int qq = lb * PPB + q;
+            if (qq >= a.m.nquad) break;
+            int in = a.m.inst[qq];
+            if (in != last_inst) {
+                last_dt = step_dt(a.m.ip[in], a.slot, a.t_inst, in, a.use_cfl);
+                last_inst = in;
+                if ((tid & 63) == 0) a.dt_cur[in] = last_dt;   // identical values from every writer
+            }
+            if ((tid & 63) == 0) pdt[q] = last_dt;
+        }
+    }
     double U[4] = {1.0, 0.0, 0.0, 1.0};
+    double B[4] = {0.0, 0.0, 0.0, 0.0};     // base state U0 of the step
+    const bool needB = (a.mode == MODE_SO && a.U0) || a.mode == MODE_RK4_ACC || a.mode == MODE_RK4_FINAL;
     if (active) {
-        const double *src = a.Us + (size_t)quad * NFLD * N2 + c;
 #pragma unroll
-        for (int f = 0; f < 4; ++f) U[f] = src[f * N2];
+        for (int f = 0; f < 4; ++f) U[f] = a.Us[base + f * N2];
+        if (needB) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) B[f] = a.U0[base + f * N2];
+        }
     }
-    __syncthreads();
-    {
-        Prim P;
-        make_prim(U[0], U[1], U[2], U[3], P);
-        if (active) store_prim<N, PPB>(lds, p, (cj + 1) * T + (ci + 1), P);
-    }
-    // ghost ring
+    // ghost ring: each thread fetches its neighbour ids itself, so the halo
+    // loads do not wait for the block barrier
     for (int hh = c; hh < 4 * N; hh += N2) {
         if (!active) break;
         int s = hh / N, k = hh % N;
-        int na = pnbr[p][2 * s], nb = pnbr[p][2 * s + 1];
-        if (na < 0) continue;
-        int ec, tpos;
+        int na = a.m.nbr[quad * 8 + 2 * s], nb = a.m.nbr[quad * 8 + 2 * s + 1];
+        int ec, oc, tpos;   // neighbour's edge cell, own edge cell, ghost position
         switch (s) {
-        case 0: ec = k * N + (N - 1); tpos = (k + 1) * T + 0; break;
-        case 1: ec = k * N + 0;       tpos = (k + 1) * T + (N + 1); break;
-        case 2: ec = (N - 1) * N + k; tpos = 0 * T + (k + 1); break;
-        default: ec = 0 * N + k;      tpos = (N + 1) * T + (k + 1); break;
+        case 0: ec = k * N + (N - 1); oc = k * N;           tpos = (k + 1) * T + 0; break;
+        case 1: ec = k * N + 0;       oc = k * N + (N - 1); tpos = (k + 1) * T + (N + 1); break;
+        case 2: ec = (N - 1) * N + k; oc = k;               tpos = 0 * T + (k + 1); break;
+        default: ec = 0 * N + k;      oc = (N - 1) * N + k; tpos = (N + 1) * T + (k + 1); break;
         }
-        const double *sa = a.Us + (size_t)na * NFLD * N2 + ec;
-        Prim P;
-        make_prim(sa[0], sa[N2], sa[2 * N2], sa[3 * N2], P);
-        store_prim<N, PPB>(lds, p, tpos, P);
+        // physical boundary: the ghost holds the own edge cell, so that the
+        // generic face evaluation stays finite before the boundary flux
+        // replaces it
+        const double *sa = na < 0 ? a.Us + (size_t)quad * NFLD * N2 + oc
+                                  : a.Us + (size_t)na * NFLD * N2 + ec;
+        double q0 = sa[0], q1 = sa[N2], q2 = sa[2 * N2], q3 = sa[3 * N2];
         if (nb >= 0) {
             const double *sb = a.Us + (size_t)nb * NFLD * N2 + ec;
-            Prim Q;
-            make_prim(sb[0], sb[N2], sb[2 * N2], sb[3 * N2], Q);
-            double *d = h2_ptr<N, PPB>(lds, p, s, k);
-            d[0] = Q.r; d[1] = Q.ru; d[2] = Q.rv; d[3] = Q.rE; d[4] = Q.u;
-            d[5] = Q.v; d[6] = Q.p; d[7] = Q.H; d[8] = Q.sr;
+            double r0 = sb[0], r1 = sb[N2], r2 = sb[2 * N2], r3 = sb[3 * N2];
+            store_rec<N, PPB>(lds, p, TL::TT + s * N + k, r0, r1, r2, r3);
         }
+        store_rec<N, PPB>(lds, p, tpos, q0, q1, q2, q3);
     }
+    if (active) store_rec<N, PPB>(lds, p, (cj + 1) * T + (ci + 1), U[0], U[1], U[2], U[3]);
     __syncthreads();
 
     // ---- face fluxes into registers --------------------------------------
-    double Fr[TL::ROUNDS][4];
-    int Fdst[TL::ROUNDS];     // packed destination: patch, x/y, index
-#pragma unroll
-    for (int r = 0; r < TL::ROUNDS; ++r) {
-        Fdst[r] = -1;
-        int task = tid + r * TL::BLK;
-        if (task >= TL::NTASK) continue;
-        int tp, isx, fi, fj;   // patch; x-face (fi in 0..N, row fj) or y-face (col fi, fj in 0..N)
-        bool edge;
-        int side = -1;
-        if (task < PPB * TL::NINT) {
-            tp = task / TL::NINT;
-            int t = task % TL::NINT;
-            edge = false;
-            if (t < N * (N - 1)) { isx = 1; fi = 1 + t % (N - 1); fj = t / (N - 1); }
-            else { t -= N * (N - 1); isx = 0; fi = t % N; fj = 1 + t / N; }
-        } else {
-            int t = task - PPB * TL::NINT;
-            tp = t / TL::NEDGE;
-            int e = t % TL::NEDGE;
-            edge = true;
-            side = e / N;
-            int k = e % N;
-            switch (side) {
-            case 0: isx = 1; fi = 0; fj = k; break;
-            case 1: isx = 1; fi = N; fj = k; break;
-            case 2: isx = 0; fi = k; fj = 0; break;
-            default: isx = 0; fi = k; fj = N; break;
+    if constexpr (DESIGN == 1) {
+        // paired faces: every cell evaluates its +x and +y faces from one
+        // load of its own record (two independent flux chains per thread);
+        // one more round covers the -x and -y patch edges
+        double FA[4] = {0, 0, 0, 0}, FB[4] = {0, 0, 0, 0}, FC[4] = {0, 0, 0, 0};
+        int dC = -1;
+        if (active) {
+            const double h = ldexp(a.m.ip[pinst[p]].h_base, -plevel[p]) / (double)N;
+            const int na1 = pnbr[p][2], nb1 = pnbr[p][3], na3 = pnbr[p][6], nb3 = pnbr[p][7];
+            const bool ex = (ci == N - 1), ey = (cj == N - 1);
+            const double hx = (ex && nb1 >= 0) ? 0.5 * h : h;
+            const double hy = (ey && nb3 >= 0) ? 0.5 * h : h;
+            const int t0 = (cj + 1) * T + (ci + 1);
+            Prim L, RX, RY;
+            load_prim<N, PPB>(lds, p, t0, L);
+            load_prim<N, PPB>(lds, p, t0 + 1, RX);
+            load_prim<N, PPB>(lds, p, t0 + T, RY);
+            flux_face<true>(L, RX, hx, FA);
+            flux_face<false>(L, RY, hy, FB);
+            if (ex && (na1 < 0 || nb1 >= 0)) {
+                if (na1 < 0) {
+                    const double nrm[2] = {h, 0.0};
+                    const double q[4] = {L.r, L.ru, L.rv, L.rE};
+                    flux_boundary(a.m.ip[pinst[p]].bc[1], nrm, q, FA);
+                } else {
+                    Prim F2;
+                    load_prim<N, PPB>(lds, p, TL::TT + 1 * N + cj, F2);
+                    double G[4];
+                    flux_face<true>(L, F2, hx, G);
+                    FA[0] += G[0]; FA[1] += G[1]; FA[2] += G[2]; FA[3] += G[3];
+                }
+            }
+            if (ey && (na3 < 0 || nb3 >= 0)) {
+                if (na3 < 0) {
+                    const double nrm[2] = {0.0, h};
+                    const double q[4] = {L.r, L.ru, L.rv, L.rE};
+                    flux_boundary(a.m.ip[pinst[p]].bc[3], nrm, q, FB);
+                } else {
+                    Prim F2;
+                    load_prim<N, PPB>(lds, p, TL::TT + 3 * N + ci, F2);
+                    double G[4];
+                    flux_face<false>(L, F2, hy, G);
+                    FB[0] += G[0]; FB[1] += G[1]; FB[2] += G[2]; FB[3] += G[3];
+                }
             }
         }
-        if (pquad[tp] < 0) continue;
-        const double h = ldexp(a.m.ip[pinst[tp]].h_base, -plevel[tp]) / (double)N;
-        double F[4];
-        // tile coordinates of the low (L) and high (R) cell of the face
-        int tl, tr;
-        if (isx) { tl = (fj + 1) * T + fi; tr = tl + 1; }
-        else     { tl = fj * T + (fi + 1); tr = tl + T; }
-        if (!edge) {
-            Prim L, R;
-            load_prim<N, PPB>(lds, tp, tl, L);
-            load_prim<N, PPB>(lds, tp, tr, R);
-            if (isx) flux_face<true>(L, R, h, F); else flux_face<false>(L, R, h, F);
-        } else {
-            int na = pnbr[tp][2 * side], nb = pnbr[tp][2 * side + 1];
-            bool lowside = (side == 0 || side == 2);
-            int town = lowside ? tr : tl;
-            if (na < 0) {
-                // physical boundary: outward normal; the face value is stored
-                // with the +x/+y orientation of the interior fluxes
-                int bc = a.m.ip[pinst[tp]].bc[side];
-                double nrm[2] = {0.0, 0.0};
-                if (side == 0) nrm[0] = -h; else if (side == 1) nrm[0] = h;
-                else if (side == 2) nrm[1] = -h; else nrm[1] = h;
-                const double *b = lds + tp * TL::PRIM_PER_PATCH;
-                double q[4] = {b[0 * TL::TT + town], b[1 * TL::TT + town],
-                               b[2 * TL::TT + town], b[3 * TL::TT + town]};
-                flux_boundary(bc, nrm, q, F);
-                if (lowside) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; F[3] = -F[3]; }
-            } else if (nb < 0) {
+        __builtin_amdgcn_sched_barrier(0);
+        if (tid < PPB * 2 * N) {
+            const int tp = tid / (2 * N), e = tid % (2 * N);
+            if (pquad[tp] >= 0) {
+                const double h = ldexp(a.m.ip[pinst[tp]].h_base, -plevel[tp]) / (double)N;
+                const bool xs = e < N;                 // side 0 (x) or side 2 (y)
+                const int k = xs ? e : e - N, side = xs ? 0 : 2;
+                const int na = pnbr[tp][2 * side], nb = pnbr[tp][2 * side + 1];
+                const double hf = nb >= 0 ? 0.5 * h : h;
+                const int tl = xs ? (k + 1) * T : (k + 1);
+                const int tr = xs ? tl + 1 : tl + T;
+                Prim L, R;
+                load_prim<N, PPB>(lds, tp, tl, L);
+                load_prim<N, PPB>(lds, tp, tr, R);
+                if (xs) flux_face<true>(L, R, hf, FC); else flux_face<false>(L, R, hf, FC);
+                if (na < 0) {
+                    const double nrm[2] = {xs ? -h : 0.0, xs ? 0.0 : -h};
+                    const double q[4] = {R.r, R.ru, R.rv, R.rE};
+                    flux_boundary(a.m.ip[pinst[tp]].bc[side], nrm, q, FC);
+                    FC[0] = -FC[0]; FC[1] = -FC[1]; FC[2] = -FC[2]; FC[3] = -FC[3];
+                } else if (nb >= 0) {
+                    Prim F2;
+                    load_prim<N, PPB>(lds, tp, TL::TT + side * N + k, F2);
+                    double G[4];
+                    if (xs) flux_face<true>(F2, R, hf, G); else flux_face<false>(F2, R, hf, G);
+                    FC[0] += G[0]; FC[1] += G[1]; FC[2] += G[2]; FC[3] += G[3];
+                }
+                dC = tp * TL::FLUX_PER_PATCH + (xs ? k * (N + 1) : NFLD * N * (N + 1) + k);
+            }
+        }
+        __syncthreads();   // all reads of the primitive tile are done
+        const int stride = N * (N + 1);
+        if (active) {
+            const int dA = p * TL::FLUX_PER_PATCH + cj * (N + 1) + ci + 1;
+            const int dB = p * TL::FLUX_PER_PATCH + NFLD * N * (N + 1) + (cj + 1) * N + ci;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                lds[dA + f * stride] = FA[f];
+                lds[dB + f * stride] = FB[f];
+            }
+        }
+        if (dC >= 0) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) lds[dC + f * stride] = FC[f];
+        }
+        __syncthreads();
+    } else {
+        double Fr[TL::ROUNDS][4];
+        int Fdst[TL::ROUNDS];     // LDS destination of the face value
+    #pragma unroll
+        for (int r = 0; r < TL::ROUNDS; ++r) {
+            Fdst[r] = -1;
+            int task = tid + r * TL::BLK;
+            if (task >= TL::NTASK) continue;
+            int tp, isx, fi, fj;   // patch; x-face (fi in 0..N, row fj) or y-face (col fi, fj in 0..N)
+            bool edge;
+            int side = -1;
+            if (task < PPB * TL::NINT) {
+                tp = task / TL::NINT;
+                int t = task % TL::NINT;
+                edge = false;
+                if (t < N * (N - 1)) { isx = 1; fi = 1 + t % (N - 1); fj = t / (N - 1); }
+                else { t -= N * (N - 1); isx = 0; fi = t % N; fj = 1 + t / N; }
+            } else {
+                int t = task - PPB * TL::NINT;
+                tp = t / TL::NEDGE;
+                int e = t % TL::NEDGE;
+                edge = true;
+                side = e / N;
+                int k = e % N;
+                switch (side) {
+                case 0: isx = 1; fi = 0; fj = k; break;
+                case 1: isx = 1; fi = N; fj = k; break;
+                case 2: isx = 0; fi = k; fj = 0; break;
+                default: isx = 0; fi = k; fj = N; break;
+                }
+            }
+            if (pquad[tp] < 0) continue;
+            const double h = ldexp(a.m.ip[pinst[tp]].h_base, -plevel[tp]) / (double)N;
+            double F[4];
+            // tile positions of the low (L) and high (R) cell of the face
+            int tl, tr;
+            if (isx) { tl = (fj + 1) * T + fi; tr = tl + 1; }
+            else     { tl = fj * T + (fi + 1); tr = tl + T; }
+            if (!edge) {
                 Prim L, R;
                 load_prim<N, PPB>(lds, tp, tl, L);
                 load_prim<N, PPB>(lds, tp, tr, R);
                 if (isx) flux_face<true>(L, R, h, F); else flux_face<false>(L, R, h, F);
             } else {
-                // coarse side of a hanging face: sub-face k couples to sub-face
-                // k of both fine neighbours, normal length of the fine level
-                // (solver_residual_face_full_hang_structured_2d.F90:86-122)
-                const double hf = 0.5 * h;
-                int k = isx ? fj : fi;
-                Prim own, f1, f2;
-                load_prim<N, PPB>(lds, tp, town, own);
-                load_prim<N, PPB>(lds, tp, lowside ? tl : tr, f1);
-                const double *d = h2_ptr<N, PPB>(lds, tp, side, k);
-                f2.r = d[0]; f2.ru = d[1]; f2.rv = d[2]; f2.rE = d[3]; f2.u = d[4];
-                f2.v = d[5]; f2.p = d[6]; f2.H = d[7]; f2.sr = d[8];
-                double G[4];
-                if (lowside) {
-                    if (isx) { flux_face<true>(f1, own, hf, F); flux_face<true>(f2, own, hf, G); }
-                    else     { flux_face<false>(f1, own, hf, F); flux_face<false>(f2, own, hf, G); }
+                int na = pnbr[tp][2 * side], nb = pnbr[tp][2 * side + 1];
+                bool lowside = (side == 0 || side == 2);
+                int town = lowside ? tr : tl;
+                if (na < 0) {
+                    // physical boundary: outward normal; the face value is stored
+                    // with the +x/+y orientation of the interior fluxes
+                    int bc = a.m.ip[pinst[tp]].bc[side];
+                    double nrm[2] = {0.0, 0.0};
+                    if (side == 0) nrm[0] = -h; else if (side == 1) nrm[0] = h;
+                    else if (side == 2) nrm[1] = -h; else nrm[1] = h;
+                    const double *b = lds + tp * TL::PRIM_PER_PATCH + town;
+                    double q[4] = {b[0], b[TL::NPOS], b[2 * TL::NPOS], b[3 * TL::NPOS]};
+                    flux_boundary(bc, nrm, q, F);
+                    if (lowside) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; F[3] = -F[3]; }
+                } else if (nb < 0) {
+                    Prim L, R;
+                    load_prim<N, PPB>(lds, tp, tl, L);
+                    load_prim<N, PPB>(lds, tp, tr, R);
+                    if (isx) flux_face<true>(L, R, h, F); else flux_face<false>(L, R, h, F);
                 } else {
-                    if (isx) { flux_face<true>(own, f1, hf, F); flux_face<true>(own, f2, hf, G); }
-                    else     { flux_face<false>(own, f1, hf, F); flux_face<false>(own, f2, hf, G); }
+                    // coarse side of a hanging face: sub-face k couples to sub-face
+                    // k of both fine neighbours, normal length of the fine level
+                    // (solver_residual_face_full_hang_structured_2d.F90:86-122)
+                    const double hf = 0.5 * h;
+                    int k = isx ? fj : fi;
+                    Prim own;
+                    load_prim<N, PPB>(lds, tp, town, own);
+                    F[0] = F[1] = F[2] = F[3] = 0.0;
+                    for (int q = 0; q < 2; ++q) {
+                        // one fine neighbour at a time keeps a single flux live
+                        Prim fn;
+                        load_prim<N, PPB>(lds, tp, q == 0 ? (lowside ? tl : tr) : TL::TT + side * N + k, fn);
+                        double G[4];
+                        if (lowside) {
+                            if (isx) flux_face<true>(fn, own, hf, G); else flux_face<false>(fn, own, hf, G);
+                        } else {
+                            if (isx) flux_face<true>(own, fn, hf, G); else flux_face<false>(own, fn, hf, G);
+                        }
+                        F[0] += G[0]; F[1] += G[1]; F[2] += G[2]; F[3] += G[3];
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
                 }
-                F[0] += G[0]; F[1] += G[1]; F[2] += G[2]; F[3] += G[3];
             }
+    #pragma unroll
+            for (int f = 0; f < 4; ++f) Fr[r][f] = F[f];
+            // FX[p][f][fj][fi] (fi in 0..N) ; FY[p][f][fj][fi] (fj in 0..N)
+            Fdst[r] = tp * TL::FLUX_PER_PATCH + (isx ? (fj * (N + 1) + fi)
+                                                      : (NFLD * N * (N + 1) + fj * N + fi));
+            // keep the rounds apart: interleaving three flux evaluations buys
+            // little ILP and costs the registers that bound occupancy
+            __builtin_amdgcn_sched_barrier(0);
         }
-#pragma unroll
-        for (int f = 0; f < 4; ++f) Fr[r][f] = F[f];
-        // FX[p][f][fj][fi] (fi in 0..N) ; FY[p][f][fj][fi] (fj in 0..N)
-        Fdst[r] = tp * TL::FLUX_PER_PATCH + (isx ? (fj * (N + 1) + fi)
-                                                  : (NFLD * N * (N + 1) + fj * N + fi));
+        __syncthreads();   // all reads of the primitive tile are done
+    #pragma unroll
+        for (int r = 0; r < TL::ROUNDS; ++r) {
+            if (Fdst[r] < 0) continue;
+            const int stride = N * (N + 1);
+    #pragma unroll
+            for (int f = 0; f < 4; ++f) lds[Fdst[r] + f * stride] = Fr[r][f];
+        }
+        __syncthreads();
     }
-    __syncthreads();   // all reads of the primitive tile are done
-#pragma unroll
-    for (int r = 0; r < TL::ROUNDS; ++r) {
-        if (Fdst[r] < 0) continue;
-        const int isy = (Fdst[r] % TL::FLUX_PER_PATCH) >= NFLD * N * (N + 1);
-        const int stride = N * (N + 1);
-        (void)isy;
-#pragma unroll
-        for (int f = 0; f < 4; ++f) lds[Fdst[r] + f * stride] = Fr[r][f];
-    }
-    __syncthreads();
 
     // ---- per-cell residual and stage update ------------------------------
     double k4[4] = {0.0, 0.0, 0.0, 0.0};
@@ -524,16 +697,14 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK)) void stage_kernel(StageArgs a)
             k4[f] = Rf * ninvV;
             sq += k4[f] * k4[f];
         }
-        const size_t base = (size_t)quad * NFLD * N2 + c;
         const int inst = pinst[p];
         double dt = 0.0;
-        if (a.mode != MODE_RHS) dt = (a.role & ROLE_DT) ? stage_dt(a, inst) : a.dt_cur[inst];
+        if (a.mode != MODE_RHS) dt = (a.role & ROLE_DT) ? pdt[p] : a.dt_cur[inst];
         switch (a.mode) {
         case MODE_SO:
             if (a.U0) {
 #pragma unroll
-                for (int f = 0; f < 4; ++f)
-                    out[f] = a.c0 * a.U0[base + f * N2] + a.c1 * U[f] + (a.c2 * dt) * k4[f];
+                for (int f = 0; f < 4; ++f) out[f] = a.c0 * B[f] + a.c1 * U[f] + (a.c2 * dt) * k4[f];
             } else {
 #pragma unroll
                 for (int f = 0; f < 4; ++f) out[f] = a.c1 * U[f] + (a.c2 * dt) * k4[f];
@@ -544,13 +715,13 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK)) void stage_kernel(StageArgs a)
             for (int f = 0; f < 4; ++f) {
                 double acc = a.first ? k4[f] : a.Acc[base + f * N2] + 2.0 * k4[f];
                 a.Acc[base + f * N2] = acc;
-                out[f] = a.U0[base + f * N2] + (a.c2 * dt) * k4[f];
+                out[f] = B[f] + (a.c2 * dt) * k4[f];
             }
             break;
         case MODE_RK4_FINAL:
 #pragma unroll
             for (int f = 0; f < 4; ++f)
-                out[f] = a.U0[base + f * N2] + (dt / 6.0) * (a.Acc[base + f * N2] + k4[f]);
+                out[f] = B[f] + (dt / 6.0) * (a.Acc[base + f * N2] + k4[f]);
             break;
         case MODE_LSERK:
 #pragma unroll
@@ -572,15 +743,16 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK)) void stage_kernel(StageArgs a)
     // time bookkeeping: stage 2 advances t by this step's dt and re-arms the
     // CFL slot that the last stage fills
     if ((a.role & ROLE_TIME) && blockIdx.x == 0) {
-        for (int i = tid; i < a.m.ninst; i += TL::BLK) {
-            a.t_inst[i] += a.dt_cur[i];
-            a.slot[i] = DBL_INF_BITS;
-        }
+        for (int i = tid; i < a.m.ninst; i += TL::BLK) a.t_inst[i] += a.dt_cur[i];
+        for (int i = tid; i < a.m.ninst * NSLOT; i += TL::BLK) a.slot[i] = DBL_INF_BITS;
     }
-    if ((a.role & ROLE_DT) && blockIdx.x == 0) {
-        for (int i = tid; i < a.m.ninst; i += TL::BLK) a.dt_cur[i] = stage_dt(a, i);
+    if (a.role & ROLE_CFL) {
+        int uinst = pinst[0];
+#pragma unroll
+        for (int k = 1; k < PPB; ++k)
+            if (pquad[k] >= 0 && pinst[k] != uinst) uinst = -1;
+        cfl_reduce<TL::BLK>(cand, active ? pinst[p] : 0, active, uinst, a.slot, red);
     }
-    if (a.role & ROLE_CFL) cfl_reduce<TL::BLK>(cand, active ? pinst[p] : 0, active, a.slot, red, flag);
     if (a.norm_part) sum_reduce_store<TL::BLK>(sq, &a.norm_part[blockIdx.x], red);
 }
 
@@ -592,12 +764,21 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK)) void cfl_kernel(MeshDev m, con
     using TL = Tile<N, PPB>;
     constexpr int N2 = TL::N2;
     __shared__ double red[(TL::BLK + 63) / 64];
-    __shared__ int flag[1];
+    __shared__ int pinst[PPB];
     const int tid = threadIdx.x;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int p = tid / N2, c = tid % N2;
     const int quad = lb * PPB + p;
     const bool active = quad < m.nquad;
+    if (tid < PPB) {
+        int qq = lb * PPB + tid;
+        pinst[tid] = qq < m.nquad ? m.inst[qq] : -1;
+    }
+    __syncthreads();
+    int uinst = pinst[0];
+#pragma unroll
+    for (int k = 1; k < PPB; ++k)
+        if (pinst[k] >= 0 && pinst[k] != uinst) uinst = -1;
     double cand = INFINITY;
     int inst = 0;
     if (active) {
@@ -607,12 +788,12 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK)) void cfl_kernel(MeshDev m, con
         double q[4] = {src[0], src[N2], src[2 * N2], src[3 * N2]};
         cand = cfl_candidate(q, m.ip[inst].cfl, h * h);
     }
-    cfl_reduce<TL::BLK>(cand, inst, active, slot, red, flag);
+    cfl_reduce<TL::BLK>(cand, inst, active, uinst, slot, red);
 }
 
 __global__ void slot_reset_kernel(unsigned long long *slot, int ninst)
 {
-    for (int i = threadIdx.x; i < ninst; i += blockDim.x) slot[i] = DBL_INF_BITS;
+    for (int i = threadIdx.x; i < ninst * NSLOT; i += blockDim.x) slot[i] = DBL_INF_BITS;
 }
 
 // RK1 has a single stage: the step's dt and time update run in this tiny
@@ -620,21 +801,14 @@ __global__ void slot_reset_kernel(unsigned long long *slot, int ninst)
 __global__ void dt_prep_kernel(MeshDev m, int use_cfl, double *dt_cur, double *t_inst,
                                unsigned long long *slot)
 {
-    for (int i = threadIdx.x; i < m.ninst; i += blockDim.x) {
-        const InstDev &ip = m.ip[i];
-        double dt = ip.dt_cap;
-        if (use_cfl) {
-            double c = bits2d(slot[i]);
-            if (c < dt) dt = c;
+    // one wave
+    for (int i = 0; i < m.ninst; ++i) {
+        double dt = step_dt(m.ip[i], slot, t_inst, i, use_cfl);
+        if (threadIdx.x == 0) {
+            dt_cur[i] = dt;
+            t_inst[i] += dt;
         }
-        if (ip.t_end > 0.0) {
-            double rem = ip.t_end - t_inst[i];
-            if (rem < 0.0) rem = 0.0;
-            if (rem < dt) dt = rem;
-        }
-        dt_cur[i] = dt;
-        t_inst[i] += dt;
-        slot[i] = DBL_INF_BITS;
+        for (int k = threadIdx.x; k < NSLOT; k += 64) slot[(size_t)i * NSLOT + k] = DBL_INF_BITS;
     }
 }
 
@@ -705,8 +879,12 @@ struct amrgpu_ctx {
     double *d_norm = nullptr;
     double *d_stage = nullptr;      // AoS staging for up/download
     int nblocks = 0;
+    int variant = -1;
+    int cap_quad = 0, cap_inst = 0, cap_blocks = 0;
+    size_t cap_state = 0;
     int cur = 0;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    bool graphs_enabled = true;   // AMRGPU_GRAPHS=0 selects direct launches
     std::vector<InstDev> h_ip;
 };
 
@@ -720,30 +898,65 @@ MeshDev mesh_of(const amrgpu_ctx *c)
     return m;
 }
 
-int ppb_of(int n)
+// Tile variants of the stage kernel: patches per workgroup (PPB) and the
+// minimum waves per SIMD the register allocator must leave room for (MINW).
+// The autotuner times every variant of the mesh's patch size.
+struct TileVariant {
+    int n, ppb, minw, design;
+};
+const TileVariant kVariants[] = {
+    {2, 16, 1, 0}, {2, 16, 1, 1},
+    {4, 8, 1, 0}, {4, 8, 1, 1},
+    {8, 2, 1, 0}, {8, 4, 1, 0}, {8, 4, 4, 0}, {8, 2, 1, 1}, {8, 4, 1, 1}, {8, 4, 4, 1},
+    {16, 1, 1, 0}, {16, 1, 4, 0}, {16, 1, 1, 1}, {16, 1, 4, 1},
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+int default_variant(int n)
 {
     switch (n) {
-    case 2: return 16;
-    case 4: return 8;
-    case 8: return 4;
-    case 16: return 1;
-    default: return 0;
+    case 2: return 0;
+    case 4: return 2;
+    case 8: return 6;
+    case 16: return 10;
+    default: return -1;
     }
+}
+
+int ppb_of(int n)
+{
+    int v = default_variant(n);
+    return v < 0 ? 0 : kVariants[v].ppb;
+}
+
+template <int N, int PPB, int MINW, int D>
+int launch_v(amrgpu_ctx *c, const StageArgs &a)
+{
+    int nblk = (c->nquad + PPB - 1) / PPB;
+    hipLaunchKernelGGL((stage_kernel<N, PPB, MINW, D>), dim3(nblk), dim3(Tile<N, PPB>::BLK), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return AMRGPU_OK;
 }
 
 int launch_stage(amrgpu_ctx *c, const StageArgs &a)
 {
-    int ppb = ppb_of(c->n);
-    int nblk = (c->nquad + ppb - 1) / ppb;
-    switch (c->n) {
-    case 2: hipLaunchKernelGGL((stage_kernel<2, 16>), dim3(nblk), dim3(64), 0, c->stream, a); break;
-    case 4: hipLaunchKernelGGL((stage_kernel<4, 8>), dim3(nblk), dim3(128), 0, c->stream, a); break;
-    case 8: hipLaunchKernelGGL((stage_kernel<8, 4>), dim3(nblk), dim3(256), 0, c->stream, a); break;
-    case 16: hipLaunchKernelGGL((stage_kernel<16, 1>), dim3(nblk), dim3(256), 0, c->stream, a); break;
-    default: set_err("unsupported nelem_subgrid %d", c->n); return AMRGPU_ERR_ARG;
+    switch (c->variant) {
+    case 0: return launch_v<2, 16, 1, 0>(c, a);
+    case 1: return launch_v<2, 16, 1, 1>(c, a);
+    case 2: return launch_v<4, 8, 1, 0>(c, a);
+    case 3: return launch_v<4, 8, 1, 1>(c, a);
+    case 4: return launch_v<8, 2, 1, 0>(c, a);
+    case 5: return launch_v<8, 4, 1, 0>(c, a);
+    case 6: return launch_v<8, 4, 4, 0>(c, a);
+    case 7: return launch_v<8, 2, 1, 1>(c, a);
+    case 8: return launch_v<8, 4, 1, 1>(c, a);
+    case 9: return launch_v<8, 4, 4, 1>(c, a);
+    case 10: return launch_v<16, 1, 1, 0>(c, a);
+    case 11: return launch_v<16, 1, 4, 0>(c, a);
+    case 12: return launch_v<16, 1, 1, 1>(c, a);
+    case 13: return launch_v<16, 1, 4, 1>(c, a);
+    default: set_err("bad tile variant %d", c->variant); return AMRGPU_ERR_ARG;
     }
-    HIPCHK(hipGetLastError());
-    return AMRGPU_OK;
 }
 
 int launch_cfl(amrgpu_ctx *c, const double *U)
@@ -873,6 +1086,8 @@ void free_mesh(amrgpu_ctx *c)
     c->d_dt_cur = c->d_t = nullptr; c->d_slot = nullptr; c->d_norm = nullptr;
     c->d_stage = nullptr;
     c->nquad = 0;
+    c->cap_quad = c->cap_inst = c->cap_blocks = 0;
+    c->cap_state = 0;
 }
 
 // face_info (reference format) -> per-quadrant face table
@@ -933,6 +1148,8 @@ const char *amrgpu_last_error(void) { return g_err; }
 amrgpu_ctx *amrgpu_create(void)
 {
     amrgpu_ctx *c = new amrgpu_ctx();
+    const char *ge = getenv("AMRGPU_GRAPHS");
+    if (ge && ge[0] == '0') c->graphs_enabled = false;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         set_err("hipStreamCreate failed");
         delete c;
@@ -989,7 +1206,21 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
     if (rc) return rc;
 
     HIPCHK(hipStreamSynchronize(c->stream));
-    free_mesh(c);
+    free_graphs(c);
+    // Buffers are kept across setups (an adaptive run calls this once per
+    // regrid interval) and grown with 25% headroom when the mesh outgrows them.
+    const bool need_acc = (time_scheme == 4 || time_scheme == 5);
+    const size_t need_state = (size_t)ncell * 4 * n2;
+    const bool reuse = c->U[0] && ncell <= c->cap_quad && ninst <= c->cap_inst &&
+                       need_state <= c->cap_state && (!need_acc || c->Acc);
+    if (!reuse) {
+        free_mesh(c);
+        int capq = ncell + ncell / 4 + 16;
+        c->cap_quad = capq;
+        c->cap_inst = ninst;
+        c->cap_state = (size_t)capq * 4 * n2;
+        c->cap_blocks = capq + 64;   // enough for one patch per workgroup
+    }
     c->n = nelem_subgrid; c->nquad = ncell; c->ninst = ninst; c->scheme = time_scheme;
     c->cur = 0;
     std::vector<int> level(ncell), qi(ncell, 0);
@@ -1006,19 +1237,26 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
         c->h_ip[i].t_end = inst[i].t_end;
     }
     const size_t nstate = (size_t)ncell * 4 * n2;
-    int ppb = ppb_of(c->n);
-    c->nblocks = (ncell + ppb - 1) / ppb;
-    HIPCHK(hipMalloc(&c->d_nbr, sizeof(int) * nbr.size()));
-    HIPCHK(hipMalloc(&c->d_level, sizeof(int) * ncell));
-    HIPCHK(hipMalloc(&c->d_inst, sizeof(int) * ncell));
-    HIPCHK(hipMalloc(&c->d_ip, sizeof(InstDev) * ninst));
-    for (int i = 0; i < 3; ++i) HIPCHK(hipMalloc(&c->U[i], sizeof(double) * nstate));
-    if (time_scheme == 4 || time_scheme == 5) HIPCHK(hipMalloc(&c->Acc, sizeof(double) * nstate));
-    HIPCHK(hipMalloc(&c->d_dt_cur, sizeof(double) * ninst));
-    HIPCHK(hipMalloc(&c->d_t, sizeof(double) * ninst));
-    HIPCHK(hipMalloc(&c->d_slot, sizeof(unsigned long long) * ninst));
-    HIPCHK(hipMalloc(&c->d_norm, sizeof(double) * c->nblocks));
-    HIPCHK(hipMalloc(&c->d_stage, sizeof(double) * nstate));
+    if (c->variant < 0 || kVariants[c->variant].n != c->n) c->variant = default_variant(c->n);
+    c->nblocks = (ncell + kVariants[c->variant].ppb - 1) / kVariants[c->variant].ppb;
+    if (c->nblocks > c->cap_blocks || nstate > c->cap_state) {
+        set_err("amrgpu_setup: internal capacity error");
+        return AMRGPU_ERR_ARG;
+    }
+    if (!reuse) {
+        const size_t cq = (size_t)c->cap_quad, cs = c->cap_state;
+        HIPCHK(hipMalloc(&c->d_nbr, sizeof(int) * 8 * cq));
+        HIPCHK(hipMalloc(&c->d_level, sizeof(int) * cq));
+        HIPCHK(hipMalloc(&c->d_inst, sizeof(int) * cq));
+        HIPCHK(hipMalloc(&c->d_ip, sizeof(InstDev) * c->cap_inst));
+        for (int i = 0; i < 3; ++i) HIPCHK(hipMalloc(&c->U[i], sizeof(double) * cs));
+        if (need_acc) HIPCHK(hipMalloc(&c->Acc, sizeof(double) * cs));
+        HIPCHK(hipMalloc(&c->d_dt_cur, sizeof(double) * c->cap_inst));
+        HIPCHK(hipMalloc(&c->d_t, sizeof(double) * c->cap_inst));
+        HIPCHK(hipMalloc(&c->d_slot, sizeof(unsigned long long) * c->cap_inst * NSLOT));
+        HIPCHK(hipMalloc(&c->d_norm, sizeof(double) * c->cap_blocks));
+        HIPCHK(hipMalloc(&c->d_stage, sizeof(double) * cs));
+    }
     HIPCHK(hipMemcpyAsync(c->d_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_level, level.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_inst, qi.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
@@ -1079,21 +1317,25 @@ int amrgpu_rhs(amrgpu_ctx *c, double *rhs)
 int amrgpu_cfl(amrgpu_ctx *c, double *dt_inst)
 {
     if (!c || !dt_inst || !c->nquad) { set_err("amrgpu_cfl: no mesh"); return AMRGPU_ERR_ARG; }
-    hipLaunchKernelGGL(slot_reset_kernel, dim3(1), dim3(64), 0, c->stream, c->d_slot, c->ninst);
+    hipLaunchKernelGGL(slot_reset_kernel, dim3(1), dim3(256), 0, c->stream, c->d_slot, c->ninst);
     HIPCHK(hipGetLastError());
     int rc = launch_cfl(c, c->U[c->cur]);
     if (rc) return rc;
-    std::vector<unsigned long long> b(c->ninst);
-    HIPCHK(hipMemcpyAsync(b.data(), c->d_slot, sizeof(unsigned long long) * c->ninst, hipMemcpyDeviceToHost, c->stream));
+    std::vector<unsigned long long> b((size_t)c->ninst * NSLOT);
+    HIPCHK(hipMemcpyAsync(b.data(), c->d_slot, sizeof(unsigned long long) * b.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (int i = 0; i < c->ninst; ++i) memcpy(&dt_inst[i], &b[i], sizeof(double));
+    for (int i = 0; i < c->ninst; ++i) {
+        unsigned long long mn = DBL_INF_BITS;
+        for (int k = 0; k < NSLOT; ++k) mn = std::min(mn, b[(size_t)i * NSLOT + k]);
+        memcpy(&dt_inst[i], &mn, sizeof(double));
+    }
     return AMRGPU_OK;
 }
 
 static int prime_run(amrgpu_ctx *c)
 {
     // CFL slot of the current state for the first step's dt
-    hipLaunchKernelGGL(slot_reset_kernel, dim3(1), dim3(64), 0, c->stream, c->d_slot, c->ninst);
+    hipLaunchKernelGGL(slot_reset_kernel, dim3(1), dim3(256), 0, c->stream, c->d_slot, c->ninst);
     HIPCHK(hipGetLastError());
     if (c->scheme != 2) return launch_cfl(c, c->U[c->cur]);
     return AMRGPU_OK;
@@ -1105,7 +1347,7 @@ static int graph_for(amrgpu_ctx *c, int cur, hipGraphExec_t *out)
     hipGraph_t g;
     int nx;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_step(c, cur, true, &nx);
+    int rc = enqueue_step(c, cur, false, &nx);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) return rc;
     if (e != hipSuccess) { set_err("graph capture: %s", hipGetErrorString(e)); return AMRGPU_ERR_HIP; }
@@ -1134,25 +1376,35 @@ static int finish_norm(amrgpu_ctx *c, double *resnorm)
     return AMRGPU_OK;
 }
 
+// One step: a replay of the captured step graph, or direct launches.
+static int one_step(amrgpu_ctx *c, bool graph, bool with_norm)
+{
+    int nx, rc;
+    if (graph && c->graphs_enabled && !with_norm) {
+        hipGraphExec_t g;
+        rc = graph_for(c, c->cur, &g);
+        if (rc) return rc;
+        HIPCHK(hipGraphLaunch(g, c->stream));
+        nx = step_next(c, c->cur);
+    } else {
+        rc = enqueue_step(c, c->cur, with_norm, &nx);
+        if (rc) return rc;
+    }
+    c->cur = nx;
+    return AMRGPU_OK;
+}
+
 int amrgpu_run(amrgpu_ctx *c, int nsteps, int use_graph, double *resnorm)
 {
     if (!c || !c->nquad || nsteps < 0) { set_err("amrgpu_run: no mesh"); return AMRGPU_ERR_ARG; }
     if (nsteps == 0) return AMRGPU_OK;
     int rc = prime_run(c);
     if (rc) return rc;
+    // the residual norm (NaN check, reported norm) is taken on the last stage
+    // of the last step, the only value the reference keeps
     for (int s = 0; s < nsteps; ++s) {
-        int nx;
-        if (use_graph) {
-            hipGraphExec_t g;
-            rc = graph_for(c, c->cur, &g);
-            if (rc) return rc;
-            HIPCHK(hipGraphLaunch(g, c->stream));
-            nx = step_next(c, c->cur);
-        } else {
-            rc = enqueue_step(c, c->cur, true, &nx);
-            if (rc) return rc;
-        }
-        c->cur = nx;
+        rc = one_step(c, use_graph != 0, s == nsteps - 1);
+        if (rc) return rc;
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return finish_norm(c, resnorm);
@@ -1180,8 +1432,8 @@ int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_
     if (!c || !c->nquad || nsteps <= 0) { set_err("amrgpu_bench: no mesh"); return AMRGPU_ERR_ARG; }
     int rc = prime_run(c);
     if (rc) return rc;
-    // capture graphs for every starting buffer reached before timing
-    {
+    // capture the graphs of every starting buffer before timing
+    if (c->graphs_enabled) {
         int cur = c->cur;
         for (int s = 0; s < 2; ++s) {
             hipGraphExec_t g;
@@ -1196,11 +1448,8 @@ int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipEventRecord(e0, c->stream));
     for (int s = 0; s < nsteps; ++s) {
-        hipGraphExec_t g;
-        rc = graph_for(c, c->cur, &g);
+        rc = one_step(c, true, false);
         if (rc) return rc;
-        HIPCHK(hipGraphLaunch(g, c->stream));
-        c->cur = step_next(c, c->cur);
     }
     HIPCHK(hipEventRecord(e1, c->stream));
     HIPCHK(hipEventSynchronize(e1));
@@ -1210,12 +1459,12 @@ int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_
 
     // per-stage kernel durations: direct launches bracketed by events
     std::vector<StageArgs> st;
-    build_step(c, c->cur, st, true);
+    build_step(c, c->cur, st, false);
     const int ns = (int)st.size();
     std::vector<hipEvent_t> ev((size_t)nsteps * (ns + 1));
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     for (int s = 0; s < nsteps; ++s) {
-        int nx = build_step(c, c->cur, st, true);
+        int nx = build_step(c, c->cur, st, false);
         if (c->scheme == 1) {
             hipLaunchKernelGGL(dt_prep_kernel, dim3(1), dim3(64), 0, c->stream, mesh_of(c), 1,
                                c->d_dt_cur, c->d_t, c->d_slot);
@@ -1242,7 +1491,101 @@ int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_
     for (auto &e : ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // one more step carrying the residual norm: NaN check of the result
+    rc = one_step(c, false, true);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
     return finish_norm(c, nullptr);
+}
+
+int amrgpu_set_tile(amrgpu_ctx *c, int ppb, int minw, int design)
+{
+    if (!c || !c->nquad) { set_err("amrgpu_set_tile: no mesh"); return AMRGPU_ERR_ARG; }
+    for (int v = 0; v < kNumVariants; ++v) {
+        if (kVariants[v].n == c->n && kVariants[v].ppb == ppb && kVariants[v].minw == minw &&
+            kVariants[v].design == design) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            free_graphs(c);
+            c->variant = v;
+            c->nblocks = (c->nquad + ppb - 1) / ppb;
+            return AMRGPU_OK;
+        }
+    }
+    set_err("amrgpu_set_tile: no variant n=%d ppb=%d minw=%d design=%d", c->n, ppb, minw, design);
+    return AMRGPU_ERR_ARG;
+}
+
+int amrgpu_tile_variants(int n, int *ppb, int *minw, int *design, int max)
+{
+    int k = 0;
+    for (int v = 0; v < kNumVariants; ++v) {
+        if (kVariants[v].n != n) continue;
+        if (k < max) {
+            if (ppb) ppb[k] = kVariants[v].ppb;
+            if (minw) minw[k] = kVariants[v].minw;
+            if (design) design[k] = kVariants[v].design;
+        }
+        ++k;
+    }
+    return k;
+}
+
+int amrgpu_get_tile(amrgpu_ctx *c, int *ppb, int *minw, int *design)
+{
+    if (!c || c->variant < 0) { set_err("amrgpu_get_tile: no mesh"); return AMRGPU_ERR_ARG; }
+    if (ppb) *ppb = kVariants[c->variant].ppb;
+    if (minw) *minw = kVariants[c->variant].minw;
+    if (design) *design = kVariants[c->variant].design;
+    return AMRGPU_OK;
+}
+
+int amrgpu_autotune(amrgpu_ctx *c, int nsteps, int *nvar, int *ppb, int *minw, int *design,
+                    double *ms_per_step)
+{
+    if (!c || !c->nquad || nsteps <= 0) { set_err("amrgpu_autotune: no mesh"); return AMRGPU_ERR_ARG; }
+    // the trial steps advance the state: keep a device copy of it and the
+    // per-instance time, and restore both afterwards
+    const size_t nstate = (size_t)c->nquad * 4 * c->n * c->n;
+    std::vector<double> t0(c->ninst);
+    HIPCHK(hipMemcpyAsync(t0.data(), c->d_t, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_stage, c->U[c->cur], sizeof(double) * nstate, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int cur0 = c->cur;
+    int best = -1, k = 0;
+    double best_ms = 1e300;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int v = 0; v < kNumVariants; ++v) {
+        if (kVariants[v].n != c->n) continue;
+        int rc = amrgpu_set_tile(c, kVariants[v].ppb, kVariants[v].minw, kVariants[v].design);
+        if (rc) return rc;
+        rc = prime_run(c);
+        if (rc) return rc;
+        for (int s = 0; s < 2; ++s) if ((rc = one_step(c, true, false))) return rc;   // warm + capture
+        HIPCHK(hipEventRecord(e0, c->stream));
+        for (int s = 0; s < nsteps; ++s) if ((rc = one_step(c, true, false))) return rc;
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        double per = ms / nsteps;
+        if (ppb) ppb[k] = kVariants[v].ppb;
+        if (minw) minw[k] = kVariants[v].minw;
+        if (design) design[k] = kVariants[v].design;
+        if (ms_per_step) ms_per_step[k] = per;
+        ++k;
+        if (per < best_ms) { best_ms = per; best = v; }
+        // restore the state for the next trial
+        HIPCHK(hipMemcpyAsync(c->U[cur0], c->d_stage, sizeof(double) * nstate, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_t, t0.data(), sizeof(double) * c->ninst, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->cur = cur0;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (nvar) *nvar = k;
+    return amrgpu_set_tile(c, kVariants[best].ppb, kVariants[best].minw, kVariants[best].design);
 }
 
 int amrgpu_copy_probe(long long nbytes, int reps, double *gbps)

@@ -39,6 +39,8 @@ void physics_gpu_get_params(physics_gpu_params_t *p)
     if (p) *p = g_p;
 }
 
+double physics_gpu_time(void) { return g_time; }
+
 /* keyword lookup with the semantics of utilities_find_keyword_*
  * (src/physics/src/physics_utilities.c:135-178): the first line containing
  * the keyword wins and the value is read at offset strlen(keyword) of that

@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Benchmark of the FP64 finite-volume Euler hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): the 2D
+four-quadrant Riemann problem on [0,1]^2, states from
+numpy.random.default_rng(seed) (rho log-uniform [0.1,2], u,v uniform [-1,1],
+p uniform [0.1,2]), 8x8 FP64 sub-grid patches on an adaptive forest with
+levels 3-7 refined by the density-gradient indicator, regrid every 10 steps,
+2:1 balance, transmissive boundaries, SSP-RK3 with input CFL 0.4.
+
+Setup (untimed): the adaptive problem is run on the GPU path through the C
+evolve loop (hpc_amr_evolve_solver + host regrid) to t = --t-end; the mesh
+and state reached there are the benchmark input, resident in HBM.
+
+A step = one SSP-RK3 step (3 fused residual + stage-update launches, CFL
+folded into the last) over that mesh.  value = sub-grid cells x 3 stages x
+steps / wall time of the timed region (graph replay, HIP events).
+
+roofline: the fused stage kernel; algorithmic bytes = 85.33 B per cell-stage
+(stage 1 reads U^n, writes U^1; stages 2-3 read U^s and U^n, write U^s+1;
+32 B per state), achieved = bytes per launch / average launch duration
+(HIP events around each launch, same stream).
+
+cpu_baseline: the CPU oracle (C restatement of the reference CPU path,
+AoS patches, face loop + element loop, k-form SSP-RK3) on the host cores,
+one worker per core on contiguous quadrant ranges, on a bounded number of
+steps of the same mesh; the faster of its -O2 and -O3 -march=native builds.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "amr-demo_amd"))
+
+METRIC = "Mcell-updates/s (RK stage) on 1 GPU, %HBM peak; plain vs learned+batched at equal L1"
+HBM_PEAK_GBS = 8000.0
+B_ALG_STAGE = (64.0 + 96.0 + 96.0) / 3.0     # bytes per cell-stage, SSP-RK3 Shu-Osher
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_riemann_mesh(amrhip, seed, lmin, lmax, n, tol, t_end, regrid):
+    """run the adaptive Riemann problem on the GPU path to t_end; return the
+    forest (mesh + state) and run statistics"""
+    import ctypes as C
+    ic = amrhip.riemann4_ic(seed)
+    f = amrhip.Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, n, lmin, lmax)
+    f.apply_ic(ic)
+    f.regrid_initial(ic, tol)
+    L = amrhip.lib()
+    amrhip.set_params(nelem_subgrid=n, time_scheme=3, time_steps=10 ** 9, dt=1.0,
+                      cfl=0.25 * 0.4 * 3, coarse_grid_h=1.0, regrid_interval=regrid,
+                      checkpoint_interval=2 ** 31 - 1, visualization_interval=2 ** 31 - 1,
+                      bc=(0, 0, 0, 0, 1, 1), t_end=t_end, verbose=0)
+    stats = EvolveStats()
+    L.amr_evolve.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_void_p]
+    L.amr_evolve.restype = C.c_int
+    t0 = time.perf_counter()
+    rc = L.amr_evolve(f.h, tol, 1, C.byref(stats))
+    wall = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError(f"amr_evolve failed rc={rc}: {amrhip.last_error()}")
+    return f, stats, wall
+
+
+def _stats_type():
+    import ctypes as C
+
+    class S(C.Structure):
+        _fields_ = [("steps", C.c_int), ("regrids", C.c_int), ("peak_quads", C.c_int),
+                    ("cell_stages", C.c_double), ("t", C.c_double), ("wall_gpu", C.c_double),
+                    ("wall_regrid", C.c_double)]
+    return S
+
+
+EvolveStats = _stats_type()
+
+
+def cpu_baseline(ci, fi, geom, soln, n, seconds, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    best = None
+    for path, label in ((pyoracle.ORACLE_SO, "-O2"), (pyoracle.ORACLE_NATIVE_SO, "-O3 -march=native")):
+        if not os.path.exists(path):
+            continue
+        orc = pyoracle.Oracle(path)
+        p = orc.params(n, bc=(0, 0, 0, 0), cfl=0.3, dt=1.0, h_base=1.0, scheme=3)
+        t0 = time.perf_counter()
+        orc.run(p, ci, fi, geom, soln, 1, threads=threads)
+        one = max(time.perf_counter() - t0, 1e-4)
+        k = max(1, int(seconds / 2 / one))
+        t0 = time.perf_counter()
+        orc.run(p, ci, fi, geom, soln, k, threads=threads)
+        dt = time.perf_counter() - t0
+        cells = soln.size // 4
+        v = cells * 3 * k / dt / 1e6
+        log(f"cpu baseline {label}: {k} steps, {dt:.2f} s, {v:.2f} Mcell-updates/s on {threads} threads")
+        if best is None or v > best[0]:
+            best = (v, label, k, dt)
+    return best
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--lmin", type=int, default=3)
+    ap.add_argument("--lmax", type=int, default=7)
+    ap.add_argument("--n", type=int, default=8)
+    ap.add_argument("--tol", type=float, default=0.1)
+    ap.add_argument("--t-end", type=float, default=0.25)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tile", type=str, default="", help="ppb,minw,design (skips the autotuner)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on exactly one GPU (the whole forest lives on one "
+              "MI355X); multi-GPU sharding is described in DESIGN.md and not launched.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import amrhip
+    amrhip.lib()
+    log(f"setup: adaptive Riemann seed={args.seed} levels {args.lmin}-{args.lmax} n={args.n} "
+        f"to t={args.t_end}")
+    f, st, wall = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, args.n, args.tol,
+                                     args.t_end, 10)
+    ci, fi, geom, soln = f.external()
+    nq = f.nquad
+    cells = nq * args.n * args.n
+    lv = f.levels()
+    log(f"setup done: {st.steps} steps, {st.regrids} regrids, t={st.t:.4f}, peak {st.peak_quads} quads, "
+        f"final {nq} quads / {cells} cells, levels {lv.min()}-{lv.max()}, {wall:.2f} s "
+        f"(gpu {st.wall_gpu:.2f} s, regrid+boundary {st.wall_regrid:.2f} s)")
+
+    g = amrhip.Gpu()
+    g.setup(args.n, 3, [amrhip.make_inst(1.0, (0, 0, 0, 0), 0.3, 1.0, 0.0)], ci, fi, geom, soln)
+    tune = None
+    if args.tile:
+        g.set_tile(*(int(x) for x in args.tile.split(",")))
+    elif not args.no_autotune:
+        tune = g.autotune(10)
+        log("autotune (ppb, minw, design, ms/step): " +
+            ", ".join(f"({a},{b},{c},{d:.4f})" for a, b, c, d in tune))
+    tile = g.tile()
+    if args.warmup:
+        g.run(args.warmup)
+    ms, stage_ms = g.bench(args.steps)
+    ms_per_step = ms / args.steps
+    value = cells * 3 * args.steps / (ms * 1e-3) / 1e6
+    avg_stage = float(np.mean(stage_ms))
+    bytes_launch = cells * B_ALG_STAGE
+    achieved = bytes_launch / (avg_stage * 1e-3) / 1e9
+    log(f"timed: {args.steps} steps, {ms:.2f} ms ({ms_per_step * 1e3:.1f} us/step); stage kernel "
+        f"avg {[round(s * 1e3, 2) for s in stage_ms]} us; {value:.1f} Mcell-updates/s")
+    try:
+        probe = amrhip.copy_probe(1 << 30, 20)
+    except Exception as e:   # noqa: BLE001
+        log(f"copy probe failed: {e}")
+        probe = None
+    g.close()
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"riemann4 seed {args.seed}, adaptive levels {args.lmin}-{args.lmax}, "
+                        f"{args.n}x{args.n} FP64 patches, SSP-RK3 CFL 0.4, mesh at t={st.t:.3f}",
+            "quads": nq, "cells": cells, "peak_quads_in_run": st.peak_quads,
+            "levels": [int(lv.min()), int(lv.max())], "parallelism": "1 GPU, whole forest",
+            "tile": {"patches_per_workgroup": tile[0], "min_waves_per_simd": tile[1],
+                     "design": ["face task list", "paired faces"][tile[2]],
+                     "autotuned": tune is not None},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": f"stage_kernel<{args.n},{tile[0]},{tile[1]},{tile[2]}>",
+            "bytes_per_launch": int(bytes_launch),
+            "avg_launch_us": round(avg_stage * 1e3, 3),
+            "stage_launch_us": [round(s * 1e3, 3) for s in stage_ms],
+            "copy_probe_GBs": round(probe, 1) if probe else None,
+        },
+        "adaptive_run": {
+            "steps": st.steps, "regrids": st.regrids, "t": round(st.t, 6),
+            "cell_stages": st.cell_stages, "wall_s": round(wall, 3),
+            "end_to_end_Mcell_updates_per_s": round(st.cell_stages / wall / 1e6, 3),
+        },
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        best = cpu_baseline(ci, fi, geom, soln, args.n, args.cpu_seconds, threads)
+        if best:
+            v, label, k, dt = best
+            out["cpu_baseline"] = {
+                "value": round(v, 3), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+                "sample": f"{k} SSP-RK3 steps of the same {cells}-cell mesh ({dt:.1f} s), "
+                          f"C oracle {label}, {threads} threads on contiguous quadrant ranges",
+            }
+            out["gpu_over_cpu"] = round(value / v, 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -63,6 +63,25 @@ int amr_regrid(amr_forest_t *f, double tol, int allow_coarsening);
  * (initialize_quadrant_data), then balance. */
 int amr_regrid_initial(amr_forest_t *f, const amr_ic_t *ic, double tol);
 
+/* Statistics of an adaptive run. */
+typedef struct {
+    int    steps;            /* time steps taken */
+    int    regrids;          /* regrid calls */
+    int    peak_quads;       /* largest quadrant count seen */
+    double cell_stages;      /* sum over steps of sub-cells x RK stages */
+    double t;                /* physical time reached */
+    double wall_gpu;         /* seconds inside physics_go_gpu */
+    double wall_regrid;      /* seconds in host regrid + boundary build */
+} amr_evolve_stats_t;
+
+/* The evolve loop of src/amr/src/amr_evolve.c:15-73 on the host forest: per
+ * interval build the boundary arrays (external_setup), call
+ * hpc_amr_evolve_solver (the GPU physics module), and regrid when it raises
+ * the flag.  The physics settings come from physics_gpu_set_params /
+ * hpc_amr_initialize_solver; tol is the density-gradient tagging tolerance.
+ * Returns 0, or the physics module's error code. */
+int amr_evolve(amr_forest_t *f, double tol, int allow_coarsening, amr_evolve_stats_t *stats);
+
 #ifdef __cplusplus
 }
 #endif

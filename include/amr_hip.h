@@ -63,6 +63,8 @@ void hpc_amr_initialize_solver(int *rank, int *dim, double *mesh_scale, int *nle
  * reference writes the globals directly). */
 void physics_gpu_set_params(const physics_gpu_params_t *p);
 void physics_gpu_get_params(physics_gpu_params_t *p);
+/* Physical time accumulated by physics_go_gpu since the last set_params. */
+double physics_gpu_time(void);
 
 /* The GPU seam: physics_go_gpu has the argument list of physics_go
  * (src/physics/src/physics_go.h:59-84), as physics_evolve_interface calls it
@@ -146,6 +148,18 @@ int amrgpu_set_time(amrgpu_ctx *c, const double *t_inst);
 int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_ms,
                  int *nstages);
 int amrgpu_num_cells(const amrgpu_ctx *c);   /* sub-grid cells */
+/* Tile shape of the stage kernel: patches per workgroup, the minimum waves
+ * per SIMD reserved by register allocation, and the face-evaluation design
+ * (0: face task list, 1: paired +x/+y faces per cell).  amrgpu_autotune
+ * times every compiled variant for the mesh's patch size over nsteps steps
+ * (state and time are restored afterwards), keeps the fastest and reports
+ * each variant's ms per step. */
+int amrgpu_set_tile(amrgpu_ctx *c, int ppb, int minw, int design);
+/* The compiled variants for patch size n (returns their number). */
+int amrgpu_tile_variants(int n, int *ppb, int *minw, int *design, int max);
+int amrgpu_get_tile(amrgpu_ctx *c, int *ppb, int *minw, int *design);
+int amrgpu_autotune(amrgpu_ctx *c, int nsteps, int *nvar, int *ppb, int *minw, int *design,
+                    double *ms_per_step);
 int amrgpu_num_quads(const amrgpu_ctx *c);
 
 /* Device-side helpers for the HBM roofline probe: copy nbytes between two

@@ -77,6 +77,28 @@ def test_rhs_adaptive_forest(gpu, oracle, n, bc):
     assert parity_err(gpu.rhs(), ref) <= RHS_TOL
 
 
+@pytest.mark.parametrize("n", [2, 4, 8, 16])
+def test_every_tile_variant(gpu, oracle, n):
+    """every stage-kernel variant the autotuner may pick meets the bar, on a
+    mesh with hanging faces and all three wall types"""
+    f = riemann_forest(n, seed=5 + n, lmax=4 if n == 16 else 5)
+    ci, fi, geom, soln = f.external()
+    bc = (2, 1, 0, 1)
+    setup(gpu, n, 3, 1.0, bc, 0.3, 1.0, ci, fi, geom, soln)
+    ref = oracle.rhs(oracle.params(n, bc=bc, h_base=1.0), ci, fi, geom, soln)
+    p = oracle.params(n, bc=bc, cfl=0.3, dt=1.0, h_base=1.0, scheme=3)
+    q_ref, _, _ = oracle.run(p, ci, fi, geom, soln, 3)
+    variants = amrhip.tile_variants(n)
+    assert variants
+    for ppb, minw, design in variants:
+        log(f"variant n={n} ppb={ppb} minw={minw} design={design}")
+        gpu.setup(n, 3, [amrhip.make_inst(1.0, bc, 0.3, 1.0)], ci, fi, geom, soln)
+        gpu.set_tile(ppb, minw, design)
+        assert parity_err(gpu.rhs(), ref) <= RHS_TOL
+        gpu.run(3)
+        assert parity_err(gpu.soln(), q_ref) <= TRAJ_TOL
+
+
 def test_rhs_config1_vortex(gpu, oracle):
     """the isentropic vortex on the uniform level-4 forest with 8x8 patches"""
     log("vortex level 4 n=8")
