@@ -74,7 +74,30 @@ EXPORTS = {
     "amr_indicator_gradient": (None, [_vp, _pd]),
     "amr_regrid": (_i, [_vp, _d, _i]),
     "amr_regrid_initial": (_i, [_vp, _vp, _d]),
+    "amr_regrid_tags": (_i, [_vp, _pi, _i]),
+    "amr_evolve": (_i, [_vp, _d, _i, _vp]),
+    "amr_evolve_device": (_i, [_vp, _vp, _vp]),
+    "amr_forest_track_begin": (None, [_vp, _i]),
+    "amr_forest_track_src": (_pi, [_vp]),
+    "amr_forest_track_end": (_i, [_vp]),
+    "amrgpu_indicator": (_i, [_vp, _pd]),
+    "amrgpu_remesh": (_i, [_vp, _i, _pi, _pi, _i, _pi, _pi]),
 }
+
+
+class EvolveStats(C.Structure):
+    """amr_evolve_stats_t"""
+    _fields_ = [("steps", _i), ("regrids", _i), ("peak_quads", _i), ("cell_stages", _d), ("t", _d),
+                ("wall_gpu", _d), ("wall_regrid", _d)]
+
+
+TAGGER_FN = C.CFUNCTYPE(_i, _vp, _vp, _vp, _pi)
+
+
+class EvolveOpts(C.Structure):
+    """amr_evolve_opts_t"""
+    _fields_ = [("tol", _d), ("allow_coarsening", _i), ("regrid_interval", _i), ("tagger", _vp),
+                ("tagger_ctx", _vp), ("tile_ppb", _i), ("tile_minw", _i), ("tile_design", _i)]
 
 
 class PhysicsParams(C.Structure):
@@ -243,6 +266,38 @@ class Forest:
     def regrid_initial(self, ic, tol):
         return self.L.amr_regrid_initial(self.h, C.byref(ic), tol)
 
+    def regrid_tags(self, tags, allow_coarsening=True):
+        t = np.ascontiguousarray(tags, dtype=np.int32)
+        return self.L.amr_regrid_tags(self.h, ipt(t), int(allow_coarsening))
+
+    def track_begin(self, keep_data=True):
+        self.L.amr_forest_track_begin(self.h, int(keep_data))
+
+    def track_end(self):
+        src = np.ctypeslib.as_array(self.L.amr_forest_track_src(self.h), shape=(self.nquad,)).copy()
+        ok = self.L.amr_forest_track_end(self.h)
+        return bool(ok), src
+
+    def evolve(self, tol, allow_coarsening=True):
+        """host-regrid evolve loop (hpc_amr_evolve_solver per interval)"""
+        st = EvolveStats()
+        rc = self.L.amr_evolve(self.h, tol, int(allow_coarsening), C.byref(st))
+        check(rc, "amr_evolve")
+        return st
+
+    def evolve_device(self, tol, regrid_interval, allow_coarsening=True, tile=None, tagger=None):
+        """device-resident evolve loop; tagger: optional TAGGER_FN"""
+        o = EvolveOpts()
+        o.tol, o.allow_coarsening, o.regrid_interval = tol, int(allow_coarsening), regrid_interval
+        o.tagger = C.cast(tagger, _vp) if tagger is not None else None
+        o.tagger_ctx = None
+        if tile:
+            o.tile_ppb, o.tile_minw, o.tile_design = tile
+        st = EvolveStats()
+        rc = self.L.amr_evolve_device(self.h, C.byref(o), C.byref(st))
+        check(rc, "amr_evolve_device")
+        return st
+
 
 def vortex_ic(strength=5.0, xo=5.0, yo=5.0, u=1.0, v=1.0, gamma=1.4, rho=1.0, p=1.0):
     """isentropic vortex through the reference's free-stream inputs: u = v = 1
@@ -333,6 +388,22 @@ class Gpu:
         check(rc, "amrgpu_setup")
         self.ninst = ninst
         self.ndof = s.size
+        self.n2 = n * n
+
+    def indicator(self):
+        out = np.empty(self.ndof // 4 // self.n2)
+        check(self.L.amrgpu_indicator(self.h, dpt(out)), "amrgpu_indicator")
+        return out
+
+    def remesh(self, cell_info, face_info, src, cell_inst=None):
+        ci = np.ascontiguousarray(cell_info, dtype=np.int32)
+        fi = np.ascontiguousarray(face_info, dtype=np.int32)
+        sr = np.ascontiguousarray(src, dtype=np.int32)
+        cinst = None if cell_inst is None else np.ascontiguousarray(cell_inst, dtype=np.int32)
+        rc = self.L.amrgpu_remesh(self.h, ci.size // 3, ipt(ci), ipt(cinst) if cinst is not None else None,
+                                  fi.size // 11, ipt(fi), ipt(sr))
+        check(rc, "amrgpu_remesh")
+        self.ndof = ci.size // 3 * 4 * self.n2
 
     def rhs(self):
         out = np.empty(self.ndof)

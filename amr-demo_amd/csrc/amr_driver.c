@@ -211,22 +211,32 @@ void amr_indicator_gradient(amr_forest_t *f, double *ind)
     free(e1);
 }
 
-int amr_regrid(amr_forest_t *f, double tol, int allow_coarsening)
+int amr_regrid_tags(amr_forest_t *f, const int *tags, int allow_coarsening)
 {
     int nq = amr_forest_num_quads(f);
-    double *ind = (double *)malloc(sizeof(double) * (size_t)(nq > 0 ? nq : 1));
     int *tag = amr_forest_tags(f);
-    amr_indicator_gradient(f, ind);
     /* tag_reset + regrid_feature */
-    for (int k = 0; k < nq; ++k) tag[k] = ind[k] > tol ? 1 : 0;
+    for (int k = 0; k < nq; ++k) tag[k] = tags[k] ? 1 : 0;
     int *flags = (int *)malloc(sizeof(int) * (size_t)(nq > 0 ? nq : 1));
     memcpy(flags, tag, sizeof(int) * (size_t)nq);
     amr_forest_refine(f, flags);
     free(flags);
-    free(ind);
     if (allow_coarsening) amr_forest_coarsen(f);
     amr_forest_balance(f);
     return amr_forest_num_quads(f);
+}
+
+int amr_regrid(amr_forest_t *f, double tol, int allow_coarsening)
+{
+    int nq = amr_forest_num_quads(f);
+    double *ind = (double *)malloc(sizeof(double) * (size_t)(nq > 0 ? nq : 1));
+    int *tags = (int *)malloc(sizeof(int) * (size_t)(nq > 0 ? nq : 1));
+    amr_indicator_gradient(f, ind);
+    for (int k = 0; k < nq; ++k) tags[k] = ind[k] > tol;
+    int r = amr_regrid_tags(f, tags, allow_coarsening);
+    free(tags);
+    free(ind);
+    return r;
 }
 
 int amr_regrid_initial(amr_forest_t *f, const amr_ic_t *ic, double tol)
@@ -306,6 +316,102 @@ int amr_evolve(amr_forest_t *f, double tol, int allow_coarsening, amr_evolve_sta
     }
     physics_gpu_get_params(&p);
     s.t = physics_gpu_time();
+    if (st) *st = s;
+    return rc;
+}
+
+/* interval length of the reference step loop for the regrid flag */
+static long long steps_to(int istep, int interval)
+{
+    if (interval <= 0 || interval == 2147483647) return 1LL << 40;
+    return (long long)interval - (istep % interval);
+}
+
+int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_stats_t *st)
+{
+    physics_gpu_params_t p;
+    physics_gpu_get_params(&p);
+    static const int stages[6] = {0, 1, 2, 3, 4, 5};
+    amr_evolve_stats_t s;
+    memset(&s, 0, sizeof(s));
+    s.peak_quads = amr_forest_num_quads(f);
+    const int n = amr_forest_nelem_subgrid(f);
+    int ncell, nghost, nface, dof, rc = 0;
+    int *cell_info, *face_info;
+    double *geom, *soln;
+    double t0 = now_s();
+    if (amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom, &soln))
+        return AMRGPU_ERR_MESH;
+    amrgpu_ctx *g = amrgpu_create();
+    if (!g) return AMRGPU_ERR_HIP;
+    amrgpu_inst_t ip;
+    ip.h_base = p.coarse_grid_h;
+    for (int k = 0; k < 4; ++k) ip.bc[k] = p.bc[k];
+    ip.cfl = p.cfl;
+    ip.dt_cap = p.dt;
+    ip.t_end = p.t_end;
+    rc = amrgpu_setup(g, n, p.time_scheme, 1, &ip, ncell, cell_info, NULL, nface, face_info, geom, soln);
+    if (!rc && o && o->tile_ppb > 0) rc = amrgpu_set_tile(g, o->tile_ppb, o->tile_minw, o->tile_design);
+    double *ind = NULL;
+    int *tags = NULL;
+    int cap = 0, istep = 0;
+    double t = 0.0;
+    s.wall_regrid += now_s() - t0;
+    while (!rc) {
+        long long nsteps = (long long)p.time_steps - istep;
+        long long r = steps_to(istep, o ? o->regrid_interval : p.regrid_interval);
+        if (r < nsteps) nsteps = r;
+        if (nsteps <= 0) break;
+        if (p.t_end > 0.0 && nsteps > 16) nsteps = 16;       /* poll the physical time */
+        double t1 = now_s();
+        double resnorm;
+        rc = amrgpu_run(g, (int)nsteps, 1, &resnorm);
+        if (rc) break;
+        istep += (int)nsteps;
+        s.steps += (int)nsteps;
+        s.cell_stages += (double)nsteps * ncell * n * n * stages[p.time_scheme];
+        if (p.t_end > 0.0) amrgpu_get_time(g, &t, NULL);
+        s.wall_gpu += now_s() - t1;
+        if (istep >= p.time_steps || (p.t_end > 0.0 && t >= p.t_end)) break;
+        int interval = o ? o->regrid_interval : p.regrid_interval;
+        if (interval <= 0 || istep % interval != 0) continue;
+        /* regrid: tags on the device, topology on the host, data on the device */
+        double t2 = now_s();
+        int nq = amr_forest_num_quads(f);
+        if (nq > cap) {
+            cap = nq + nq / 2;
+            free(ind); free(tags);
+            ind = (double *)malloc(sizeof(double) * (size_t)cap);
+            tags = (int *)malloc(sizeof(int) * (size_t)cap);
+        }
+        if (o && o->tagger) {
+            rc = o->tagger(o->tagger_ctx, g, f, tags);
+        } else {
+            rc = amrgpu_indicator(g, ind);
+            for (int k = 0; k < nq; ++k) tags[k] = ind[k] > (o ? o->tol : 0.1);
+        }
+        if (rc) break;
+        amr_forest_track_begin(f, 0);
+        amr_regrid_tags(f, tags, o ? o->allow_coarsening : 1);
+        if (!amr_forest_track_end(f)) { rc = AMRGPU_ERR_MESH; break; }
+        if (amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom,
+                                      &soln)) {
+            rc = AMRGPU_ERR_MESH;
+            break;
+        }
+        rc = amrgpu_remesh(g, ncell, cell_info, NULL, nface, face_info, amr_forest_track_src(f));
+        s.regrids++;
+        if (ncell > s.peak_quads) s.peak_quads = ncell;
+        s.wall_regrid += now_s() - t2;
+    }
+    if (!rc) {
+        amrgpu_get_time(g, &t, NULL);
+        rc = amrgpu_download_soln(g, amr_forest_soln(f));
+    }
+    s.t = t;
+    amrgpu_destroy(g);
+    free(ind);
+    free(tags);
     if (st) *st = s;
     return rc;
 }

@@ -26,6 +26,9 @@ struct amr_forest {
     int *cell_info, *face_info;
     double *geom;
     int nface;
+    /* regrid tracking: source of every quadrant's patch (see track_begin) */
+    int *src;
+    int track, track_data, track_ok;
 };
 
 static inline int qlen(int level) { return 1 << (AMR_MAXLEVEL - level); }
@@ -98,7 +101,7 @@ void amr_forest_destroy(amr_forest_t *f)
 {
     if (!f) return;
     free(f->q); free(f->key); free(f->soln); free(f->tag); free(f->type);
-    free(f->cell_info); free(f->face_info); free(f->geom);
+    free(f->cell_info); free(f->face_info); free(f->geom); free(f->src);
     free(f);
 }
 
@@ -278,6 +281,8 @@ int amr_forest_refine(amr_forest_t *f, const int *flags)
     double *soln = (double *)malloc(sizeof(double) * (size_t)ps * (size_t)nn);
     int *tag = (int *)malloc(sizeof(int) * (size_t)nn);
     int *type = (int *)malloc(sizeof(int) * (size_t)nn);
+    int *src = f->track ? (int *)malloc(sizeof(int) * (size_t)nn) : NULL;
+    const int move = !f->track || f->track_data;
     int j = 0;
     for (int k = 0; k < nq; ++k) {
         if (flags[k] && f->q[k].level < f->max_level) {
@@ -289,20 +294,23 @@ int amr_forest_refine(amr_forest_t *f, const int *flags)
                 q[j].y = f->q[k].y + (c >> 1) * h;
                 key[j] = morton(q[j].x, q[j].y);
                 /* solver_refine_operator_2d.F90:49-52: children = parent */
-                memcpy(&soln[(size_t)ps * j], &f->soln[(size_t)ps * k], sizeof(double) * ps);
+                if (move) memcpy(&soln[(size_t)ps * j], &f->soln[(size_t)ps * k], sizeof(double) * ps);
+                if (src) src[j] = f->src[k];
                 tag[j] = f->tag[k];
                 type[j] = f->type[k];
                 ++j;
             }
         } else {
             q[j] = f->q[k]; key[j] = f->key[k];
-            memcpy(&soln[(size_t)ps * j], &f->soln[(size_t)ps * k], sizeof(double) * ps);
+            if (move) memcpy(&soln[(size_t)ps * j], &f->soln[(size_t)ps * k], sizeof(double) * ps);
+            if (src) src[j] = f->src[k];
             tag[j] = f->tag[k]; type[j] = f->type[k];
             ++j;
         }
     }
     free(f->q); free(f->key); free(f->soln); free(f->tag); free(f->type);
     f->q = q; f->key = key; f->soln = soln; f->tag = tag; f->type = type;
+    if (src) { free(f->src); f->src = src; }
     f->nquad = nn; f->cap = nn;
     return cnt;
 }
@@ -335,12 +343,24 @@ int amr_forest_coarsen(amr_forest_t *f)
             int h = qlen(p.level - 1);
             p.level -= 1;
             p.x = (p.x / h) * h; p.y = (p.y / h) * h;
-            double *dst = &f->soln[(size_t)ps * j];
-            const double *c1 = &f->soln[(size_t)ps * k], *c2 = c1 + ps, *c3 = c2 + ps, *c4 = c3 + ps;
-            /* solver_coarsen_operator_2d.F90:49-50 */
-            for (int e = 0; e < ps; ++e) {
-                double s = c1[e] + c2[e] + c3[e] + c4[e];
-                dst[e] = 0.25 * s;
+            if (!f->track || f->track_data) {
+                double *dst = &f->soln[(size_t)ps * j];
+                const double *c1 = &f->soln[(size_t)ps * k], *c2 = c1 + ps, *c3 = c2 + ps, *c4 = c3 + ps;
+                /* solver_coarsen_operator_2d.F90:49-50 */
+                for (int e = 0; e < ps; ++e) {
+                    double s = c1[e] + c2[e] + c3[e] + c4[e];
+                    dst[e] = 0.25 * s;
+                }
+            }
+            if (f->track) {
+                /* a family is always four consecutive copies of an old family
+                 * (children refined in this regrid inherit their parent's tag
+                 * and are never coarsened in it) */
+                int s0 = f->src[k];
+                if (s0 < 0 || f->src[k + 1] != s0 + 1 || f->src[k + 2] != s0 + 2 ||
+                    f->src[k + 3] != s0 + 3)
+                    f->track_ok = 0;
+                f->src[j] = -(s0 + 1);
             }
             f->q[j] = p; f->key[j] = morton(p.x, p.y);
             f->tag[j] = 0;
@@ -349,7 +369,9 @@ int amr_forest_coarsen(amr_forest_t *f)
         } else {
             if (j != k) {
                 f->q[j] = f->q[k]; f->key[j] = f->key[k];
-                memmove(&f->soln[(size_t)ps * j], &f->soln[(size_t)ps * k], sizeof(double) * ps);
+                if (!f->track || f->track_data)
+                    memmove(&f->soln[(size_t)ps * j], &f->soln[(size_t)ps * k], sizeof(double) * ps);
+                if (f->track) f->src[j] = f->src[k];
                 f->tag[j] = f->tag[k]; f->type[j] = f->type[k];
             }
             ++j; ++k;
@@ -404,4 +426,23 @@ int amr_forest_is_balanced(const amr_forest_t *f)
     int any = balance_flags(f, flags);
     free(flags);
     return !any;
+}
+
+void amr_forest_track_begin(amr_forest_t *f, int keep_data)
+{
+    free(f->src);
+    f->src = (int *)malloc(sizeof(int) * (size_t)(f->nquad > 0 ? f->nquad : 1));
+    for (int k = 0; k < f->nquad; ++k) f->src[k] = k;
+    f->track = 1;
+    f->track_data = keep_data;
+    f->track_ok = 1;
+}
+
+const int *amr_forest_track_src(const amr_forest_t *f) { return f->src; }
+
+int amr_forest_track_end(amr_forest_t *f)
+{
+    f->track = 0;
+    f->track_data = 1;
+    return f->track_ok;
 }

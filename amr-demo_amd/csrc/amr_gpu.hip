@@ -832,6 +832,83 @@ __global__ void soa_to_aos_kernel(const double *soa, double *aos, long long ncel
     for (int f = 0; f < 4; ++f) d[f] = soa[(q * NFLD + f) * n2 + c];
 }
 
+// Project-defined density-gradient indicator, one thread per sub-cell: the
+// largest |rho_a - rho_b| / (0.5 (rho_a + rho_b)) over the cell's +x and +y
+// neighbours in its patch and, on patch edges, over the index-aligned edge
+// cells of the face neighbours (both fine ones on a coarse side); reduced to
+// one value per quadrant.  Max is order-free, so the result equals the host
+// indicator bit for bit.
+__device__ __forceinline__ double rel_jump(double a, double b) { return fabs(a - b) / (0.5 * (a + b)); }
+
+template <int N, int PPB>
+__global__ __launch_bounds__((Tile<N, PPB>::BLK)) void indicator_kernel(MeshDev m, const double *U,
+                                                                        double *ind)
+{
+    constexpr int N2 = N * N;
+    __shared__ unsigned long long pmax[PPB];
+    const int tid = threadIdx.x;
+    const int p = tid / N2, c = tid % N2, ci = c % N, cj = c / N;
+    const int quad = blockIdx.x * PPB + p;
+    const bool active = quad < m.nquad;
+    if (tid < PPB) pmax[tid] = 0ull;
+    __syncthreads();
+    if (active) {
+        const double *Q = U + (size_t)quad * NFLD * N2;      // field 0 plane
+        const double r = Q[c];
+        double v = 0.0;
+        if (ci + 1 < N) v = fmax(v, rel_jump(r, Q[c + 1]));
+        if (cj + 1 < N) v = fmax(v, rel_jump(r, Q[c + N]));
+        const bool on[4] = {ci == 0, ci == N - 1, cj == 0, cj == N - 1};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (!on[s]) continue;
+            const int k = (s < 2) ? cj : ci;
+            int ec;
+            switch (s) {
+            case 0: ec = k * N + (N - 1); break;
+            case 1: ec = k * N; break;
+            case 2: ec = (N - 1) * N + k; break;
+            default: ec = k; break;
+            }
+            const int na = m.nbr[quad * 8 + 2 * s], nb = m.nbr[quad * 8 + 2 * s + 1];
+            if (na >= 0) v = fmax(v, rel_jump(r, U[(size_t)na * NFLD * N2 + ec]));
+            if (nb >= 0) v = fmax(v, rel_jump(r, U[(size_t)nb * NFLD * N2 + ec]));
+        }
+        atomicMax(&pmax[p], d2bits(v));    // non-negative doubles order as integers
+    }
+    __syncthreads();
+    if (tid < PPB && blockIdx.x * PPB + tid < m.nquad) ind[blockIdx.x * PPB + tid] = bits2d(pmax[tid]);
+}
+
+// New state after a regrid, one thread per new sub-cell: src >= 0 copies
+// old quadrant src (solver_refine_operator_2d.F90:49-52 copies the parent);
+// src < 0 averages the old family -src-1.. (solver_coarsen_operator_2d.F90:
+// 49-50: sum of the four children, then times 0.25).
+__global__ void remap_kernel(const double *Uold, double *Unew, const int *src, long long ncells, int n2)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncells) return;
+    const long long q = t / n2, c = t % n2;
+    const int s = src[q];
+    double *d = Unew + (size_t)q * NFLD * n2 + c;
+    if (s >= 0) {
+        const double *o = Uold + (size_t)s * NFLD * n2 + c;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) d[f * n2] = o[f * n2];
+    } else {
+        const long long k = -(long long)s - 1;
+        const double *o = Uold + (size_t)k * NFLD * n2 + c;
+        const size_t ps = (size_t)NFLD * n2;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            double sum = o[f * n2] + o[ps + f * n2];
+            sum = sum + o[2 * ps + f * n2];
+            sum = sum + o[3 * ps + f * n2];
+            d[f * n2] = 0.25 * sum;
+        }
+    }
+}
+
 __global__ void copy_kernel(const double4 *__restrict__ src, double4 *__restrict__ dst, long long n)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -878,6 +955,8 @@ struct amrgpu_ctx {
     unsigned long long *d_slot = nullptr;
     double *d_norm = nullptr;
     double *d_stage = nullptr;      // AoS staging for up/download
+    int *d_map = nullptr;           // regrid source map
+    double *d_ind = nullptr;        // per-quadrant indicator
     int nblocks = 0;
     int variant = -1;
     int cap_quad = 0, cap_inst = 0, cap_blocks = 0;
@@ -969,6 +1048,23 @@ int launch_cfl(amrgpu_ctx *c, const double *U)
     case 4: hipLaunchKernelGGL((cfl_kernel<4, 8>), dim3(nblk), dim3(128), 0, c->stream, m, U, c->d_slot); break;
     case 8: hipLaunchKernelGGL((cfl_kernel<8, 4>), dim3(nblk), dim3(256), 0, c->stream, m, U, c->d_slot); break;
     case 16: hipLaunchKernelGGL((cfl_kernel<16, 1>), dim3(nblk), dim3(256), 0, c->stream, m, U, c->d_slot); break;
+    default: set_err("unsupported nelem_subgrid %d", c->n); return AMRGPU_ERR_ARG;
+    }
+    HIPCHK(hipGetLastError());
+    return AMRGPU_OK;
+}
+
+int launch_indicator(amrgpu_ctx *c, double *d_ind)
+{
+    int ppb = ppb_of(c->n);
+    int nblk = (c->nquad + ppb - 1) / ppb;
+    MeshDev m = mesh_of(c);
+    const double *U = c->U[c->cur];
+    switch (c->n) {
+    case 2: hipLaunchKernelGGL((indicator_kernel<2, 16>), dim3(nblk), dim3(64), 0, c->stream, m, U, d_ind); break;
+    case 4: hipLaunchKernelGGL((indicator_kernel<4, 8>), dim3(nblk), dim3(128), 0, c->stream, m, U, d_ind); break;
+    case 8: hipLaunchKernelGGL((indicator_kernel<8, 4>), dim3(nblk), dim3(256), 0, c->stream, m, U, d_ind); break;
+    case 16: hipLaunchKernelGGL((indicator_kernel<16, 1>), dim3(nblk), dim3(256), 0, c->stream, m, U, d_ind); break;
     default: set_err("unsupported nelem_subgrid %d", c->n); return AMRGPU_ERR_ARG;
     }
     HIPCHK(hipGetLastError());
@@ -1081,6 +1177,8 @@ void free_mesh(amrgpu_ctx *c)
     for (int i = 0; i < 3; ++i) (void)hipFree(c->U[i]);
     (void)hipFree(c->Acc); (void)hipFree(c->d_dt_cur); (void)hipFree(c->d_t); (void)hipFree(c->d_slot);
     (void)hipFree(c->d_norm); (void)hipFree(c->d_stage);
+    (void)hipFree(c->d_map); (void)hipFree(c->d_ind);
+    c->d_map = nullptr; c->d_ind = nullptr;
     c->d_nbr = c->d_level = c->d_inst = nullptr; c->d_ip = nullptr;
     c->U[0] = c->U[1] = c->U[2] = nullptr; c->Acc = nullptr;
     c->d_dt_cur = c->d_t = nullptr; c->d_slot = nullptr; c->d_norm = nullptr;
@@ -1170,6 +1268,81 @@ void amrgpu_destroy(amrgpu_ctx *c)
 int amrgpu_num_cells(const amrgpu_ctx *c) { return c ? c->nquad * c->n * c->n : 0; }
 int amrgpu_num_quads(const amrgpu_ctx *c) { return c ? c->nquad : 0; }
 
+static int validate_cells(int n2, int ninst, int ncell, const int *cell_info, const int *cell_inst,
+                          const char *who)
+{
+    for (int q = 0; q < ncell; ++q) {
+        if (cell_info[3 * q + 2] != 4 * n2 * q) {
+            set_err("%s: cell %d soln offset %d is not patch-contiguous", who, q, cell_info[3 * q + 2]);
+            return AMRGPU_ERR_MESH;
+        }
+        int lv = cell_info[3 * q + 1];
+        if (lv < 0 || lv > 29) { set_err("%s: cell %d level %d", who, q, lv); return AMRGPU_ERR_MESH; }
+        if (cell_inst && (cell_inst[q] < 0 || cell_inst[q] >= ninst)) {
+            set_err("%s: cell %d instance %d", who, q, cell_inst[q]);
+            return AMRGPU_ERR_ARG;
+        }
+    }
+    return AMRGPU_OK;
+}
+
+// Buffers are kept across setups and remeshes (an adaptive run changes the
+// mesh once per regrid interval) and grown with 25% headroom when the mesh
+// outgrows them.  *fresh tells whether they were reallocated.
+static int ensure_capacity(amrgpu_ctx *c, int n, int ncell, int ninst, bool need_acc, bool *fresh)
+{
+    const int n2 = n * n;
+    const size_t need_state = (size_t)ncell * 4 * n2;
+    const bool reuse = c->U[0] && c->n == n && ncell <= c->cap_quad && ninst <= c->cap_inst &&
+                       need_state <= c->cap_state && (!need_acc || c->Acc);
+    *fresh = !reuse;
+    if (reuse) return AMRGPU_OK;
+    free_mesh(c);
+    const int capq = ncell + ncell / 4 + 16;
+    c->cap_quad = capq;
+    c->cap_inst = ninst;
+    c->cap_state = (size_t)capq * 4 * n2;
+    c->cap_blocks = capq + 64;   // enough for one patch per workgroup
+    const size_t cq = (size_t)c->cap_quad, cs = c->cap_state;
+    HIPCHK(hipMalloc(&c->d_nbr, sizeof(int) * 8 * cq));
+    HIPCHK(hipMalloc(&c->d_level, sizeof(int) * cq));
+    HIPCHK(hipMalloc(&c->d_inst, sizeof(int) * cq));
+    HIPCHK(hipMalloc(&c->d_ip, sizeof(InstDev) * c->cap_inst));
+    for (int i = 0; i < 3; ++i) HIPCHK(hipMalloc(&c->U[i], sizeof(double) * cs));
+    if (need_acc) HIPCHK(hipMalloc(&c->Acc, sizeof(double) * cs));
+    HIPCHK(hipMalloc(&c->d_dt_cur, sizeof(double) * c->cap_inst));
+    HIPCHK(hipMalloc(&c->d_t, sizeof(double) * c->cap_inst));
+    HIPCHK(hipMalloc(&c->d_slot, sizeof(unsigned long long) * c->cap_inst * NSLOT));
+    HIPCHK(hipMalloc(&c->d_norm, sizeof(double) * c->cap_blocks));
+    HIPCHK(hipMalloc(&c->d_stage, sizeof(double) * cs));
+    HIPCHK(hipMalloc(&c->d_map, sizeof(int) * cq));
+    HIPCHK(hipMalloc(&c->d_ind, sizeof(double) * cq));
+    return AMRGPU_OK;
+}
+
+// Upload the face table, levels and instances of a mesh already validated.
+static int upload_mesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst,
+                       const std::vector<int> &nbr)
+{
+    std::vector<int> level(ncell), qi(ncell, 0);
+    for (int q = 0; q < ncell; ++q) {
+        level[q] = cell_info[3 * q + 1];
+        if (cell_inst) qi[q] = cell_inst[q];
+    }
+    c->nquad = ncell;
+    if (c->variant < 0 || kVariants[c->variant].n != c->n) c->variant = default_variant(c->n);
+    c->nblocks = (ncell + kVariants[c->variant].ppb - 1) / kVariants[c->variant].ppb;
+    if (c->nblocks > c->cap_blocks || (size_t)ncell * 4 * c->n * c->n > c->cap_state) {
+        set_err("internal capacity error");
+        return AMRGPU_ERR_ARG;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_level, level.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_inst, qi.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMRGPU_OK;
+}
+
 int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
                  const amrgpu_inst_t *inst, int ncell, const int *cell_info,
                  const int *cell_inst, int nface, const int *face_info, const double *geom,
@@ -1189,45 +1362,19 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
         return AMRGPU_ERR_ARG;
     }
     const int n2 = nelem_subgrid * nelem_subgrid;
-    for (int q = 0; q < ncell; ++q) {
-        if (cell_info[3 * q + 2] != 4 * n2 * q) {
-            set_err("amrgpu_setup: cell %d soln offset %d is not patch-contiguous", q, cell_info[3 * q + 2]);
-            return AMRGPU_ERR_MESH;
-        }
-        int lv = cell_info[3 * q + 1];
-        if (lv < 0 || lv > 29) { set_err("amrgpu_setup: cell %d level %d", q, lv); return AMRGPU_ERR_MESH; }
-        if (cell_inst && (cell_inst[q] < 0 || cell_inst[q] >= ninst)) {
-            set_err("amrgpu_setup: cell %d instance %d", q, cell_inst[q]);
-            return AMRGPU_ERR_ARG;
-        }
-    }
+    int rc = validate_cells(n2, ninst, ncell, cell_info, cell_inst, "amrgpu_setup");
+    if (rc) return rc;
     std::vector<int> nbr;
-    int rc = build_nbr(ncell, nface, face_info, nbr);
+    rc = build_nbr(ncell, nface, face_info, nbr);
     if (rc) return rc;
 
     HIPCHK(hipStreamSynchronize(c->stream));
     free_graphs(c);
-    // Buffers are kept across setups (an adaptive run calls this once per
-    // regrid interval) and grown with 25% headroom when the mesh outgrows them.
-    const bool need_acc = (time_scheme == 4 || time_scheme == 5);
-    const size_t need_state = (size_t)ncell * 4 * n2;
-    const bool reuse = c->U[0] && ncell <= c->cap_quad && ninst <= c->cap_inst &&
-                       need_state <= c->cap_state && (!need_acc || c->Acc);
-    if (!reuse) {
-        free_mesh(c);
-        int capq = ncell + ncell / 4 + 16;
-        c->cap_quad = capq;
-        c->cap_inst = ninst;
-        c->cap_state = (size_t)capq * 4 * n2;
-        c->cap_blocks = capq + 64;   // enough for one patch per workgroup
-    }
-    c->n = nelem_subgrid; c->nquad = ncell; c->ninst = ninst; c->scheme = time_scheme;
+    bool fresh;
+    rc = ensure_capacity(c, nelem_subgrid, ncell, ninst, time_scheme == 4 || time_scheme == 5, &fresh);
+    if (rc) return rc;
+    c->n = nelem_subgrid; c->ninst = ninst; c->scheme = time_scheme;
     c->cur = 0;
-    std::vector<int> level(ncell), qi(ncell, 0);
-    for (int q = 0; q < ncell; ++q) {
-        level[q] = cell_info[3 * q + 1];
-        if (cell_inst) qi[q] = cell_inst[q];
-    }
     c->h_ip.resize(ninst);
     for (int i = 0; i < ninst; ++i) {
         c->h_ip[i].h_base = inst[i].h_base;
@@ -1236,37 +1383,87 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
         c->h_ip[i].dt_cap = inst[i].dt_cap;
         c->h_ip[i].t_end = inst[i].t_end;
     }
-    const size_t nstate = (size_t)ncell * 4 * n2;
-    if (c->variant < 0 || kVariants[c->variant].n != c->n) c->variant = default_variant(c->n);
-    c->nblocks = (ncell + kVariants[c->variant].ppb - 1) / kVariants[c->variant].ppb;
-    if (c->nblocks > c->cap_blocks || nstate > c->cap_state) {
-        set_err("amrgpu_setup: internal capacity error");
-        return AMRGPU_ERR_ARG;
-    }
-    if (!reuse) {
-        const size_t cq = (size_t)c->cap_quad, cs = c->cap_state;
-        HIPCHK(hipMalloc(&c->d_nbr, sizeof(int) * 8 * cq));
-        HIPCHK(hipMalloc(&c->d_level, sizeof(int) * cq));
-        HIPCHK(hipMalloc(&c->d_inst, sizeof(int) * cq));
-        HIPCHK(hipMalloc(&c->d_ip, sizeof(InstDev) * c->cap_inst));
-        for (int i = 0; i < 3; ++i) HIPCHK(hipMalloc(&c->U[i], sizeof(double) * cs));
-        if (need_acc) HIPCHK(hipMalloc(&c->Acc, sizeof(double) * cs));
-        HIPCHK(hipMalloc(&c->d_dt_cur, sizeof(double) * c->cap_inst));
-        HIPCHK(hipMalloc(&c->d_t, sizeof(double) * c->cap_inst));
-        HIPCHK(hipMalloc(&c->d_slot, sizeof(unsigned long long) * c->cap_inst * NSLOT));
-        HIPCHK(hipMalloc(&c->d_norm, sizeof(double) * c->cap_blocks));
-        HIPCHK(hipMalloc(&c->d_stage, sizeof(double) * cs));
-    }
-    HIPCHK(hipMemcpyAsync(c->d_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_level, level.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_inst, qi.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice, c->stream));
     std::vector<double> zeros(ninst, 0.0);
     HIPCHK(hipMemcpyAsync(c->d_t, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_dt_cur, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    rc = upload_mesh(c, ncell, cell_info, cell_inst, nbr);
+    if (rc) return rc;
     if (soln) return amrgpu_upload_soln(c, soln);
     return AMRGPU_OK;
+}
+
+int amrgpu_indicator(amrgpu_ctx *c, double *ind)
+{
+    if (!c || !ind || !c->nquad) { set_err("amrgpu_indicator: no mesh"); return AMRGPU_ERR_ARG; }
+    int rc = launch_indicator(c, c->d_ind);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ind, c->d_ind, sizeof(double) * c->nquad, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMRGPU_OK;
+}
+
+int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst, int nface,
+                  const int *face_info, const int *src)
+{
+    if (!c || !c->nquad || !cell_info || !face_info || !src || ncell <= 0 || nface < 0) {
+        set_err("amrgpu_remesh: bad arguments");
+        return AMRGPU_ERR_ARG;
+    }
+    const int n2 = c->n * c->n, nold = c->nquad;
+    for (int q = 0; q < ncell; ++q) {
+        const int s = src[q];
+        if ((s >= 0 && s >= nold) || (s < 0 && (-s - 1) + 3 >= nold)) {
+            set_err("amrgpu_remesh: quadrant %d source %d outside the old mesh (%d)", q, s, nold);
+            return AMRGPU_ERR_ARG;
+        }
+    }
+    int rc = validate_cells(n2, c->ninst, ncell, cell_info, cell_inst, "amrgpu_remesh");
+    if (rc) return rc;
+    std::vector<int> nbr;
+    rc = build_nbr(ncell, nface, face_info, nbr);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    free_graphs(c);
+    const long long ncells = (long long)ncell * n2;
+    const size_t old_state = (size_t)nold * 4 * n2;
+    const bool fits = ncell <= c->cap_quad && (size_t)ncell * 4 * n2 <= c->cap_state;
+    if (fits) {
+        // old state in U[cur], new state into the next buffer
+        const int nx = (c->cur + 1) % 3;
+        HIPCHK(hipMemcpyAsync(c->d_map, src, sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(remap_kernel, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, c->stream,
+                           c->U[c->cur], c->U[nx], c->d_map, ncells, n2);
+        HIPCHK(hipGetLastError());
+        c->cur = nx;
+    } else {
+        // keep the old state, time and dt in scratch allocations, regrow, remap
+        double *keep = nullptr;
+        int *map = nullptr;
+        std::vector<double> t(c->ninst), dt(c->ninst);
+        HIPCHK(hipMemcpyAsync(t.data(), c->d_t, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(dt.data(), c->d_dt_cur, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMalloc(&keep, sizeof(double) * old_state));
+        HIPCHK(hipMalloc(&map, sizeof(int) * ncell));
+        HIPCHK(hipMemcpyAsync(keep, c->U[c->cur], sizeof(double) * old_state, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(map, src, sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const int ninst = c->ninst;
+        bool fresh;
+        rc = ensure_capacity(c, c->n, ncell + ncell / 2, ninst, c->scheme == 4 || c->scheme == 5, &fresh);
+        if (rc) { (void)hipFree(keep); (void)hipFree(map); return rc; }
+        HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_t, t.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_dt_cur, dt.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(remap_kernel, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, c->stream,
+                           keep, c->U[0], map, ncells, n2);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        (void)hipFree(keep);
+        (void)hipFree(map);
+        c->cur = 0;
+    }
+    return upload_mesh(c, ncell, cell_info, cell_inst, nbr);
 }
 
 int amrgpu_upload_soln(amrgpu_ctx *c, const double *soln)

@@ -9,8 +9,11 @@ levels 3-7 refined by the density-gradient indicator, regrid every 10 steps,
 2:1 balance, transmissive boundaries, SSP-RK3 with input CFL 0.4.
 
 Setup (untimed): the adaptive problem is run on the GPU path through the C
-evolve loop (hpc_amr_evolve_solver + host regrid) to t = --t-end; the mesh
-and state reached there are the benchmark input, resident in HBM.
+evolve loop with the state resident in HBM (device indicator and remesh, host
+forest topology) to t = --t-end; the mesh and state reached there are the
+benchmark input, resident in HBM.  The end-to-end rate of that adaptive run is
+reported beside the headline (and, with --host-loop, that of the
+interval-by-interval path through hpc_amr_evolve_solver).
 
 A step = one SSP-RK3 step (3 fused residual + stage-update launches, CFL
 folded into the last) over that mesh.  value = sub-grid cells x 3 stages x
@@ -46,44 +49,26 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_riemann_mesh(amrhip, seed, lmin, lmax, n, tol, t_end, regrid):
+def build_riemann_mesh(amrhip, seed, lmin, lmax, n, tol, t_end, regrid, device=True):
     """run the adaptive Riemann problem on the GPU path to t_end; return the
-    forest (mesh + state) and run statistics"""
-    import ctypes as C
+    forest (mesh + state) and run statistics.  device=True keeps the state in
+    HBM across regrids (amr_evolve_device); False is the interval-by-interval
+    path through hpc_amr_evolve_solver with host regrids (amr_evolve)"""
     ic = amrhip.riemann4_ic(seed)
     f = amrhip.Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, n, lmin, lmax)
     f.apply_ic(ic)
     f.regrid_initial(ic, tol)
-    L = amrhip.lib()
     amrhip.set_params(nelem_subgrid=n, time_scheme=3, time_steps=10 ** 9, dt=1.0,
                       cfl=0.25 * 0.4 * 3, coarse_grid_h=1.0, regrid_interval=regrid,
                       checkpoint_interval=2 ** 31 - 1, visualization_interval=2 ** 31 - 1,
                       bc=(0, 0, 0, 0, 1, 1), t_end=t_end, verbose=0)
-    stats = EvolveStats()
-    L.amr_evolve.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_void_p]
-    L.amr_evolve.restype = C.c_int
     t0 = time.perf_counter()
-    rc = L.amr_evolve(f.h, tol, 1, C.byref(stats))
+    st = f.evolve_device(tol, regrid) if device else f.evolve(tol)
     wall = time.perf_counter() - t0
-    if rc != 0:
-        raise RuntimeError(f"amr_evolve failed rc={rc}: {amrhip.last_error()}")
-    return f, stats, wall
+    return f, st, wall
 
 
-def _stats_type():
-    import ctypes as C
-
-    class S(C.Structure):
-        _fields_ = [("steps", C.c_int), ("regrids", C.c_int), ("peak_quads", C.c_int),
-                    ("cell_stages", C.c_double), ("t", C.c_double), ("wall_gpu", C.c_double),
-                    ("wall_regrid", C.c_double)]
-    return S
-
-
-EvolveStats = _stats_type()
-
-
-def cpu_baseline(ci, fi, geom, soln, n, seconds, threads):
+def cpu_baseline(ci, fi, geom, soln, n, seconds, threads, h_base):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     best = None
@@ -91,7 +76,7 @@ def cpu_baseline(ci, fi, geom, soln, n, seconds, threads):
         if not os.path.exists(path):
             continue
         orc = pyoracle.Oracle(path)
-        p = orc.params(n, bc=(0, 0, 0, 0), cfl=0.3, dt=1.0, h_base=1.0, scheme=3)
+        p = orc.params(n, bc=(0, 0, 0, 0), cfl=0.3, dt=1.0, h_base=h_base, scheme=3)
         t0 = time.perf_counter()
         orc.run(p, ci, fi, geom, soln, 1, threads=threads)
         one = max(time.perf_counter() - t0, 1e-4)
@@ -105,6 +90,27 @@ def cpu_baseline(ci, fi, geom, soln, n, seconds, threads):
         if best is None or v > best[0]:
             best = (v, label, k, dt)
     return best
+
+
+def measured_traffic(workload, cells, tile):
+    """HBM bytes per stage-kernel launch from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE per dispatch, scaled
+    by the copy kernel's reading of a known size), when it was taken on the
+    same mesh size; None otherwise (the summary names the tile it used)"""
+    path = os.path.join(ROOT, "profiles", f"stage_kernel_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    try:
+        if int(kv["cells"]) != cells:
+            return None
+        return int(float(kv["hbm_bytes_per_launch"])), kv.get("tile", "")
+    except (KeyError, ValueError):
+        return None
 
 
 def main():
@@ -122,6 +128,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile", type=str, default="", help="ppb,minw,design (skips the autotuner)")
+    ap.add_argument("--workload", choices=["riemann4", "vortex"], default="riemann4",
+                    help="riemann4: the adaptive Riemann problem the metric is quoted on; "
+                         "vortex: uniform periodic isentropic vortex at --level (4 = config 1)")
+    ap.add_argument("--level", type=int, default=4)
+    ap.add_argument("--host-loop", action="store_true",
+                    help="also time the interval-by-interval host-regrid loop")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (the whole forest lives on one "
@@ -131,20 +143,39 @@ def main():
 
     import amrhip
     amrhip.lib()
-    log(f"setup: adaptive Riemann seed={args.seed} levels {args.lmin}-{args.lmax} n={args.n} "
-        f"to t={args.t_end}")
-    f, st, wall = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, args.n, args.tol,
-                                     args.t_end, 10)
+    if args.workload == "riemann4":
+        log(f"setup: adaptive Riemann seed={args.seed} levels {args.lmin}-{args.lmax} n={args.n} "
+            f"to t={args.t_end}")
+        f, st, wall = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, args.n, args.tol,
+                                         args.t_end, 10, device=True)
+        host_run = None
+        if args.host_loop:
+            _, sth, wallh = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, args.n,
+                                               args.tol, args.t_end, 10, device=False)
+            host_run = (sth, wallh)
+        h_base, bc = 1.0, (0, 0, 0, 0)
+        workload = (f"riemann4 seed {args.seed}, adaptive levels {args.lmin}-{args.lmax}, "
+                    f"{args.n}x{args.n} FP64 patches, SSP-RK3 CFL 0.4, mesh at t={st.t:.3f}")
+    else:
+        f = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 10.0, args.n, args.level, args.level)
+        f.apply_ic(amrhip.vortex_ic(5.0, 5.0, 5.0))
+        st, wall, host_run = None, 0.0, None
+        h_base, bc = 10.0, (0, 0, 0, 0)
+        workload = (f"isentropic vortex strength 5, periodic [0,10]^2, uniform level {args.level}, "
+                    f"{args.n}x{args.n} FP64 patches, SSP-RK3 CFL 0.4")
     ci, fi, geom, soln = f.external()
     nq = f.nquad
     cells = nq * args.n * args.n
     lv = f.levels()
-    log(f"setup done: {st.steps} steps, {st.regrids} regrids, t={st.t:.4f}, peak {st.peak_quads} quads, "
-        f"final {nq} quads / {cells} cells, levels {lv.min()}-{lv.max()}, {wall:.2f} s "
-        f"(gpu {st.wall_gpu:.2f} s, regrid+boundary {st.wall_regrid:.2f} s)")
+    if st is not None:
+        log(f"setup done: {st.steps} steps, {st.regrids} regrids, t={st.t:.4f}, peak {st.peak_quads} "
+            f"quads, final {nq} quads / {cells} cells, levels {lv.min()}-{lv.max()}, {wall:.2f} s "
+            f"(gpu {st.wall_gpu:.2f} s, regrid+boundary {st.wall_regrid:.2f} s)")
+    else:
+        log(f"setup done: {nq} quads / {cells} cells")
 
     g = amrhip.Gpu()
-    g.setup(args.n, 3, [amrhip.make_inst(1.0, (0, 0, 0, 0), 0.3, 1.0, 0.0)], ci, fi, geom, soln)
+    g.setup(args.n, 3, [amrhip.make_inst(h_base, bc, 0.3, 1.0, 0.0)], ci, fi, geom, soln)
     tune = None
     if args.tile:
         g.set_tile(*(int(x) for x in args.tile.split(",")))
@@ -184,9 +215,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"riemann4 seed {args.seed}, adaptive levels {args.lmin}-{args.lmax}, "
-                        f"{args.n}x{args.n} FP64 patches, SSP-RK3 CFL 0.4, mesh at t={st.t:.3f}",
-            "quads": nq, "cells": cells, "peak_quads_in_run": st.peak_quads,
+            "workload": workload,
+            "quads": nq, "cells": cells, "peak_quads_in_run": st.peak_quads if st else nq,
             "levels": [int(lv.min()), int(lv.max())], "parallelism": "1 GPU, whole forest",
             "tile": {"patches_per_workgroup": tile[0], "min_waves_per_simd": tile[1],
                      "design": ["face task list", "paired faces"][tile[2]],
@@ -205,15 +235,29 @@ def main():
             "stage_launch_us": [round(s * 1e3, 3) for s in stage_ms],
             "copy_probe_GBs": round(probe, 1) if probe else None,
         },
-        "adaptive_run": {
+    }
+    if st is not None:
+        out["adaptive_run"] = {
+            "loop": "state resident in HBM, device indicator + remesh, host topology",
             "steps": st.steps, "regrids": st.regrids, "t": round(st.t, 6),
             "cell_stages": st.cell_stages, "wall_s": round(wall, 3),
+            "gpu_s": round(st.wall_gpu, 3), "regrid_s": round(st.wall_regrid, 3),
             "end_to_end_Mcell_updates_per_s": round(st.cell_stages / wall / 1e6, 3),
-        },
-    }
+        }
+        if host_run is not None:
+            sth, wallh = host_run
+            out["adaptive_run_host_regrid"] = {
+                "loop": "hpc_amr_evolve_solver per interval, host regrid",
+                "steps": sth.steps, "regrids": sth.regrids, "wall_s": round(wallh, 3),
+                "end_to_end_Mcell_updates_per_s": round(sth.cell_stages / wallh / 1e6, 3),
+            }
+    traffic = measured_traffic(args.workload, cells, tile)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic[0]
+        out["roofline"]["traffic_tile"] = traffic[1]
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        best = cpu_baseline(ci, fi, geom, soln, args.n, args.cpu_seconds, threads)
+        best = cpu_baseline(ci, fi, geom, soln, args.n, args.cpu_seconds, threads, h_base)
         if best:
             v, label, k, dt = best
             out["cpu_baseline"] = {

@@ -58,6 +58,8 @@ void amr_indicator_gradient(amr_forest_t *f, double *ind);
  * (below max_level), coarsen untagged families (unless initial), 2:1 balance.
  * Returns the new number of quadrants. */
 int amr_regrid(amr_forest_t *f, double tol, int allow_coarsening);
+/* The same regrid driven by given per-quadrant tags (non-zero: refine). */
+int amr_regrid_tags(amr_forest_t *f, const int *tags, int allow_coarsening);
 /* Initial refinement (regrid_feature with initial = 1): repeatedly tag and
  * refine, re-evaluating the initial condition on new quadrants
  * (initialize_quadrant_data), then balance. */
@@ -81,6 +83,27 @@ typedef struct {
  * hpc_amr_initialize_solver; tol is the density-gradient tagging tolerance.
  * Returns 0, or the physics module's error code. */
 int amr_evolve(amr_forest_t *f, double tol, int allow_coarsening, amr_evolve_stats_t *stats);
+
+struct amrgpu_ctx;
+/* Tag source for amr_evolve_device: fills tags[nquad] (non-zero: refine)
+ * for the device state of ctx; returns 0 or an error code. */
+typedef int (*amr_tagger_fn)(void *user, struct amrgpu_ctx *ctx, amr_forest_t *f, int *tags);
+
+typedef struct {
+    double tol;                  /* density-gradient tolerance */
+    int    allow_coarsening;
+    int    regrid_interval;      /* steps between regrids (<= 0: never) */
+    amr_tagger_fn tagger;        /* NULL: device density-gradient indicator */
+    void  *tagger_ctx;
+    int    tile_ppb, tile_minw, tile_design;   /* stage-kernel tile (ppb <= 0: default) */
+} amr_evolve_opts_t;
+
+/* The evolve loop with the state resident in HBM across regrids: tags are
+ * computed on the device, the forest topology is regridded on the host in
+ * tracking mode, the state is remapped on the device (amrgpu_remesh).  Time
+ * stepping settings come from the physics parameters; the final state is
+ * downloaded into the forest. */
+int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *opts, amr_evolve_stats_t *stats);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,18 @@ int amr_forest_balance(amr_forest_t *f);
 /* 1 if the forest is 2:1 balanced across faces and corners. */
 int amr_forest_is_balanced(const amr_forest_t *f);
 
+/* Regrid tracking.  Between track_begin and track_end, refine / coarsen /
+ * balance record where each quadrant's patch comes from; with keep_data = 0
+ * the host patches are not moved at all (the state lives on the device and
+ * is remapped there).  amr_forest_track_src() gives, per current quadrant,
+ * src >= 0: copy of old quadrant src (refine copies the parent, as
+ * solver_refine_operator_2d); src < 0: element-wise mean of the old family
+ * -src-1 .. -src+2 (solver_coarsen_operator_2d).  track_end returns 1 when
+ * every quadrant has such a source. */
+void amr_forest_track_begin(amr_forest_t *f, int keep_data);
+const int *amr_forest_track_src(const amr_forest_t *f);
+int amr_forest_track_end(amr_forest_t *f);
+
 /* Leaf index containing the point (tree, x, y) or -1. */
 int amr_forest_find_leaf(const amr_forest_t *f, int tree, int x, int y);
 

@@ -131,6 +131,15 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme,
                  int nface, const int *face_info, const double *geom, const double *soln);
 int amrgpu_upload_soln(amrgpu_ctx *c, const double *soln);
 int amrgpu_download_soln(amrgpu_ctx *c, double *soln);
+/* Project-defined density-gradient indicator of the current state, one
+ * value per quadrant (host out); tagging is ind > tolerance. */
+int amrgpu_indicator(amrgpu_ctx *c, double *ind);
+/* Replace the mesh after a regrid without leaving the device: src[ncell]
+ * from amr_forest_track_src (copy of an old quadrant, or mean of an old
+ * family); cell_info/face_info/cell_inst describe the new mesh.  Instance
+ * settings and time are kept. */
+int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst, int nface,
+                  const int *face_info, const int *src);
 /* Residual rhs = -R/V of the current state (reference layout, host out). */
 int amrgpu_rhs(amrgpu_ctx *c, double *rhs);
 /* CFL minimum of the current state per instance (no dt cap). */

@@ -140,3 +140,45 @@ def test_regrid_refines_discontinuity():
         xlo, xhi, ylo, yhi = corners(geom, k)
         near = min(abs(xlo - 0.5), abs(xhi - 0.5), abs(ylo - 0.5), abs(yhi - 0.5))
         assert near < 2 * (xhi - xlo) + 1e-12
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_regrid_tracking_reproduces_host_data(seed):
+    """the recorded source map (copy / family mean) applied to the old data
+    equals the data the host operators moved, bit for bit"""
+    rng = np.random.default_rng(100 + seed)
+    n = 4
+    f = amrhip.Forest(2, 1, (1, 0), 0.0, 0.0, 1.0, n, 1, 5)
+    for _ in range(3):
+        f.refine((rng.uniform(size=f.nquad) < 0.3).astype(np.int32))
+        f.balance()
+    s = f.soln()
+    s[:] = rng.uniform(0.5, 1.5, s.shape)
+    old = s.copy()
+    tags = (rng.uniform(size=f.nquad) < 0.2).astype(np.int32)
+    f.track_begin(keep_data=True)
+    f.regrid_tags(tags, allow_coarsening=True)
+    ok, src = f.track_end()
+    assert ok
+    assert np.any(src < 0) and f.is_balanced()
+    new = f.soln()
+    for k, sk in enumerate(src):
+        if sk >= 0:
+            exp = old[sk]
+        else:
+            b = -sk - 1
+            exp = 0.25 * (((old[b] + old[b + 1]) + old[b + 2]) + old[b + 3])
+        assert np.array_equal(new[k], exp)
+
+
+def test_tracking_without_data_gives_same_topology():
+    rng = np.random.default_rng(7)
+    fa = amrhip.Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, 4, 2, 5)
+    fb = amrhip.Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, 4, 2, 5)
+    for _ in range(2):
+        fl = (rng.uniform(size=fa.nquad) < 0.3).astype(np.int32)
+        fa.refine(fl); fa.balance(); fb.refine(fl); fb.balance()
+    tags = (rng.uniform(size=fa.nquad) < 0.2).astype(np.int32)
+    fa.track_begin(keep_data=True); fa.regrid_tags(tags); oka, sa = fa.track_end()
+    fb.track_begin(keep_data=False); fb.regrid_tags(tags); okb, sb = fb.track_end()
+    assert oka and okb and np.array_equal(sa, sb) and fa.quads() == fb.quads()

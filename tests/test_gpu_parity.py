@@ -229,3 +229,44 @@ def test_physics_go_gpu_interval(oracle):
     p = oracle.params(n, cfl=0.3, dt=1.0, h_base=10.0, scheme=3)
     q_ref, _, _ = oracle.run(p, ci, fi, geom, soln, 5)
     assert np.max(np.abs(q - q_ref) / np.maximum(np.abs(q_ref), 1.0)) <= TRAJ_TOL
+
+
+def test_device_indicator_matches_host(gpu):
+    for n in (4, 8, 16):
+        f = riemann_forest(n, seed=11, lmax=4 if n == 16 else 5)
+        ci, fi, geom, soln = f.external()
+        setup(gpu, n, 3, 1.0, (0, 0, 0, 0), 0.3, 1.0, ci, fi, geom, soln)
+        assert np.array_equal(gpu.indicator(), f.indicator())
+
+
+def test_device_remesh_matches_host_operators(gpu):
+    rng = np.random.default_rng(3)
+    f = riemann_forest(8, seed=12, lmax=5)
+    ci, fi, geom, soln = f.external()
+    setup(gpu, 8, 3, 1.0, (0, 0, 0, 0), 0.3, 1.0, ci, fi, geom, soln)
+    tags = (rng.uniform(size=f.nquad) < 0.3).astype(np.int32)
+    f.track_begin(keep_data=True)
+    f.regrid_tags(tags)
+    ok, src = f.track_end()
+    assert ok and np.any(src < 0)
+    ci2, fi2, geom2, soln2 = f.external()
+    gpu.remesh(ci2, fi2, src)
+    assert np.array_equal(gpu.soln(), soln2)
+
+
+@pytest.mark.parametrize("allow_coarsening", [True, False])
+def test_device_evolve_equals_host_regrid_loop(allow_coarsening):
+    """state resident in HBM across regrids gives bit for bit the result of
+    the interval-by-interval host path (download, host regrid, upload)"""
+    def make():
+        return riemann_forest(8, seed=21, lmin=2, lmax=5)
+    amrhip.set_params(nelem_subgrid=8, time_scheme=3, time_steps=60, dt=1.0, cfl=0.3,
+                      coarse_grid_h=1.0, regrid_interval=10, checkpoint_interval=2 ** 31 - 1,
+                      visualization_interval=2 ** 31 - 1, bc=(0, 0, 0, 0, 1, 1), t_end=0.0)
+    fa = make()
+    sa = fa.evolve(0.1, allow_coarsening)
+    fb = make()
+    sb = fb.evolve_device(0.1, 10, allow_coarsening)
+    assert sa.steps == sb.steps == 60 and sa.regrids == sb.regrids
+    assert fa.quads() == fb.quads()
+    assert np.array_equal(fa.soln(), fb.soln())

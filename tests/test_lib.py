@@ -18,6 +18,8 @@ def header_functions():
         txt = open(h).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", txt, flags=re.M):
+            if m.group(0).startswith("typedef"):
+                continue
             if m.group(1) not in ("if", "while", "for", "return", "sizeof"):
                 names.add(m.group(1))
     return names
