@@ -82,7 +82,31 @@ EXPORTS = {
     "amr_forest_track_end": (_i, [_vp]),
     "amrgpu_indicator": (_i, [_vp, _pd]),
     "amrgpu_remesh": (_i, [_vp, _i, _pi, _pi, _i, _pi, _pi]),
+    "amrgpu_device_state": (_i, [_vp, _vp, _vp, _pi, _pi, _vp]),
+    # include/amr_tagger.h
+    "amr_mlp_create": (_vp, [C.c_ulonglong]),
+    "amr_mlp_destroy": (None, [_vp]),
+    "amr_mlp_get_params": (_i, [_vp, _vp]),
+    "amr_mlp_set_params": (_i, [_vp, _vp]),
+    "amr_mlp_forward": (_i, [_vp, _vp, _i, _vp]),
+    "amr_mlp_grad": (_i, [_vp, _vp, _pi, _i, _vp, _vp, _vp]),
+    "amr_tagset_create": (_vp, []),
+    "amr_tagset_destroy": (None, [_vp]),
+    "amr_tagset_size": (_i, [_vp]),
+    "amr_tagset_append": (_i, [_vp, _vp, _pi, _pd, _i]),
+    "amr_tagset_get": (_i, [_vp, _i, _i, _vp, _pi, _pd]),
+    "amr_tagset_append_from": (_i, [_vp, _vp, _i, _i]),
+    "amrgpu_tagger_features": (_i, [_vp, _vp]),
+    "amrgpu_tagger_features_host": (_i, [_vp, _vp]),
+    "amr_mlp_train": (_i, [_vp, _vp, _i, _i, C.c_float, _vp, C.c_ulonglong, _vp]),
+    "amr_mlp_predict_set": (_i, [_vp, _vp, _pi]),
+    "amr_mlp_predict_ctx": (_i, [_vp, _vp, _pi]),
+    "amr_tagger_collect": (_i, [_vp, _vp, _vp, _pi]),
+    "amr_mlp_tagger": (_i, [_vp, _vp, _vp, _pi]),
 }
+
+MLP_NPARAM = 64 * 384 + 64 + 64 * 64 + 64 + 3 * 64 + 3
+TAG_COARSEN, TAG_KEEP, TAG_REFINE = 0, 1, 2
 
 
 class EvolveStats(C.Structure):
@@ -92,6 +116,18 @@ class EvolveStats(C.Structure):
 
 
 TAGGER_FN = C.CFUNCTYPE(_i, _vp, _vp, _vp, _pi)
+
+
+class CollectOpts(C.Structure):
+    """amr_collect_opts_t"""
+    _fields_ = [("tol", _d), ("coarsen_frac", _d), ("window", _i), ("label_stride", _i),
+                ("allow_coarsening", _i)]
+
+
+class MlpTaggerStruct(C.Structure):
+    """amr_mlp_tagger_t"""
+    _fields_ = [("mlp", _vp), ("tol", _d), ("union_plain", _i), ("predicted", C.c_longlong * 3),
+                ("plain_refine", C.c_longlong), ("added_by_plain", C.c_longlong)]
 
 
 class EvolveOpts(C.Structure):
@@ -285,18 +321,32 @@ class Forest:
         check(rc, "amr_evolve")
         return st
 
-    def evolve_device(self, tol, regrid_interval, allow_coarsening=True, tile=None, tagger=None):
-        """device-resident evolve loop; tagger: optional TAGGER_FN"""
+    def evolve_device(self, tol, regrid_interval, allow_coarsening=True, tile=None, tagger=None,
+                      tagger_ctx=None):
+        """device-resident evolve loop; tagger: optional TAGGER_FN or MlpTagger"""
         o = EvolveOpts()
         o.tol, o.allow_coarsening, o.regrid_interval = tol, int(allow_coarsening), regrid_interval
-        o.tagger = C.cast(tagger, _vp) if tagger is not None else None
-        o.tagger_ctx = None
+        if isinstance(tagger, MlpTagger):
+            o.tagger = C.cast(self.L.amr_mlp_tagger, _vp)
+            o.tagger_ctx = C.cast(C.pointer(tagger.s), _vp)
+        else:
+            o.tagger = C.cast(tagger, _vp) if tagger is not None else None
+            o.tagger_ctx = tagger_ctx
         if tile:
             o.tile_ppb, o.tile_minw, o.tile_design = tile
         st = EvolveStats()
         rc = self.L.amr_evolve_device(self.h, C.byref(o), C.byref(st))
         check(rc, "amr_evolve_device")
         return st
+
+    def collect(self, tagset, tol, window=40, label_stride=5, coarsen_frac=0.5, allow_coarsening=True):
+        """run to the physical stop collecting labelled snapshots (amr_tagger_collect)"""
+        o = CollectOpts()
+        o.tol, o.coarsen_frac, o.window, o.label_stride = tol, coarsen_frac, window, label_stride
+        o.allow_coarsening = int(allow_coarsening)
+        rows = C.c_int(0)
+        check(self.L.amr_tagger_collect(self.h, C.byref(o), tagset.h, C.byref(rows)), "amr_tagger_collect")
+        return rows.value
 
 
 def vortex_ic(strength=5.0, xo=5.0, yo=5.0, u=1.0, v=1.0, gamma=1.4, rho=1.0, p=1.0):
@@ -467,6 +517,14 @@ class Gpu:
     def ncells(self):
         return self.L.amrgpu_num_cells(self.h)
 
+    def nquad(self):
+        return self.L.amrgpu_num_quads(self.h)
+
+    def tagger_features(self):
+        X = np.zeros((self.nquad(), 384), np.uint16)
+        check(self.L.amrgpu_tagger_features_host(self.h, X.ctypes.data), "amrgpu_tagger_features_host")
+        return X
+
 
 def set_params(**kw):
     p = PhysicsParams()
@@ -497,3 +555,130 @@ def copy_probe(nbytes, reps=20):
     g = _d()
     check(lib().amrgpu_copy_probe(int(nbytes), reps, C.byref(g)), "amrgpu_copy_probe")
     return g.value
+
+
+# ---------------------------------------------------------------------------
+# learned tagger
+# ---------------------------------------------------------------------------
+class Mlp:
+    """amr_mlp_t: 384-64-64-3 BF16 perceptron on the MFMA units"""
+
+    def __init__(self, seed=0):
+        self.L = lib()
+        self.h = self.L.amr_mlp_create(seed)
+        if not self.h:
+            raise GpuError(f"amr_mlp_create failed: {last_error()}")
+
+    def close(self):
+        if self.h:
+            self.L.amr_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        P = np.zeros(MLP_NPARAM, np.float32)
+        check(self.L.amr_mlp_get_params(self.h, P.ctypes.data), "amr_mlp_get_params")
+        return P
+
+    def set_params(self, P):
+        P = np.ascontiguousarray(P, np.float32)
+        assert P.size == MLP_NPARAM
+        check(self.L.amr_mlp_set_params(self.h, P.ctypes.data), "amr_mlp_set_params")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.uint16)
+        out = np.zeros((X.shape[0], 3), np.float32)
+        check(self.L.amr_mlp_forward(self.h, X.ctypes.data, X.shape[0], out.ctypes.data), "amr_mlp_forward")
+        return out
+
+    def grad(self, X, y, cw=(1.0, 1.0, 1.0)):
+        X = np.ascontiguousarray(X, np.uint16)
+        y = np.ascontiguousarray(y, np.int32)
+        cw = np.ascontiguousarray(cw, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_float(0.0)
+        check(self.L.amr_mlp_grad(self.h, X.ctypes.data, ipt(y), X.shape[0], cw.ctypes.data, g.ctypes.data,
+                                  C.byref(loss)), "amr_mlp_grad")
+        return loss.value, g
+
+    def train(self, tagset, epochs, batch=4096, lr=1e-3, cw=None, seed=0):
+        loss = np.zeros(max(epochs, 1), np.float32)
+        cwp = None
+        if cw is not None:
+            cwa = np.ascontiguousarray(cw, np.float32)
+            cwp = cwa.ctypes.data
+        check(self.L.amr_mlp_train(self.h, tagset.h, epochs, batch, lr, cwp, seed, loss.ctypes.data),
+              "amr_mlp_train")
+        return loss[:epochs]
+
+    def predict(self, tagset):
+        out = np.zeros(tagset.size, np.int32)
+        check(self.L.amr_mlp_predict_set(self.h, tagset.h, ipt(out)), "amr_mlp_predict_set")
+        return out
+
+    def predict_ctx(self, gpu):
+        out = np.zeros(gpu.nquad(), np.int32)
+        check(self.L.amr_mlp_predict_ctx(self.h, gpu.h, ipt(out)), "amr_mlp_predict_ctx")
+        return out
+
+
+class Tagset:
+    """amr_tagset_t: labelled feature rows in device memory"""
+
+    def __init__(self):
+        self.L = lib()
+        self.h = self.L.amr_tagset_create()
+
+    def close(self):
+        if self.h:
+            self.L.amr_tagset_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def size(self):
+        return self.L.amr_tagset_size(self.h)
+
+    def append(self, X, y, ind=None):
+        X = np.ascontiguousarray(X, np.uint16)
+        y = np.ascontiguousarray(y, np.int32)
+        indp = dpt(np.ascontiguousarray(ind, float)) if ind is not None else None
+        check(self.L.amr_tagset_append(self.h, X.ctypes.data, ipt(y), indp, X.shape[0]), "amr_tagset_append")
+
+    def append_from(self, other, first=0, n=None):
+        n = other.size - first if n is None else n
+        check(self.L.amr_tagset_append_from(self.h, other.h, first, n), "amr_tagset_append_from")
+
+    def get(self, first=0, n=None, features=True):
+        n = self.size - first if n is None else n
+        X = np.zeros((n, 384), np.uint16) if features else None
+        y = np.zeros(n, np.int32)
+        ind = np.zeros(n)
+        check(self.L.amr_tagset_get(self.h, first, n, X.ctypes.data if features else None, ipt(y), dpt(ind)),
+              "amr_tagset_get")
+        return X, y, ind
+
+
+class MlpTagger:
+    """amr_mlp_tagger_t for Forest.evolve_device(tagger=...)"""
+
+    def __init__(self, mlp, tol, union_plain=True):
+        self.mlp = mlp
+        self.s = MlpTaggerStruct()
+        self.s.mlp = mlp.h
+        self.s.tol = tol
+        self.s.union_plain = int(union_plain)
+
+    def stats(self):
+        return {"predicted": list(self.s.predicted), "plain_refine": self.s.plain_refine,
+                "added_by_plain": self.s.added_by_plain}

@@ -1,0 +1,166 @@
+"""Learned refinement tagger: data collection, training, recall and the
+deployment comparison against the plain density-gradient indicator.
+
+Everything runs through the library's C ABI (include/amr_tagger.h): the solver
+produces the snapshots on the GPU, the network is trained on the GPU by the
+library's own BF16-MFMA forward and backward kernels, and the deployed tagger
+is called by the device evolve loop at each regrid.
+
+Problem family: the four-quadrant Riemann problem on [0,1]^2 with seeded
+states (riemann4_ic), 8x8 FP64 patches, SSP-RK3, effective CFL 0.3,
+transmissive walls, run to a physical stop.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import (Forest, Mlp, MlpTagger, Tagset, riemann4_ic, set_params, vortex_ic, TAG_REFINE)
+
+ROOT_LEN = 1 << 30
+
+
+def riemann_params(t_end, regrid=10):
+    set_params(nelem_subgrid=8, time_scheme=3, time_steps=10 ** 9, dt=1.0, cfl=0.25 * 0.4 * 3,
+               coarse_grid_h=1.0, regrid_interval=regrid, checkpoint_interval=2 ** 31 - 1,
+               visualization_interval=2 ** 31 - 1, bc=(0, 0, 0, 0, 1, 1), t_end=t_end, verbose=0)
+
+
+def riemann_forest(seed, lmin, lmax, tol=0.1):
+    ic = riemann4_ic(seed)
+    f = Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, 8, lmin, lmax)
+    f.apply_ic(ic)
+    f.regrid_initial(ic, tol)
+    return f
+
+
+def vortex_forest(lmin, lmax, tol=0.1, strength=5.0):
+    """the isentropic vortex in the periodic [0,10]^2 box, adaptive"""
+    ic = vortex_ic(strength=strength, xo=5.0, yo=5.0)
+    f = Forest(1, 1, (1, 1), 0.0, 0.0, 10.0, 8, lmin, lmax)
+    f.apply_ic(ic)
+    f.regrid_initial(ic, tol)
+    return f
+
+
+def collect(seeds, lmin=3, lmax=7, t_end=0.25, tol=0.1, window=40, label_stride=5, vortex_runs=0,
+            tagset=None, log=None):
+    """labelled snapshots from seeded Riemann runs (and adaptive vortex runs)"""
+    ts = tagset if tagset is not None else Tagset()
+    for s in seeds:
+        riemann_params(t_end)
+        f = riemann_forest(s, lmin, lmax, tol)
+        t0 = time.perf_counter()
+        rows = f.collect(ts, tol, window=window, label_stride=label_stride)
+        if log:
+            log(f"collect riemann seed {s}: {rows} rows in {time.perf_counter() - t0:.2f} s")
+    for k in range(vortex_runs):
+        set_params(nelem_subgrid=8, time_scheme=3, time_steps=10 ** 9, dt=1.0, cfl=0.3, coarse_grid_h=10.0,
+                   regrid_interval=10, checkpoint_interval=2 ** 31 - 1, visualization_interval=2 ** 31 - 1,
+                   bc=(0, 0, 0, 0, 1, 1), t_end=2.0 + k, verbose=0)
+        f = vortex_forest(2, 5, tol=0.02)
+        rows = f.collect(ts, 0.02, window=window, label_stride=label_stride)
+        if log:
+            log(f"collect vortex run {k}: {rows} rows")
+    return ts
+
+
+def class_counts(y):
+    return [int(np.sum(y == c)) for c in range(3)]
+
+
+def recall_report(mlp, ts, tol=0.1):
+    """tag recall of the network against the plain indicator's future flags
+    (label refine), beside the indicator-now predictor on the same rows"""
+    _, y, ind = ts.get(features=False)
+    pred = mlp.predict(ts)
+    fut = y == TAG_REFINE
+    now = ind > tol
+    net = pred == TAG_REFINE
+    nf = max(int(fut.sum()), 1)
+    rep = {
+        "rows": int(y.size),
+        "label_counts": class_counts(y),
+        "refine_labels": int(fut.sum()),
+        "recall_mlp": float(np.sum(net & fut) / nf),
+        "recall_indicator_now": float(np.sum(now & fut) / nf),
+        "recall_union": float(np.sum((net | now) & fut) / nf),
+        "refine_fraction_mlp": float(net.mean()),
+        "refine_fraction_indicator_now": float(now.mean()),
+        "precision_mlp": float(np.sum(net & fut) / max(int(net.sum()), 1)),
+        "precision_indicator_now": float(np.sum(now & fut) / max(int(now.sum()), 1)),
+        "accuracy_3class": float(np.mean(pred == y)),
+    }
+    return rep
+
+
+def train(ts, epochs=12, batch=4096, lr=2e-3, seed=0, cw=None):
+    m = Mlp(seed)
+    losses = m.train(ts, epochs=epochs, batch=batch, lr=lr, cw=cw, seed=seed)
+    return m, losses
+
+
+def density_on_grid(f, level):
+    """piecewise-constant density of the forest's patches sampled on the
+    uniform grid of `level` quadrants (8x8 cells each): (N, N) array, N =
+    8 * 2**level (tree 0 of a one-tree brick)"""
+    N = 8 * (1 << level)
+    out = np.zeros((N, N))
+    q = f.soln()                         # (nquad, 8, 8, 4) [quad][j][i][field]
+    quads = f.quads()
+    lv = np.array([qq[1] for qq in quads])
+    xs = np.array([qq[2] for qq in quads], np.int64)
+    ys = np.array([qq[3] for qq in quads], np.int64)
+    for L in np.unique(lv):
+        if L > level:
+            raise ValueError("forest finer than the sampling grid")
+        idx = np.nonzero(lv == L)[0]
+        r = 1 << (level - L)             # fine cells per patch cell
+        x0 = (xs[idx] >> (30 - level)) * 8
+        y0 = (ys[idx] >> (30 - level)) * 8
+        rho = q[idx, :, :, 0]            # (k, 8, 8) [j][i]
+        blk = np.repeat(np.repeat(rho, r, axis=1), r, axis=2)   # (k, 8r, 8r)
+        for k in range(idx.size):
+            out[y0[k]:y0[k] + 8 * r, x0[k]:x0[k] + 8 * r] = blk[k]
+    return out
+
+
+def uniform_reference(seed, level, t_end, log=None):
+    """plain-method FP64 solution on the uniform forest of the finest level"""
+    riemann_params(t_end, regrid=2 ** 31 - 1)
+    ic = riemann4_ic(seed)
+    f = Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, 8, level, level)
+    f.apply_ic(ic)
+    t0 = time.perf_counter()
+    st = f.evolve_device(0.1, 0)
+    if log:
+        log(f"uniform level-{level} reference: {f.nquad * 64} cells, {st.steps} steps, "
+            f"{time.perf_counter() - t0:.1f} s")
+    return density_on_grid(f, level), st
+
+
+def l1_error(rho, ref):
+    N = ref.shape[0]
+    return float(np.abs(rho - ref).sum() / (N * N))
+
+
+def deploy(seed, lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None):
+    """one adaptive run on the device loop; end-to-end rate and L1 error"""
+    riemann_params(t_end, regrid=interval)
+    f = riemann_forest(seed, lmin, lmax, tol)
+    t0 = time.perf_counter()
+    st = f.evolve_device(tol, interval, tagger=tagger)
+    wall = time.perf_counter() - t0
+    out = {
+        "regrid_interval": interval,
+        "steps": st.steps, "regrids": st.regrids, "t": st.t,
+        "peak_cells": st.peak_quads * 64, "final_cells": f.nquad * 64,
+        "cell_stages": st.cell_stages, "wall_s": wall, "gpu_s": st.wall_gpu, "regrid_s": st.wall_regrid,
+        "end_to_end_Mcell_upd_s": st.cell_stages / wall / 1e6,
+    }
+    if ref is not None:
+        out["l1_density"] = l1_error(density_on_grid(f, ref_level), ref)
+    if isinstance(tagger, MlpTagger):
+        out["tagger"] = tagger.stats()
+    return out, f

@@ -5,6 +5,7 @@
 #define _POSIX_C_SOURCE 200809L
 #include "amr_driver.h"
 #include "amr_hip.h"
+#include "amr_internal.h"
 
 #include <time.h>
 
@@ -215,10 +216,13 @@ int amr_regrid_tags(amr_forest_t *f, const int *tags, int allow_coarsening)
 {
     int nq = amr_forest_num_quads(f);
     int *tag = amr_forest_tags(f);
-    /* tag_reset + regrid_feature */
-    for (int k = 0; k < nq; ++k) tag[k] = tags[k] ? 1 : 0;
+    /* tag_reset + regrid_feature; a "keep" tag (2) refines nothing but, being
+     * non-zero, stops tag_coarsen from merging the family */
     int *flags = (int *)malloc(sizeof(int) * (size_t)(nq > 0 ? nq : 1));
-    memcpy(flags, tag, sizeof(int) * (size_t)nq);
+    for (int k = 0; k < nq; ++k) {
+        tag[k] = tags[k] == 1 ? 1 : (tags[k] ? 2 : 0);
+        flags[k] = tags[k] == 1;
+    }
     amr_forest_refine(f, flags);
     free(flags);
     if (allow_coarsening) amr_forest_coarsen(f);
@@ -327,19 +331,14 @@ static long long steps_to(int istep, int interval)
     return (long long)interval - (istep % interval);
 }
 
-int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_stats_t *st)
+int amr_device_setup(amr_forest_t *f, amrgpu_ctx **out)
 {
     physics_gpu_params_t p;
     physics_gpu_get_params(&p);
-    static const int stages[6] = {0, 1, 2, 3, 4, 5};
-    amr_evolve_stats_t s;
-    memset(&s, 0, sizeof(s));
-    s.peak_quads = amr_forest_num_quads(f);
-    const int n = amr_forest_nelem_subgrid(f);
-    int ncell, nghost, nface, dof, rc = 0;
+    int ncell, nghost, nface, dof;
     int *cell_info, *face_info;
     double *geom, *soln;
-    double t0 = now_s();
+    *out = NULL;
     if (amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom, &soln))
         return AMRGPU_ERR_MESH;
     amrgpu_ctx *g = amrgpu_create();
@@ -350,8 +349,44 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
     ip.cfl = p.cfl;
     ip.dt_cap = p.dt;
     ip.t_end = p.t_end;
-    rc = amrgpu_setup(g, n, p.time_scheme, 1, &ip, ncell, cell_info, NULL, nface, face_info, geom, soln);
-    if (!rc && o && o->tile_ppb > 0) rc = amrgpu_set_tile(g, o->tile_ppb, o->tile_minw, o->tile_design);
+    int rc = amrgpu_setup(g, amr_forest_nelem_subgrid(f), p.time_scheme, 1, &ip, ncell, cell_info, NULL, nface,
+                          face_info, geom, soln);
+    if (rc) {
+        amrgpu_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return AMRGPU_OK;
+}
+
+int amr_device_regrid(amr_forest_t *f, amrgpu_ctx *g, const int *tags, int allow_coarsening)
+{
+    int ncell, nghost, nface, dof;
+    int *cell_info, *face_info;
+    double *geom, *soln;
+    amr_forest_track_begin(f, 0);
+    amr_regrid_tags(f, tags, allow_coarsening);
+    if (!amr_forest_track_end(f)) return AMRGPU_ERR_MESH;
+    if (amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom, &soln))
+        return AMRGPU_ERR_MESH;
+    return amrgpu_remesh(g, ncell, cell_info, NULL, nface, face_info, amr_forest_track_src(f));
+}
+
+int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_stats_t *st)
+{
+    physics_gpu_params_t p;
+    physics_gpu_get_params(&p);
+    static const int stages[6] = {0, 1, 2, 3, 4, 5};
+    amr_evolve_stats_t s;
+    memset(&s, 0, sizeof(s));
+    s.peak_quads = amr_forest_num_quads(f);
+    const int n = amr_forest_nelem_subgrid(f);
+    int ncell = amr_forest_num_quads(f), rc = 0;
+    double t0 = now_s();
+    amrgpu_ctx *g = NULL;
+    rc = amr_device_setup(f, &g);
+    if (rc) return rc;
+    if (o && o->tile_ppb > 0) rc = amrgpu_set_tile(g, o->tile_ppb, o->tile_minw, o->tile_design);
     double *ind = NULL;
     int *tags = NULL;
     int cap = 0, istep = 0;
@@ -391,15 +426,8 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
             for (int k = 0; k < nq; ++k) tags[k] = ind[k] > (o ? o->tol : 0.1);
         }
         if (rc) break;
-        amr_forest_track_begin(f, 0);
-        amr_regrid_tags(f, tags, o ? o->allow_coarsening : 1);
-        if (!amr_forest_track_end(f)) { rc = AMRGPU_ERR_MESH; break; }
-        if (amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom,
-                                      &soln)) {
-            rc = AMRGPU_ERR_MESH;
-            break;
-        }
-        rc = amrgpu_remesh(g, ncell, cell_info, NULL, nface, face_info, amr_forest_track_src(f));
+        rc = amr_device_regrid(f, g, tags, o ? o->allow_coarsening : 1);
+        ncell = amr_forest_num_quads(f);
         s.regrids++;
         if (ncell > s.peak_quads) s.peak_quads = ncell;
         s.wall_regrid += now_s() - t2;

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "amr_hip.h"
+#include "amr_internal.h"
 
 namespace {
 
@@ -944,6 +945,14 @@ struct StageLaunch {
 
 }  // namespace
 
+extern "C" void amr_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
 struct amrgpu_ctx {
     int n = 0, nquad = 0, ninst = 0, scheme = 3;
     hipStream_t stream = nullptr;
@@ -1267,6 +1276,18 @@ void amrgpu_destroy(amrgpu_ctx *c)
 
 int amrgpu_num_cells(const amrgpu_ctx *c) { return c ? c->nquad * c->n * c->n : 0; }
 int amrgpu_num_quads(const amrgpu_ctx *c) { return c ? c->nquad : 0; }
+
+int amrgpu_device_state(amrgpu_ctx *c, const double **U, const int **nbr, int *nquad, int *n,
+                        void **stream)
+{
+    if (!c || !c->nquad) { set_err("amrgpu_device_state: no mesh"); return AMRGPU_ERR_ARG; }
+    if (U) *U = c->U[c->cur];
+    if (nbr) *nbr = c->d_nbr;
+    if (nquad) *nquad = c->nquad;
+    if (n) *n = c->n;
+    if (stream) *stream = (void *)c->stream;
+    return AMRGPU_OK;
+}
 
 static int validate_cells(int n2, int ninst, int ncell, const int *cell_info, const int *cell_inst,
                           const char *who)

@@ -1,0 +1,30 @@
+/*
+ * amr_internal.h -- helpers shared by the library's translation units; not
+ * part of the C ABI (hidden visibility).
+ */
+#ifndef AMR_INTERNAL_H
+#define AMR_INTERNAL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMR_HIDDEN __attribute__((visibility("hidden")))
+
+struct amrgpu_ctx;
+struct amr_forest;
+
+/* Formats the message returned by amrgpu_last_error(). */
+AMR_HIDDEN void amr_set_error(const char *fmt, ...);
+
+/* Device context for the whole forest with the current physics settings
+ * (one instance; h_base = coarse_grid_h, walls bc[0..3]). */
+AMR_HIDDEN int amr_device_setup(struct amr_forest *f, struct amrgpu_ctx **out);
+/* Regrid the forest by tags (amr_regrid_tags) and remap the device state. */
+AMR_HIDDEN int amr_device_regrid(struct amr_forest *f, struct amrgpu_ctx *g, const int *tags,
+                                 int allow_coarsening);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

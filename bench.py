@@ -21,8 +21,15 @@ steps / wall time of the timed region (graph replay, HIP events).
 
 roofline: the fused stage kernel; algorithmic bytes = 85.33 B per cell-stage
 (stage 1 reads U^n, writes U^1; stages 2-3 read U^s and U^n, write U^s+1;
-32 B per state), achieved = bytes per launch / average launch duration
-(HIP events around each launch, same stream).
+32 B per state), achieved = bytes per launch / average launch duration, the
+latter = the timed region (HIP events on the launch stream) / launches.
+traffic = FETCH_SIZE + WRITE_SIZE per launch from the committed rocprofv3
+summary of the same mesh (profiles/stage_kernel_<workload>_traffic.txt).
+
+learned: the learned-tagger comparison (amrhip.learned): snapshots from
+seeded runs, BF16-MFMA training on the GPU, held-out recall, and plain vs
+learned deployment on a finer seeded run with L1 density errors against the
+uniform finest-level plain solution (--no-learned skips it).
 
 cpu_baseline: the CPU oracle (C restatement of the reference CPU path,
 AoS patches, face loop + element loop, k-form SSP-RK3) on the host cores,
@@ -113,6 +120,53 @@ def measured_traffic(workload, cells, tile):
         return None
 
 
+def learned_experiment(amrhip, args):
+    """plain indicator every 10 steps vs the learned tagger (with the plain
+    indicator as fallback) every --learned-interval steps, on a seeded
+    Riemann run with levels lmin..--learned-lmax: end-to-end cell-updates/s
+    and L1 density error against the plain-method FP64 solution on the
+    uniform finest level.  The network is trained on the GPU from snapshots
+    of other seeds (1..k); recall is measured on two held-out seeds."""
+    from amrhip import learned as L
+    lmax, iv = args.learned_lmax, args.learned_interval
+    t0 = time.perf_counter()
+    ts = L.collect(range(1, 1 + args.learned_seeds), lmin=args.lmin, lmax=args.lmax, t_end=args.t_end,
+                   tol=args.tol, window=iv, log=log)
+    held = L.collect([100, 101], lmin=args.lmin, lmax=args.lmax, t_end=args.t_end, tol=args.tol, window=iv)
+    t_collect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m, losses = L.train(ts, epochs=args.learned_epochs)
+    t_train = time.perf_counter() - t0
+    rec = L.recall_report(m, held, args.tol)
+    log(f"tagger: {ts.size} training rows ({t_collect:.1f} s to collect), trained in {t_train:.2f} s, "
+        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}; held-out recall mlp {rec['recall_mlp']:.3f} "
+        f"vs indicator-now {rec['recall_indicator_now']:.3f}")
+    ref, _ = L.uniform_reference(args.seed, lmax, args.t_end, log=log)
+    runs = {}
+    for name, interval, tagger in (("plain_every_10", 10, None), (f"plain_every_{iv}", iv, None),
+                                   (f"learned_every_{iv}", iv, amrhip.MlpTagger(m, args.tol, True))):
+        r, _ = L.deploy(args.seed, args.lmin, lmax, args.t_end, interval, tagger, args.tol, ref, lmax)
+        runs[name] = r
+        log(f"{name}: {r['steps']} steps, {r['regrids']} regrids, peak {r['peak_cells']} cells, "
+            f"{r['wall_s']:.2f} s, {r['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s end to end, "
+            f"L1 {r['l1_density']:.5e}")
+    base = runs["plain_every_10"]
+    lr = runs[f"learned_every_{iv}"]
+    m.close()
+    return {
+        "deployed_workload": f"riemann4 seed {args.seed}, levels {args.lmin}-{lmax}, 8x8 FP64, t={args.t_end}",
+        "reference": f"plain FP64 on the uniform level-{lmax} forest ({ref.shape[0]}^2 cells)",
+        "training": {"rows": ts.size, "label_counts": L.class_counts(ts.get(features=False)[1]),
+                     "seeds": args.learned_seeds, "window": iv, "label_stride": 5,
+                     "epochs": args.learned_epochs, "loss_first_last": [float(losses[0]), float(losses[-1])],
+                     "collect_s": round(t_collect, 2), "train_s": round(t_train, 3)},
+        "held_out_recall": rec,
+        "runs": runs,
+        "speedup_end_to_end": lr["end_to_end_Mcell_upd_s"] / base["end_to_end_Mcell_upd_s"],
+        "l1_ratio": lr["l1_density"] / base["l1_density"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +188,11 @@ def main():
     ap.add_argument("--level", type=int, default=4)
     ap.add_argument("--host-loop", action="store_true",
                     help="also time the interval-by-interval host-regrid loop")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-tagger comparison")
+    ap.add_argument("--learned-seeds", type=int, default=8, help="training runs (seeds 1..k)")
+    ap.add_argument("--learned-lmax", type=int, default=8, help="finest level of the deployed run")
+    ap.add_argument("--learned-interval", type=int, default=40)
+    ap.add_argument("--learned-epochs", type=int, default=12)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (the whole forest lives on one "
@@ -189,7 +248,9 @@ def main():
     ms, stage_ms = g.bench(args.steps)
     ms_per_step = ms / args.steps
     value = cells * 3 * args.steps / (ms * 1e-3) / 1e6
-    avg_stage = float(np.mean(stage_ms))
+    # average launch duration over the timed region (HIP events on the
+    # context's stream around the graph replays): region time / launches
+    avg_stage = ms / (3 * args.steps)
     bytes_launch = cells * B_ALG_STAGE
     achieved = bytes_launch / (avg_stage * 1e-3) / 1e9
     log(f"timed: {args.steps} steps, {ms:.2f} ms ({ms_per_step * 1e3:.1f} us/step); stage kernel "
@@ -232,7 +293,8 @@ def main():
             "kernel": f"stage_kernel<{args.n},{tile[0]},{tile[1]},{tile[2]}>",
             "bytes_per_launch": int(bytes_launch),
             "avg_launch_us": round(avg_stage * 1e3, 3),
-            "stage_launch_us": [round(s * 1e3, 3) for s in stage_ms],
+            "avg_launch_from": "timed region / launches (HIP events, graph replay)",
+            "stage_launch_us_bracketed": [round(s * 1e3, 3) for s in stage_ms],
             "copy_probe_GBs": round(probe, 1) if probe else None,
         },
     }
@@ -266,6 +328,8 @@ def main():
                           f"C oracle {label}, {threads} threads on contiguous quadrant ranges",
             }
             out["gpu_over_cpu"] = round(value / v, 1)
+    if args.workload == "riemann4" and args.n == 8 and not args.no_learned:
+        out["learned"] = learned_experiment(amrhip, args)
     print(json.dumps(out), flush=True)
 
 

@@ -58,7 +58,9 @@ void amr_indicator_gradient(amr_forest_t *f, double *ind);
  * (below max_level), coarsen untagged families (unless initial), 2:1 balance.
  * Returns the new number of quadrants. */
 int amr_regrid(amr_forest_t *f, double tol, int allow_coarsening);
-/* The same regrid driven by given per-quadrant tags (non-zero: refine). */
+/* The same regrid driven by given per-quadrant tags: 1 refine, 0 may be
+ * coarsened with its family, any other value keeps the quadrant (no refine,
+ * its family is not coarsened). */
 int amr_regrid_tags(amr_forest_t *f, const int *tags, int allow_coarsening);
 /* Initial refinement (regrid_feature with initial = 1): repeatedly tag and
  * refine, re-evaluating the initial condition on new quadrants
@@ -85,8 +87,8 @@ typedef struct {
 int amr_evolve(amr_forest_t *f, double tol, int allow_coarsening, amr_evolve_stats_t *stats);
 
 struct amrgpu_ctx;
-/* Tag source for amr_evolve_device: fills tags[nquad] (non-zero: refine)
- * for the device state of ctx; returns 0 or an error code. */
+/* Tag source for amr_evolve_device: fills tags[nquad] (amr_regrid_tags
+ * values) for the device state of ctx; returns 0 or an error code. */
 typedef int (*amr_tagger_fn)(void *user, struct amrgpu_ctx *ctx, amr_forest_t *f, int *tags);
 
 typedef struct {

@@ -170,6 +170,13 @@ int amrgpu_get_tile(amrgpu_ctx *c, int *ppb, int *minw, int *design);
 int amrgpu_autotune(amrgpu_ctx *c, int nsteps, int *nvar, int *ppb, int *minw, int *design,
                     double *ms_per_step);
 int amrgpu_num_quads(const amrgpu_ctx *c);
+/* Read-only view of the device-resident mesh and state for kernels outside
+ * the library's stage path (the learned tagger): state U[quad][field][j][i]
+ * (FP64, n x n patches), neighbour table nbr[quad][side][2] (side -x,+x,-y,+y;
+ * a < 0 boundary, b < 0 one index-aligned neighbour, else two finer ones in
+ * increasing coordinate), and the stream that orders work on them. */
+int amrgpu_device_state(amrgpu_ctx *c, const double **U, const int **nbr, int *nquad, int *n,
+                        void **stream);
 
 /* Device-side helpers for the HBM roofline probe: copy nbytes between two
  * device buffers allocated by the library, timed with events. */

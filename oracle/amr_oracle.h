@@ -138,6 +138,29 @@ void orc_coarsen_2d(int n, const double *c1, const double *c2, const double *c3,
 /* src/solver/src/tagging_methods_module.F90:14-71 via solver_tag_feature.F90:9-52 */
 int orc_tag_feature(int ntags, const int *methods, const double *tol, int n, const double *Q);
 
+/* ---- learned tagger (no reference counterpart; tagger_oracle.c) --------- */
+/* The reference has no learned tagger.  These restate the library's own
+ * definitions (include/amr_tagger.h) on the CPU so the GPU kernels have a
+ * checker; the MLP backward pass is pinned by finite differences and by an
+ * independent autograd implementation in the tests. */
+
+/* FP32 -> BF16 bits, round to nearest even. */
+unsigned short orc_bf16(float x);
+float orc_bf16_to_float(unsigned short h);
+/* Features of every quadrant of a mesh given in the reference's arrays
+ * (face_info as hpc_amr_evolve_solver receives it, soln Q(4,8,8) per
+ * quadrant); X[ncell][384] BF16 bits. */
+int orc_tagger_features(int n, int ncell, int nface, const int *face_info, const double *soln,
+                        unsigned short *X);
+/* Logits [n][3] of the BF16 network (exact = 0) or of the same network with
+ * no rounding of weights and activations (exact = 1). */
+void orc_mlp_forward(const float *P, const unsigned short *X, int n, int exact, double *logits);
+/* Class-weighted mean cross-entropy and its gradient (AMR_MLP_NPARAM
+ * doubles).  exact = 0 rounds as the GPU pass does; exact = 1 is the smooth
+ * function of the FP32 parameters. */
+double orc_mlp_grad(const float *P, const unsigned short *X, const int *y, int n, const float cw[3],
+                    int exact, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,9 +55,14 @@ class Oracle:
             "orc_coarsen_2d": [_i, _pd, _pd, _pd, _pd, _pd],
             "orc_tag_feature": [_i, _pi, _pd, _i, _pd],
             "orc_rk_generic": [_i, _i, _d, _pd, _pd, RHS_FN, _vp],
+            "orc_bf16": [C.c_float],
+            "orc_tagger_features": [_i, _i, _i, _pi, _pd, _vp],
+            "orc_mlp_forward": [_vp, _vp, _i, _i, _pd],
+            "orc_mlp_grad": [_vp, _vp, _pi, _i, _vp, _i, _pd],
         }
         res = {"orc_run_steps": _d, "orc_run_steps_threaded": _d, "orc_rk_step": _d,
-               "orc_tag_feature": _i, "orc_rk_generic": _d}
+               "orc_tag_feature": _i, "orc_rk_generic": _d, "orc_bf16": C.c_ushort,
+               "orc_tagger_features": _i, "orc_mlp_grad": _d}
         for k, a in sig.items():
             f = getattr(L, k)
             f.argtypes = a
@@ -141,6 +146,35 @@ class Oracle:
         self.L.orc_initial_condition_2d(kind, n, dpt(np.ascontiguousarray(geom, float)), dpt(q), mach,
                                         alpha, beta, gamma, density, pressure, beta_l, xo, yo)
         return q
+
+    # ---- learned tagger (no reference counterpart) ----
+    def tagger_features(self, n, cell_info, face_info, soln):
+        ci = np.ascontiguousarray(cell_info, np.int32)
+        fi = np.ascontiguousarray(face_info, np.int32)
+        s = np.ascontiguousarray(soln, float)
+        ncell = ci.size // 3
+        X = np.zeros((ncell, 384), np.uint16)
+        rc = self.L.orc_tagger_features(n, ncell, fi.size // 11, ipt(fi), dpt(s), X.ctypes.data)
+        if rc:
+            raise ValueError("orc_tagger_features failed")
+        return X
+
+    def mlp_forward(self, P, X, exact=False):
+        P = np.ascontiguousarray(P, np.float32)
+        X = np.ascontiguousarray(X, np.uint16)
+        out = np.zeros((X.shape[0], 3))
+        self.L.orc_mlp_forward(P.ctypes.data, X.ctypes.data, X.shape[0], int(exact), dpt(out))
+        return out
+
+    def mlp_grad(self, P, X, y, cw, exact=False):
+        P = np.ascontiguousarray(P, np.float32)
+        X = np.ascontiguousarray(X, np.uint16)
+        y = np.ascontiguousarray(y, np.int32)
+        cw = np.ascontiguousarray(cw, np.float32)
+        g = np.zeros(P.size)
+        loss = self.L.orc_mlp_grad(P.ctypes.data, X.ctypes.data, ipt(y), X.shape[0], cw.ctypes.data,
+                                   int(exact), dpt(g))
+        return loss, g
 
 
 # ---------------------------------------------------------------------------
