@@ -77,6 +77,7 @@ EXPORTS = {
     "amr_regrid_tags": (_i, [_vp, _pi, _i]),
     "amr_evolve": (_i, [_vp, _d, _i, _vp]),
     "amr_evolve_device": (_i, [_vp, _vp, _vp]),
+    "amr_evolve_batch": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "amr_forest_track_begin": (None, [_vp, _i]),
     "amr_forest_track_src": (_pi, [_vp]),
     "amr_forest_track_end": (_i, [_vp]),
@@ -99,6 +100,7 @@ EXPORTS = {
     "amrgpu_tagger_features": (_i, [_vp, _vp]),
     "amrgpu_tagger_features_host": (_i, [_vp, _vp]),
     "amr_mlp_train": (_i, [_vp, _vp, _i, _i, C.c_float, _vp, C.c_ulonglong, _vp]),
+    "amr_mlp_set_refine_bias": (_i, [_vp, C.c_float]),
     "amr_mlp_predict_set": (_i, [_vp, _vp, _pi]),
     "amr_mlp_predict_ctx": (_i, [_vp, _vp, _pi]),
     "amr_tagger_collect": (_i, [_vp, _vp, _vp, _pi]),
@@ -347,6 +349,26 @@ class Forest:
         rows = C.c_int(0)
         check(self.L.amr_tagger_collect(self.h, C.byref(o), tagset.h, C.byref(rows)), "amr_tagger_collect")
         return rows.value
+
+
+def evolve_batch(forests, insts, tol, regrid_interval, allow_coarsening=True, tile=None, tagger=None):
+    """amr_evolve_batch: every forest advanced in one device context;
+    insts[k] = make_inst(...) of forest k.  Returns (per-instance stats, total)"""
+    L = lib()
+    k = len(forests)
+    fa = (_vp * k)(*[f.h for f in forests])
+    ia = (InstParams * k)(*insts)
+    o = EvolveOpts()
+    o.tol, o.allow_coarsening, o.regrid_interval = tol, int(allow_coarsening), regrid_interval
+    if isinstance(tagger, MlpTagger):
+        o.tagger = C.cast(L.amr_mlp_tagger, _vp)
+        o.tagger_ctx = C.cast(C.pointer(tagger.s), _vp)
+    if tile:
+        o.tile_ppb, o.tile_minw, o.tile_design = tile
+    per = (EvolveStats * k)()
+    tot = EvolveStats()
+    check(L.amr_evolve_batch(fa, ia, k, C.byref(o), per, C.byref(tot)), "amr_evolve_batch")
+    return list(per), tot
 
 
 def vortex_ic(strength=5.0, xo=5.0, yo=5.0, u=1.0, v=1.0, gamma=1.4, rho=1.0, p=1.0):
@@ -615,6 +637,9 @@ class Mlp:
         check(self.L.amr_mlp_train(self.h, tagset.h, epochs, batch, lr, cwp, seed, loss.ctypes.data),
               "amr_mlp_train")
         return loss[:epochs]
+
+    def set_refine_bias(self, bias):
+        check(self.L.amr_mlp_set_refine_bias(self.h, float(bias)), "amr_mlp_set_refine_bias")
 
     def predict(self, tagset):
         out = np.zeros(tagset.size, np.int32)

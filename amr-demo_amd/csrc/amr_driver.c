@@ -443,3 +443,216 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
     if (st) *st = s;
     return rc;
 }
+
+/* ---- many problems in one context -------------------------------------- */
+
+typedef struct {
+    int *ci, *fi, *cinst, *src;
+    double *geom, *soln;
+    size_t cap_c, cap_f;
+} batch_bufs_t;
+
+static void batch_free(batch_bufs_t *b)
+{
+    free(b->ci); free(b->fi); free(b->cinst); free(b->src); free(b->geom); free(b->soln);
+    memset(b, 0, sizeof(*b));
+}
+
+static int batch_grow(batch_bufs_t *b, size_t nc, size_t nf, int n2)
+{
+    if (nc > b->cap_c) {
+        size_t c = nc + nc / 2;
+        int *ci = (int *)realloc(b->ci, sizeof(int) * 3 * c);
+        if (ci) b->ci = ci;
+        int *cinst = (int *)realloc(b->cinst, sizeof(int) * c);
+        if (cinst) b->cinst = cinst;
+        int *src = (int *)realloc(b->src, sizeof(int) * c);
+        if (src) b->src = src;
+        double *geom = (double *)realloc(b->geom, sizeof(double) * 12 * c);
+        if (geom) b->geom = geom;
+        double *soln = (double *)realloc(b->soln, sizeof(double) * 4 * (size_t)n2 * c);
+        if (soln) b->soln = soln;
+        if (!ci || !cinst || !src || !geom || !soln) return -1;
+        b->cap_c = c;
+    }
+    if (nf > b->cap_f || !b->fi) {
+        size_t c = nf + nf / 2 + 1;
+        int *fi = (int *)realloc(b->fi, sizeof(int) * 11 * c);
+        if (!fi) return -1;
+        b->fi = fi;
+        b->cap_f = c;
+    }
+    return 0;
+}
+
+/* Concatenated boundary arrays of all forests (quadrants of instance k at
+ * base[k] .. base[k+1]-1; face cell ids shifted by the same offsets). */
+static int batch_build(amr_forest_t **f, int ninst, int n, int *base, batch_bufs_t *b, int *ncell_out,
+                       int *nface_out, int with_data)
+{
+    const int n2 = n * n;
+    size_t nc = 0, nf = 0;
+    for (int k = 0; k < ninst; ++k) {
+        int ncell, nghost, nface, dof;
+        int *cell_info, *face_info;
+        double *geom, *soln;
+        if (amr_forest_build_external(f[k], &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom, &soln))
+            return AMRGPU_ERR_MESH;
+        base[k] = (int)nc;
+        if (batch_grow(b, nc + (size_t)ncell, nf + (size_t)nface, n2)) return AMRGPU_ERR_ARG;
+        for (int q = 0; q < ncell; ++q) {
+            const size_t g = nc + (size_t)q;
+            b->ci[3 * g] = cell_info[3 * q];
+            b->ci[3 * g + 1] = cell_info[3 * q + 1];
+            b->ci[3 * g + 2] = 4 * n2 * (int)g;
+            b->cinst[g] = k;
+        }
+        memcpy(b->geom + 12 * nc, geom, sizeof(double) * 12 * (size_t)ncell);
+        if (with_data) memcpy(b->soln + 4 * (size_t)n2 * nc, soln, sizeof(double) * 4 * (size_t)n2 * ncell);
+        for (int e = 0; e < nface; ++e) {
+            const int *src = face_info + 11 * e;
+            int *dst = b->fi + 11 * (nf + (size_t)e);
+            for (int j = 0; j < 11; ++j) dst[j] = src[j];
+            dst[2] = src[2] + (int)nc;
+            for (int j = 3; j < 7; ++j) dst[j] = src[j] >= 0 ? src[j] + (int)nc : src[j];
+        }
+        nc += (size_t)ncell;
+        nf += (size_t)nface;
+    }
+    base[ninst] = (int)nc;
+    *ncell_out = (int)nc;
+    *nface_out = (int)nf;
+    return AMRGPU_OK;
+}
+
+int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, const amr_evolve_opts_t *o,
+                     amr_evolve_stats_t *per_inst, amr_evolve_stats_t *total)
+{
+    if (!f || !inst || ninst <= 0) return AMRGPU_ERR_ARG;
+    const int n = amr_forest_nelem_subgrid(f[0]);
+    for (int k = 1; k < ninst; ++k)
+        if (amr_forest_nelem_subgrid(f[k]) != n) return AMRGPU_ERR_ARG;
+    physics_gpu_params_t p;
+    physics_gpu_get_params(&p);
+    static const int stages[6] = {0, 1, 2, 3, 4, 5};
+    const int n2 = n * n, interval = o ? o->regrid_interval : p.regrid_interval;
+    int poll = 0;
+    for (int k = 0; k < ninst; ++k)
+        if (inst[k].t_end > 0.0) poll = 1;
+    amr_evolve_stats_t tot;
+    memset(&tot, 0, sizeof(tot));
+    amr_evolve_stats_t *st = (amr_evolve_stats_t *)calloc((size_t)ninst, sizeof(amr_evolve_stats_t));
+    int *base = (int *)malloc(sizeof(int) * (size_t)(ninst + 1));
+    int *done = (int *)calloc((size_t)ninst, sizeof(int));
+    double *t = (double *)calloc((size_t)ninst, sizeof(double));
+    batch_bufs_t b;
+    memset(&b, 0, sizeof(b));
+    double *ind = NULL;
+    int *tags = NULL;
+    int ncell = 0, nface = 0, rc = 0, istep = 0, cap = 0;
+    amrgpu_ctx *g = NULL;
+    double t0 = now_s();
+    if (!st || !base || !done || !t) rc = AMRGPU_ERR_ARG;
+    if (!rc) rc = batch_build(f, ninst, n, base, &b, &ncell, &nface, 1);
+    if (!rc) {
+        g = amrgpu_create();
+        if (!g) rc = AMRGPU_ERR_HIP;
+    }
+    if (!rc)
+        rc = amrgpu_setup(g, n, p.time_scheme, ninst, inst, ncell, b.ci, b.cinst, nface, b.fi, b.geom, b.soln);
+    if (!rc && o && o->tile_ppb > 0) rc = amrgpu_set_tile(g, o->tile_ppb, o->tile_minw, o->tile_design);
+    for (int k = 0; k < ninst && !rc; ++k) st[k].peak_quads = base[k + 1] - base[k];
+    tot.peak_quads = ncell;
+    tot.wall_regrid += now_s() - t0;
+    while (!rc) {
+        long long nsteps = (long long)p.time_steps - istep;
+        long long r = steps_to(istep, interval);
+        if (r < nsteps) nsteps = r;
+        if (nsteps <= 0) break;
+        if (poll && nsteps > 16) nsteps = 16;
+        double t1 = now_s();
+        double resnorm;
+        rc = amrgpu_run(g, (int)nsteps, 1, &resnorm);
+        if (rc) break;
+        for (int k = 0; k < ninst; ++k) {
+            if (done[k]) continue;
+            st[k].steps += (int)nsteps;
+            st[k].cell_stages += (double)nsteps * (base[k + 1] - base[k]) * n2 * stages[p.time_scheme];
+        }
+        istep += (int)nsteps;
+        tot.steps += (int)nsteps;
+        tot.cell_stages += (double)nsteps * ncell * n2 * stages[p.time_scheme];
+        rc = amrgpu_get_time(g, t, NULL);
+        tot.wall_gpu += now_s() - t1;
+        if (rc) break;
+        int all = 1;
+        for (int k = 0; k < ninst; ++k) {
+            if (inst[k].t_end > 0.0 && t[k] >= inst[k].t_end) done[k] = 1;
+            if (!done[k]) all = 0;
+        }
+        if (istep >= p.time_steps || all) break;
+        if (interval <= 0 || istep % interval != 0) continue;
+        /* regrid every unfinished instance on its own forest */
+        double t2 = now_s();
+        if (ncell > cap) {
+            cap = ncell + ncell / 2;
+            free(ind); free(tags);
+            ind = (double *)malloc(sizeof(double) * (size_t)cap);
+            tags = (int *)malloc(sizeof(int) * (size_t)cap);
+            if (!ind || !tags) { rc = AMRGPU_ERR_ARG; break; }
+        }
+        if (o && o->tagger) {
+            rc = o->tagger(o->tagger_ctx, g, NULL, tags);
+        } else {
+            rc = amrgpu_indicator(g, ind);
+            for (int q = 0; q < ncell; ++q) tags[q] = ind[q] > (o ? o->tol : 0.1);
+        }
+        if (rc) break;
+        /* old quadrant ranges, then the tracked sources shifted into them */
+        int *obase = (int *)malloc(sizeof(int) * (size_t)(ninst + 1));
+        if (!obase) { rc = AMRGPU_ERR_ARG; break; }
+        memcpy(obase, base, sizeof(int) * (size_t)(ninst + 1));
+        for (int k = 0; k < ninst && !rc; ++k) {
+            if (done[k]) continue;
+            amr_forest_track_begin(f[k], 0);
+            amr_regrid_tags(f[k], tags + obase[k], o ? o->allow_coarsening : 1);
+            if (!amr_forest_track_end(f[k])) rc = AMRGPU_ERR_MESH;
+            st[k].regrids++;
+        }
+        if (!rc) rc = batch_build(f, ninst, n, base, &b, &ncell, &nface, 0);
+        for (int k = 0; k < ninst && !rc; ++k) {
+            const int nk = base[k + 1] - base[k];
+            if (done[k]) {
+                for (int q = 0; q < nk; ++q) b.src[base[k] + q] = obase[k] + q;
+            } else {
+                const int *s = amr_forest_track_src(f[k]);
+                for (int q = 0; q < nk; ++q)
+                    b.src[base[k] + q] = s[q] >= 0 ? obase[k] + s[q] : -(obase[k] + (-s[q] - 1)) - 1;
+            }
+            if (nk > st[k].peak_quads) st[k].peak_quads = nk;
+        }
+        free(obase);
+        if (!rc) rc = amrgpu_remesh(g, ncell, b.ci, b.cinst, nface, b.fi, b.src);
+        tot.regrids++;
+        if (ncell > tot.peak_quads) tot.peak_quads = ncell;
+        tot.wall_regrid += now_s() - t2;
+    }
+    if (!rc) {
+        rc = amrgpu_get_time(g, t, NULL);
+        double *all = (double *)malloc(sizeof(double) * 4 * (size_t)n2 * (size_t)(ncell > 0 ? ncell : 1));
+        if (!all) rc = AMRGPU_ERR_ARG;
+        if (!rc) rc = amrgpu_download_soln(g, all);
+        for (int k = 0; k < ninst && !rc; ++k) {
+            memcpy(amr_forest_soln(f[k]), all + 4 * (size_t)n2 * base[k],
+                   sizeof(double) * 4 * (size_t)n2 * (size_t)(base[k + 1] - base[k]));
+            st[k].t = t[k];
+        }
+        free(all);
+    }
+    if (g) amrgpu_destroy(g);
+    if (per_inst && st) memcpy(per_inst, st, sizeof(amr_evolve_stats_t) * (size_t)ninst);
+    if (total) *total = tot;
+    batch_free(&b);
+    free(st); free(base); free(done); free(t); free(ind); free(tags);
+    return rc;
+}

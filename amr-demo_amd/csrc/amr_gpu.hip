@@ -737,6 +737,13 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             for (int f = 0; f < 4; ++f) out[f] = k4[f];
             break;
         }
+        // an instance past its physical stop (dt = 0) keeps its state bit for
+        // bit: every stage state equals U^n then, while a convex combination
+        // such as 0.75 U + 0.25 U can round away from U
+        if (a.mode != MODE_RHS && dt == 0.0) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) out[f] = U[f];
+        }
 #pragma unroll
         for (int f = 0; f < 4; ++f) a.Uout[base + f * N2] = out[f];
         if (a.role & ROLE_CFL) cand = cfl_candidate(out, a.m.ip[inst].cfl, vol);

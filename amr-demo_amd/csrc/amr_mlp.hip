@@ -107,10 +107,12 @@ __global__ __launch_bounds__(256) void features_kernel(const double *__restrict_
         feat[f] = s > 0.0 ? u[f] / s : 0.0;
     }
     const double r = u[0], s0 = bits2d(smax[p][0]);
-    const double rx = ci < 7 ? Q[c + 1] : edge_value(U, nbr, quad, 1, cj * 8, r);
-    const double ry = cj < 7 ? Q[c + 8] : edge_value(U, nbr, quad, 3, ci, r);
-    feat[4] = s0 > 0.0 ? (rx - r) / s0 : 0.0;
-    feat[5] = s0 > 0.0 ? (ry - r) / s0 : 0.0;
+    const double re = ci < 7 ? Q[c + 1] : edge_value(U, nbr, quad, 1, cj * 8, r);
+    const double rw = ci > 0 ? Q[c - 1] : edge_value(U, nbr, quad, 0, cj * 8 + 7, r);
+    const double rn = cj < 7 ? Q[c + 8] : edge_value(U, nbr, quad, 3, ci, r);
+    const double rs = cj > 0 ? Q[c - 8] : edge_value(U, nbr, quad, 2, 56 + ci, r);
+    feat[4] = s0 > 0.0 ? (re - rw) / s0 : 0.0;
+    feat[5] = s0 > 0.0 ? (rn - rs) / s0 : 0.0;
     uint16_t *o = X + (size_t)quad * NIN + c;
 #pragma unroll
     for (int f = 0; f < 6; ++f) o[f * 64] = f2bf((float)feat[f]);
@@ -251,7 +253,8 @@ __device__ void forward_rows(NetLds &s)
 
 // Inference: logits (optional) and argmax class (optional) of `count` rows.
 __global__ __launch_bounds__(BLK) void infer_kernel(Weights wt, const uint16_t *__restrict__ X, int count,
-                                                    float *__restrict__ logits, int *__restrict__ cls)
+                                                    float refine_bias, float *__restrict__ logits,
+                                                    int *__restrict__ cls)
 {
     __shared__ NetLds s;
     stage_weights(s, wt);
@@ -262,11 +265,11 @@ __global__ __launch_bounds__(BLK) void infer_kernel(Weights wt, const uint16_t *
         const int t = threadIdx.x;
         const int row = chunk * ROWS + t;
         if (t < ROWS && row < count) {
-            const float z0 = s.Z[t * 4], z1 = s.Z[t * 4 + 1], z2 = s.Z[t * 4 + 2];
+            const float z0 = s.Z[t * 4], z1 = s.Z[t * 4 + 1], z2 = s.Z[t * 4 + 2] + refine_bias;
             if (logits) {
                 logits[(size_t)row * 3] = z0;
                 logits[(size_t)row * 3 + 1] = z1;
-                logits[(size_t)row * 3 + 2] = z2;
+                logits[(size_t)row * 3 + 2] = s.Z[t * 4 + 2];
             }
             if (cls) cls[row] = (z0 >= z1 && z0 >= z2) ? 0 : (z1 >= z2 ? 1 : 2);
         }
@@ -555,6 +558,7 @@ struct amr_mlp {
     float *logits = nullptr;
     size_t cap_logits = 0;
     long long step = 0;
+    float refine_bias = 0.0f;
 };
 
 struct amr_tagset {
@@ -575,7 +579,8 @@ int launch_infer(amr_mlp *m, const uint16_t *dX, int count, float *dlogits, int 
     if (count <= 0) return AMRGPU_OK;
     int chunks = (count + ROWS - 1) / ROWS;
     int grid = std::min(chunks, 1024);
-    hipLaunchKernelGGL(infer_kernel, dim3(grid), dim3(BLK), 0, st, weights_of(m), dX, count, dlogits, dcls);
+    hipLaunchKernelGGL(infer_kernel, dim3(grid), dim3(BLK), 0, st, weights_of(m), dX, count, m->refine_bias,
+                       dlogits, dcls);
     MCHK(hipGetLastError());
     return AMRGPU_OK;
 }
@@ -678,6 +683,13 @@ int amr_mlp_set_params(amr_mlp_t *m, const float *params)
     MCHK(hipGetLastError());
     MCHK(hipStreamSynchronize(m->stream));
     m->step = 0;
+    return AMRGPU_OK;
+}
+
+int amr_mlp_set_refine_bias(amr_mlp_t *m, float bias)
+{
+    if (!m || !std::isfinite(bias)) { amr_set_error("amr_mlp_set_refine_bias: bad argument"); return AMRGPU_ERR_ARG; }
+    m->refine_bias = bias;
     return AMRGPU_OK;
 }
 

@@ -121,11 +121,14 @@ int amr_tagger_collect(amr_forest_t *f, const amr_collect_opts_t *o, amr_tagset_
 int amr_mlp_tagger(void *user, amrgpu_ctx *ctx, amr_forest_t *f, int *tags)
 {
     amr_mlp_tagger_t *u = (amr_mlp_tagger_t *)user;
-    if (!u || !u->mlp || !ctx || !f || !tags) {
+    if (!u || !u->mlp || !ctx || !tags) {
         amr_set_error("amr_mlp_tagger: bad argument");
         return AMRGPU_ERR_ARG;
     }
-    const int nq = amr_forest_num_quads(f);
+    /* every quadrant of the context (f is NULL when the context holds many
+     * forests, amr_evolve_batch) */
+    const int nq = amrgpu_num_quads(ctx);
+    (void)f;
     int *cls = (int *)malloc(sizeof(int) * (size_t)(nq > 0 ? nq : 1));
     double *ind = u->union_plain ? (double *)malloc(sizeof(double) * (size_t)(nq > 0 ? nq : 1)) : NULL;
     if (!cls || (u->union_plain && !ind)) {

@@ -18,6 +18,7 @@
 #define AMR_DRIVER_H
 
 #include "amr_forest.h"
+#include "amr_hip.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -87,8 +88,9 @@ typedef struct {
 int amr_evolve(amr_forest_t *f, double tol, int allow_coarsening, amr_evolve_stats_t *stats);
 
 struct amrgpu_ctx;
-/* Tag source for amr_evolve_device: fills tags[nquad] (amr_regrid_tags
- * values) for the device state of ctx; returns 0 or an error code. */
+/* Tag source for amr_evolve_device / amr_evolve_batch: fills tags[q] for
+ * every quadrant q of ctx (amr_regrid_tags values) from its device state;
+ * f is the forest (NULL for a batch context); returns 0 or an error code. */
 typedef int (*amr_tagger_fn)(void *user, struct amrgpu_ctx *ctx, amr_forest_t *f, int *tags);
 
 typedef struct {
@@ -106,6 +108,23 @@ typedef struct {
  * stepping settings come from the physics parameters; the final state is
  * downloaded into the forest. */
 int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *opts, amr_evolve_stats_t *stats);
+
+/* Many independent problems in one device context: the quadrants of every
+ * forest are concatenated (instance k owns a contiguous range, cell_inst = k)
+ * and each RK stage is one launch over all of them, with per-instance CFL
+ * minimum, dt, physical time and stop (inst[k]).  Every `regrid_interval`
+ * steps each instance that has not reached its stop is regridded on its own
+ * forest by the device indicator (tracking mode) and the combined state is
+ * remapped on the device.  An instance past its stop keeps its state (dt =
+ * 0), so each instance evolves exactly as it would alone under
+ * amr_evolve_device with the same settings.  All forests must have the same
+ * patch size; time scheme and step limit come from the physics parameters.
+ * per_inst[ninst] (may be NULL) receives each instance's statistics
+ * (cell_stages counted while it has not reached its stop at the start of a
+ * poll of at most 16 steps); *total the whole run's (cell_stages of every
+ * launched cell).  Final states are downloaded into the forests. */
+int amr_evolve_batch(amr_forest_t **forests, const amrgpu_inst_t *inst, int ninst,
+                     const amr_evolve_opts_t *opts, amr_evolve_stats_t *per_inst, amr_evolve_stats_t *total);
 
 #ifdef __cplusplus
 }

@@ -18,9 +18,9 @@
  *
  * Features (384 per quadrant, BF16, channel-major c*64 + j*8 + i):
  *   c = 0..3  U_c(i,j) / max over the patch of |U_c|       (rho, rho u, rho v, rho E)
- *   c = 4     (rho(i+1,j) - rho(i,j)) / max rho           (+x difference)
- *   c = 5     (rho(i,j+1) - rho(i,j)) / max rho           (+y difference)
- * Past the patch edge the neighbour value is the face neighbour's
+ *   c = 4     (rho(i+1,j) - rho(i-1,j)) / max rho         (x difference)
+ *   c = 5     (rho(i,j+1) - rho(i,j-1)) / max rho         (y difference)
+ * Past a patch edge the neighbour value is the face neighbour's
  * index-aligned edge cell (the mean of the two when the side has two finer
  * neighbours), or the cell itself on a domain boundary.  Each feature is
  * computed in FP64 (IEEE divide), rounded to FP32, then to BF16 (nearest even).
@@ -93,7 +93,10 @@ int amrgpu_tagger_features_host(struct amrgpu_ctx *c, uint16_t *X);
  * minibatch loss of each pass (may be NULL). */
 int amr_mlp_train(amr_mlp_t *m, const amr_tagset_t *s, int epochs, int batch, float lr,
                   const float *cw, unsigned long long seed, float *loss);
-/* Predicted class of every row of the set (argmax of the logits). */
+/* Decision rule of the predictions below: class = argmax(z0, z1, z2 + bias)
+ * (bias > 0 trades precision for recall of "refine"; default 0). */
+int amr_mlp_set_refine_bias(amr_mlp_t *m, float bias);
+/* Predicted class of every row of the set. */
 int amr_mlp_predict_set(amr_mlp_t *m, const amr_tagset_t *s, int *cls);
 /* Predicted class of every quadrant of the context's current state. */
 int amr_mlp_predict_ctx(amr_mlp_t *m, struct amrgpu_ctx *c, int *cls);

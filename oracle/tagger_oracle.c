@@ -48,6 +48,17 @@ static double round_bf(double v) { return (double)orc_bf16_to_float(orc_bf16((fl
 /* ------------------------------------------------------------------------- */
 /* features                                                                  */
 /* ------------------------------------------------------------------------- */
+/* density of the neighbour's facing edge cell ec across side s of quadrant q:
+ * one neighbour, the mean of two finer ones, or the own value r at a wall */
+static double edge_rho(const int *nb, const double *soln, int q, int s, int ec, double r)
+{
+    const int a = nb[8 * q + 2 * s], b = nb[8 * q + 2 * s + 1];
+    if (a < 0) return r;
+    const double va = soln[(size_t)256 * a + 4 * ec];
+    if (b < 0) return va;
+    return 0.5 * (va + soln[(size_t)256 * b + 4 * ec]);
+}
+
 int orc_tagger_features(int n, int ncell, int nface, const int *face_info, const double *soln,
                         unsigned short *X)
 {
@@ -86,29 +97,12 @@ int orc_tagger_features(int n, int ncell, int nface, const int *face_info, const
                 double feat[6];
                 for (int f = 0; f < 4; ++f) feat[f] = smax[f] > 0.0 ? Q[f + 4 * c] / smax[f] : 0.0;
                 const double r = Q[4 * c];
-                double rx, ry;
-                if (i < 7) {
-                    rx = Q[4 * (c + 1)];
-                } else {
-                    /* +x side: the neighbour's -x edge cell in row j */
-                    const int a = nb[8 * q + 2], b = nb[8 * q + 3];
-                    const int ec = 8 * j;
-                    if (a < 0) rx = r;
-                    else if (b < 0) rx = soln[(size_t)4 * n2 * a + 4 * ec];
-                    else rx = 0.5 * (soln[(size_t)4 * n2 * a + 4 * ec] + soln[(size_t)4 * n2 * b + 4 * ec]);
-                }
-                if (j < 7) {
-                    ry = Q[4 * (c + 8)];
-                } else {
-                    /* +y side: the neighbour's -y edge cell in column i */
-                    const int a = nb[8 * q + 6], b = nb[8 * q + 7];
-                    const int ec = i;
-                    if (a < 0) ry = r;
-                    else if (b < 0) ry = soln[(size_t)4 * n2 * a + 4 * ec];
-                    else ry = 0.5 * (soln[(size_t)4 * n2 * a + 4 * ec] + soln[(size_t)4 * n2 * b + 4 * ec]);
-                }
-                feat[4] = smax[0] > 0.0 ? (rx - r) / smax[0] : 0.0;
-                feat[5] = smax[0] > 0.0 ? (ry - r) / smax[0] : 0.0;
+                const double re = i < 7 ? Q[4 * (c + 1)] : edge_rho(nb, soln, q, 1, 8 * j, r);
+                const double rw = i > 0 ? Q[4 * (c - 1)] : edge_rho(nb, soln, q, 0, 8 * j + 7, r);
+                const double rn = j < 7 ? Q[4 * (c + 8)] : edge_rho(nb, soln, q, 3, i, r);
+                const double rs = j > 0 ? Q[4 * (c - 8)] : edge_rho(nb, soln, q, 2, 56 + i, r);
+                feat[4] = smax[0] > 0.0 ? (re - rw) / smax[0] : 0.0;
+                feat[5] = smax[0] > 0.0 ? (rn - rs) / smax[0] : 0.0;
                 for (int f = 0; f < 6; ++f) X[(size_t)NIN * q + 64 * f + c] = orc_bf16((float)feat[f]);
             }
     }

@@ -1,0 +1,69 @@
+"""Many independent adaptive problems in one device context
+(amr_evolve_batch): each instance must evolve exactly as it does alone under
+amr_evolve_device with the same settings -- same steps, regrids, final mesh
+and state bit for bit -- although the instances have different sizes, time
+steps, regrid outcomes and finish at different step counts."""
+import numpy as np
+import pytest
+
+import amrhip
+
+pytestmark = pytest.mark.gpu
+
+T_END = 0.04
+
+
+def params(t_end=T_END):
+    amrhip.set_params(nelem_subgrid=8, time_scheme=3, time_steps=10 ** 9, dt=1.0, cfl=0.3,
+                      coarse_grid_h=1.0, regrid_interval=10, checkpoint_interval=2 ** 31 - 1,
+                      visualization_interval=2 ** 31 - 1, bc=(0, 0, 0, 0, 1, 1), t_end=t_end, verbose=0)
+
+
+def forest(seed, lmax, lmin=2):
+    ic = amrhip.riemann4_ic(seed)
+    f = amrhip.Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, 8, lmin, lmax)
+    f.apply_ic(ic)
+    f.regrid_initial(ic, 0.1)
+    return f
+
+
+CASES = [(0, 4), (1, 5), (2, 5), (3, 6)]
+
+
+@pytest.mark.parametrize("allow_coarsening", [True, False])
+def test_batch_equals_each_instance_alone(allow_coarsening):
+    params()
+    batch = [forest(s, lm) for s, lm in CASES]
+    insts = [amrhip.make_inst(1.0, (0, 0, 0, 0), 0.3, 1.0, T_END) for _ in CASES]
+    print(f"batch sizes {[f.nquad for f in batch]}", flush=True)
+    per, tot = amrhip.evolve_batch(batch, insts, 0.1, 10, allow_coarsening=allow_coarsening)
+    assert tot.regrids > 0
+    steps = []
+    for (s, lm), fb, st in zip(CASES, batch, per):
+        fa = forest(s, lm)
+        sa = fa.evolve_device(0.1, 10, allow_coarsening=allow_coarsening)
+        print(f"seed {s}: alone {sa.steps} steps {sa.regrids} regrids; batch {st.steps} / {st.regrids}",
+              flush=True)
+        assert st.t == sa.t == pytest.approx(T_END, abs=0)
+        assert st.regrids == sa.regrids
+        assert fb.quads() == fa.quads()
+        assert np.array_equal(fb.soln(), fa.soln())
+        steps.append(sa.steps)
+    # the instances finish at different step counts: the batch really froze some
+    assert len(set(steps)) > 1
+    assert tot.steps == max(steps) or tot.steps - max(steps) < 16
+
+
+def test_batch_with_learned_tagger_runs():
+    params()
+    batch = [forest(s, lm) for s, lm in CASES[:3]]
+    insts = [amrhip.make_inst(1.0, (0, 0, 0, 0), 0.3, 1.0, T_END) for _ in batch]
+    m = amrhip.Mlp(3)
+    tg = amrhip.MlpTagger(m, 0.1, union_plain=True)
+    per, tot = amrhip.evolve_batch(batch, insts, 0.1, 20, tagger=tg)
+    s = tg.stats()
+    assert tot.regrids > 0 and sum(s["predicted"]) > 0
+    for f, st in zip(batch, per):
+        assert st.t == pytest.approx(T_END, abs=0)
+        assert f.is_balanced()
+    m.close()

@@ -173,27 +173,23 @@ def test_features_constant_and_linear(oracle):
     assert np.all(X[:, 4] == 0) and np.all(X[:, 5] == 0)
 
 
-def geometric_neighbour_density(f, soln, k, side, idx):
-    """the neighbour's facing edge cell at sub-face index idx, found by point
-    location in the forest (independent of face_info)"""
-    quads = f.quads()
-    tree, level, x, y = quads[k]
+def geometric_neighbour(f, k, side, idx):
+    """the quadrant holding the neighbour's facing edge cell at sub-face
+    index idx, found by point location in the forest (independent of
+    face_info); None outside the domain"""
+    tree, level, x, y = f.quads()[k]
     size = (1 << 30) >> level
-    L = amrhip.lib()
-    n = 8
-    if side == 1:
-        px, py = x + size, y + (idx * size) // n
-    else:
-        px, py = x + (idx * size) // n, y + size
-    if px >= (1 << 30) or py >= (1 << 30):
+    off = (idx * size) // 8
+    px, py = {0: (x - 1, y + off), 1: (x + size, y + off), 2: (x + off, y - 1), 3: (x + off, y + size)}[side]
+    if not (0 <= px < (1 << 30) and 0 <= py < (1 << 30)):
         return None
-    d = L.amr_forest_find_leaf(f.h, tree, px, py)
-    return d
+    return amrhip.lib().amr_forest_find_leaf(f.h, tree, px, py)
 
 
 def test_features_edge_differences_by_geometry(oracle):
-    """uniform periodic mesh: the +x / +y edge differences use the geometric
-    neighbour's first cell (on a uniform mesh the index-aligned cell is it)"""
+    """uniform mesh with walls: the edge differences use the geometric
+    neighbour's facing cell (on a uniform mesh the index-aligned cell is it)
+    and the cell itself at a wall"""
     rng = np.random.default_rng(6)
     f = amrhip.Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, 8, 2, 2)
     ci, fi, geom, soln = f.external()
@@ -203,15 +199,18 @@ def test_features_edge_differences_by_geometry(oracle):
     for k in range(f.nquad):
         s0 = np.abs(q[k, :, :, 0]).max()
         for j in range(8):
-            d = geometric_neighbour_density(f, soln, k, 1, j)
-            r = q[k, j, 7, 0]
-            rx = r if d is None or d < 0 else q[d, j, 0, 0]
-            assert X[k, 4, j, 7] == bf16_bits(np.float32((rx - r) / s0))
+            e, w = geometric_neighbour(f, k, 1, j), geometric_neighbour(f, k, 0, j)
+            re = q[k, j, 7, 0] if e is None else q[e, j, 0, 0]
+            rw = q[k, j, 0, 0] if w is None else q[w, j, 7, 0]
+            assert X[k, 4, j, 7] == bf16_bits(np.float32((re - q[k, j, 6, 0]) / s0))
+            assert X[k, 4, j, 0] == bf16_bits(np.float32((q[k, j, 1, 0] - rw) / s0))
+            assert X[k, 4, j, 3] == bf16_bits(np.float32((q[k, j, 4, 0] - q[k, j, 2, 0]) / s0))
         for i in range(8):
-            d = geometric_neighbour_density(f, soln, k, 3, i)
-            r = q[k, 7, i, 0]
-            ry = r if d is None or d < 0 else q[d, 0, i, 0]
-            assert X[k, 5, 7, i] == bf16_bits(np.float32((ry - r) / s0))
+            n_, s_ = geometric_neighbour(f, k, 3, i), geometric_neighbour(f, k, 2, i)
+            rn = q[k, 7, i, 0] if n_ is None else q[n_, 0, i, 0]
+            rs = q[k, 0, i, 0] if s_ is None else q[s_, 7, i, 0]
+            assert X[k, 5, 7, i] == bf16_bits(np.float32((rn - q[k, 6, i, 0]) / s0))
+            assert X[k, 5, 0, i] == bf16_bits(np.float32((q[k, 1, i, 0] - rs) / s0))
 
 
 def test_features_hanging_side_uses_both_fine_neighbours(oracle):
@@ -228,6 +227,10 @@ def test_features_hanging_side_uses_both_fine_neighbours(oracle):
     assert len(fine) == 2
     s0 = np.abs(q[0, :, :, 0]).max()
     for j in range(8):
-        r = q[0, j, 7, 0]
-        rx = 0.5 * (q[fine[0], j, 0, 0] + q[fine[1], j, 0, 0])
-        assert X[0, 4, j, 7] == bf16_bits(np.float32((rx - r) / s0))
+        re = 0.5 * (q[fine[0], j, 0, 0] + q[fine[1], j, 0, 0])
+        assert X[0, 4, j, 7] == bf16_bits(np.float32((re - q[0, j, 6, 0]) / s0))
+    # each fine quadrant sees the coarse one across its -x side (index-aligned)
+    for k in fine:
+        sk = np.abs(q[k, :, :, 0]).max()
+        for j in range(8):
+            assert X[k, 4, j, 0] == bf16_bits(np.float32((q[k, j, 1, 0] - q[0, j, 7, 0]) / sk))
