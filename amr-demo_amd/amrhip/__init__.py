@@ -114,7 +114,8 @@ TAG_COARSEN, TAG_KEEP, TAG_REFINE = 0, 1, 2
 class EvolveStats(C.Structure):
     """amr_evolve_stats_t"""
     _fields_ = [("steps", _i), ("regrids", _i), ("peak_quads", _i), ("cell_stages", _d), ("t", _d),
-                ("wall_gpu", _d), ("wall_regrid", _d)]
+                ("wall_gpu", _d), ("wall_regrid", _d), ("wall_tags", _d), ("wall_forest", _d),
+                ("wall_remesh", _d)]
 
 
 TAGGER_FN = C.CFUNCTYPE(_i, _vp, _vp, _vp, _pi)

@@ -7,7 +7,9 @@
 #include "amr_hip.h"
 #include "amr_internal.h"
 
+#include <pthread.h>
 #include <time.h>
+#include <unistd.h>
 
 #include <math.h>
 #include <stdlib.h>
@@ -359,17 +361,25 @@ int amr_device_setup(amr_forest_t *f, amrgpu_ctx **out)
     return AMRGPU_OK;
 }
 
-int amr_device_regrid(amr_forest_t *f, amrgpu_ctx *g, const int *tags, int allow_coarsening)
+double amr_now(void) { return now_s(); }
+
+int amr_device_regrid(amr_forest_t *f, amrgpu_ctx *g, const int *tags, int allow_coarsening, double *t_forest,
+                      double *t_remesh)
 {
     int ncell, nghost, nface, dof;
     int *cell_info, *face_info;
     double *geom, *soln;
+    double t0 = now_s();
     amr_forest_track_begin(f, 0);
     amr_regrid_tags(f, tags, allow_coarsening);
     if (!amr_forest_track_end(f)) return AMRGPU_ERR_MESH;
     if (amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom, &soln))
         return AMRGPU_ERR_MESH;
-    return amrgpu_remesh(g, ncell, cell_info, NULL, nface, face_info, amr_forest_track_src(f));
+    double t1 = now_s();
+    int rc = amrgpu_remesh(g, ncell, cell_info, NULL, nface, face_info, amr_forest_track_src(f));
+    if (t_forest) *t_forest += t1 - t0;
+    if (t_remesh) *t_remesh += now_s() - t1;
+    return rc;
 }
 
 int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_stats_t *st)
@@ -425,8 +435,9 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
             rc = amrgpu_indicator(g, ind);
             for (int k = 0; k < nq; ++k) tags[k] = ind[k] > (o ? o->tol : 0.1);
         }
+        s.wall_tags += now_s() - t2;
         if (rc) break;
-        rc = amr_device_regrid(f, g, tags, o ? o->allow_coarsening : 1);
+        rc = amr_device_regrid(f, g, tags, o ? o->allow_coarsening : 1, &s.wall_forest, &s.wall_remesh);
         ncell = amr_forest_num_quads(f);
         s.regrids++;
         if (ncell > s.peak_quads) s.peak_quads = ncell;
@@ -485,19 +496,35 @@ static int batch_grow(batch_bufs_t *b, size_t nc, size_t nf, int n2)
     return 0;
 }
 
+typedef struct {
+    int ncell, nface, rc;
+    int *cell_info, *face_info;
+    double *geom, *soln;
+} ext_t;
+
+static int ext_of(amr_forest_t *f, ext_t *e)
+{
+    int nghost, dof;
+    e->rc = amr_forest_build_external(f, &e->ncell, &nghost, &e->nface, &dof, &e->cell_info, &e->face_info,
+                                      &e->geom, &e->soln)
+                ? AMRGPU_ERR_MESH
+                : AMRGPU_OK;
+    return e->rc;
+}
+
 /* Concatenated boundary arrays of all forests (quadrants of instance k at
- * base[k] .. base[k+1]-1; face cell ids shifted by the same offsets). */
-static int batch_build(amr_forest_t **f, int ninst, int n, int *base, batch_bufs_t *b, int *ncell_out,
+ * base[k] .. base[k+1]-1; face cell ids shifted by the same offsets) from
+ * each forest's own arrays ext[k]. */
+static int batch_build(const ext_t *ext, int ninst, int n, int *base, batch_bufs_t *b, int *ncell_out,
                        int *nface_out, int with_data)
 {
     const int n2 = n * n;
     size_t nc = 0, nf = 0;
     for (int k = 0; k < ninst; ++k) {
-        int ncell, nghost, nface, dof;
-        int *cell_info, *face_info;
-        double *geom, *soln;
-        if (amr_forest_build_external(f[k], &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom, &soln))
-            return AMRGPU_ERR_MESH;
+        if (ext[k].rc) return ext[k].rc;
+        const int ncell = ext[k].ncell, nface = ext[k].nface;
+        const int *cell_info = ext[k].cell_info, *face_info = ext[k].face_info;
+        const double *geom = ext[k].geom, *soln = ext[k].soln;
         base[k] = (int)nc;
         if (batch_grow(b, nc + (size_t)ncell, nf + (size_t)nface, n2)) return AMRGPU_ERR_ARG;
         for (int q = 0; q < ncell; ++q) {
@@ -525,6 +552,61 @@ static int batch_build(amr_forest_t **f, int ninst, int n, int *base, batch_bufs
     return AMRGPU_OK;
 }
 
+/* Per-instance regrid work of one batch regrid, shared by the host threads:
+ * instance k (taken in turn from a counter) is regridded by its tags when it
+ * has not reached its stop, and its boundary arrays are rebuilt. */
+typedef struct {
+    amr_forest_t **f;
+    const int *done, *tags, *obase;
+    int allow, ninst;
+    ext_t *ext;
+    int *ok;
+    int next;
+    pthread_mutex_t lock;
+} regrid_job_t;
+
+static void *regrid_worker(void *arg)
+{
+    regrid_job_t *j = (regrid_job_t *)arg;
+    for (;;) {
+        pthread_mutex_lock(&j->lock);
+        const int k = j->next++;
+        pthread_mutex_unlock(&j->lock);
+        if (k >= j->ninst) break;
+        j->ok[k] = 1;
+        if (!j->done[k]) {
+            amr_forest_track_begin(j->f[k], 0);
+            amr_regrid_tags(j->f[k], j->tags + j->obase[k], j->allow);
+            j->ok[k] = amr_forest_track_end(j->f[k]);
+        }
+        ext_of(j->f[k], &j->ext[k]);
+    }
+    return NULL;
+}
+
+static int host_threads(int work)
+{
+    const char *e = getenv("AMR_HOST_THREADS");
+    long n = e ? strtol(e, NULL, 10) : sysconf(_SC_NPROCESSORS_ONLN);
+    if (n > 16) n = 16;
+    if (n > work) n = work;
+    return n < 1 ? 1 : (int)n;
+}
+
+static void regrid_all(regrid_job_t *j)
+{
+    const int nt = host_threads(j->ninst);
+    j->next = 0;
+    pthread_mutex_init(&j->lock, NULL);
+    pthread_t th[16];
+    int started = 0;
+    for (int t = 1; t < nt; ++t)
+        if (pthread_create(&th[started], NULL, regrid_worker, j) == 0) ++started;
+    regrid_worker(j);
+    for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
+    pthread_mutex_destroy(&j->lock);
+}
+
 int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, const amr_evolve_opts_t *o,
                      amr_evolve_stats_t *per_inst, amr_evolve_stats_t *total)
 {
@@ -545,6 +627,9 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
     int *base = (int *)malloc(sizeof(int) * (size_t)(ninst + 1));
     int *done = (int *)calloc((size_t)ninst, sizeof(int));
     double *t = (double *)calloc((size_t)ninst, sizeof(double));
+    ext_t *ext = (ext_t *)calloc((size_t)ninst, sizeof(ext_t));
+    int *ok = (int *)calloc((size_t)ninst, sizeof(int));
+    int *obase = (int *)malloc(sizeof(int) * (size_t)(ninst + 1));
     batch_bufs_t b;
     memset(&b, 0, sizeof(b));
     double *ind = NULL;
@@ -552,8 +637,9 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
     int ncell = 0, nface = 0, rc = 0, istep = 0, cap = 0;
     amrgpu_ctx *g = NULL;
     double t0 = now_s();
-    if (!st || !base || !done || !t) rc = AMRGPU_ERR_ARG;
-    if (!rc) rc = batch_build(f, ninst, n, base, &b, &ncell, &nface, 1);
+    if (!st || !base || !done || !t || !ext || !ok || !obase) rc = AMRGPU_ERR_ARG;
+    for (int k = 0; k < ninst && !rc; ++k) rc = ext_of(f[k], &ext[k]);
+    if (!rc) rc = batch_build(ext, ninst, n, base, &b, &ncell, &nface, 1);
     if (!rc) {
         g = amrgpu_create();
         if (!g) rc = AMRGPU_ERR_HIP;
@@ -607,19 +693,23 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
             rc = amrgpu_indicator(g, ind);
             for (int q = 0; q < ncell; ++q) tags[q] = ind[q] > (o ? o->tol : 0.1);
         }
+        double t3 = now_s();
+        tot.wall_tags += t3 - t2;
         if (rc) break;
-        /* old quadrant ranges, then the tracked sources shifted into them */
-        int *obase = (int *)malloc(sizeof(int) * (size_t)(ninst + 1));
-        if (!obase) { rc = AMRGPU_ERR_ARG; break; }
+        /* old quadrant ranges, the instances regridded on the host threads,
+         * then the tracked sources shifted into the old ranges */
         memcpy(obase, base, sizeof(int) * (size_t)(ninst + 1));
-        for (int k = 0; k < ninst && !rc; ++k) {
-            if (done[k]) continue;
-            amr_forest_track_begin(f[k], 0);
-            amr_regrid_tags(f[k], tags + obase[k], o ? o->allow_coarsening : 1);
-            if (!amr_forest_track_end(f[k])) rc = AMRGPU_ERR_MESH;
-            st[k].regrids++;
+        regrid_job_t job;
+        memset(&job, 0, sizeof(job));
+        job.f = f; job.done = done; job.tags = tags; job.obase = obase;
+        job.allow = o ? o->allow_coarsening : 1;
+        job.ninst = ninst; job.ext = ext; job.ok = ok;
+        regrid_all(&job);
+        for (int k = 0; k < ninst; ++k) {
+            if (!ok[k]) rc = AMRGPU_ERR_MESH;
+            if (!done[k]) st[k].regrids++;
         }
-        if (!rc) rc = batch_build(f, ninst, n, base, &b, &ncell, &nface, 0);
+        if (!rc) rc = batch_build(ext, ninst, n, base, &b, &ncell, &nface, 0);
         for (int k = 0; k < ninst && !rc; ++k) {
             const int nk = base[k + 1] - base[k];
             if (done[k]) {
@@ -631,8 +721,10 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
             }
             if (nk > st[k].peak_quads) st[k].peak_quads = nk;
         }
-        free(obase);
+        double t4 = now_s();
+        tot.wall_forest += t4 - t3;
         if (!rc) rc = amrgpu_remesh(g, ncell, b.ci, b.cinst, nface, b.fi, b.src);
+        tot.wall_remesh += now_s() - t4;
         tot.regrids++;
         if (ncell > tot.peak_quads) tot.peak_quads = ncell;
         tot.wall_regrid += now_s() - t2;
@@ -653,6 +745,6 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
     if (per_inst && st) memcpy(per_inst, st, sizeof(amr_evolve_stats_t) * (size_t)ninst);
     if (total) *total = tot;
     batch_free(&b);
-    free(st); free(base); free(done); free(t); free(ind); free(tags);
+    free(st); free(base); free(done); free(t); free(ind); free(tags); free(ext); free(ok); free(obase);
     return rc;
 }

@@ -29,6 +29,16 @@ struct amr_forest {
     /* regrid tracking: source of every quadrant's patch (see track_begin) */
     int *src;
     int track, track_data, track_ok;
+    /* hash index (tree, level, Morton key) -> quadrant, rebuilt lazily after
+     * any change of the quadrant array */
+    struct hentry *hash;
+    int hcap, hvalid;
+};
+
+struct hentry {
+    uint64_t key;      /* Morton key of the lower-left corner */
+    int meta;          /* tree * 32 + level, -1 = empty */
+    int idx;
 };
 
 static inline int qlen(int level) { return 1 << (AMR_MAXLEVEL - level); }
@@ -46,6 +56,51 @@ static inline uint64_t spread(uint32_t v)
 static inline uint64_t morton(int x, int y) { return spread((uint32_t)x) | (spread((uint32_t)y) << 1); }
 
 static int patch_size(const amr_forest_t *f) { return 4 * f->n * f->n; }
+
+static inline uint64_t hmix(uint64_t key, int meta)
+{
+    uint64_t z = key ^ ((uint64_t)(uint32_t)meta * 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static void hash_build(amr_forest_t *f)
+{
+    int cap = 16;
+    while (cap < 2 * f->nquad) cap <<= 1;
+    if (cap != f->hcap) {
+        free(f->hash);
+        f->hash = (struct hentry *)malloc(sizeof(struct hentry) * (size_t)cap);
+        f->hcap = cap;
+    }
+    for (int i = 0; i < cap; ++i) f->hash[i].meta = -1;
+    const uint64_t mask = (uint64_t)cap - 1;
+    for (int k = 0; k < f->nquad; ++k) {
+        const int meta = f->q[k].tree * 32 + f->q[k].level;
+        uint64_t h = hmix(f->key[k], meta) & mask;
+        while (f->hash[h].meta >= 0) h = (h + 1) & mask;
+        f->hash[h].key = f->key[k];
+        f->hash[h].meta = meta;
+        f->hash[h].idx = k;
+    }
+    f->hvalid = 1;
+}
+
+/* index of the quadrant (tree, level, x, y) with (x, y) its lower-left
+ * corner, or -1 */
+static int hash_find(amr_forest_t *f, int tree, int level, int x, int y)
+{
+    if (!f->hvalid) hash_build(f);
+    const uint64_t key = morton(x, y);
+    const int meta = tree * 32 + level;
+    const uint64_t mask = (uint64_t)f->hcap - 1;
+    for (uint64_t h = hmix(key, meta) & mask;; h = (h + 1) & mask) {
+        const struct hentry *e = &f->hash[h];
+        if (e->meta < 0) return -1;
+        if (e->meta == meta && e->key == key) return e->idx;
+    }
+}
 
 static void reserve(amr_forest_t *f, int cap)
 {
@@ -101,7 +156,7 @@ void amr_forest_destroy(amr_forest_t *f)
 {
     if (!f) return;
     free(f->q); free(f->key); free(f->soln); free(f->tag); free(f->type);
-    free(f->cell_info); free(f->face_info); free(f->geom); free(f->src);
+    free(f->cell_info); free(f->face_info); free(f->geom); free(f->src); free(f->hash);
     free(f);
 }
 
@@ -169,6 +224,23 @@ int amr_forest_find_leaf(const amr_forest_t *f, int tree, int x, int y)
     return best;
 }
 
+/* Leaf containing the point (tree, x, y), looked up first at levels
+ * level, level - 1 and level + 1 through the hash index (the only
+ * possibilities across a face or corner of a 2:1 balanced forest), then by
+ * binary search. */
+static int leaf_near(amr_forest_t *f, int tree, int x, int y, int level)
+{
+    static const int order[3] = {0, -1, 1};
+    for (int o = 0; o < 3; ++o) {
+        const int l = level + order[o];
+        if (l < 0 || l >= AMR_MAXLEVEL) continue;
+        const int len = qlen(l);
+        const int k = hash_find(f, tree, l, x & ~(len - 1), y & ~(len - 1));
+        if (k >= 0) return k;
+    }
+    return amr_forest_find_leaf(f, tree, x, y);
+}
+
 /* Same-size neighbour region of quadrant q in direction (dx, dy); returns 0
  * if it lies outside a non-periodic domain. */
 static int neighbour_region(const amr_forest_t *f, const amr_quad_t *q, int dx, int dy,
@@ -200,12 +272,12 @@ static const int SIDE_DY[4] = {0, 0, -1, 1};
 /* Classify side s of quadrant k: returns 0 boundary, 1 same-size neighbour
  * (nb[0]), 2 coarser neighbour (nb[0]), 3 two finer neighbours (nb[0], nb[1])
  * in order of increasing coordinate along the face, -1 on an inconsistency. */
-static int classify_side(const amr_forest_t *f, int k, int s, int nb[2])
+static int classify_side(amr_forest_t *f, int k, int s, int nb[2])
 {
     const amr_quad_t *q = &f->q[k];
     int t, nx, ny;
     if (!neighbour_region(f, q, SIDE_DX[s], SIDE_DY[s], &t, &nx, &ny)) return 0;
-    int L = amr_forest_find_leaf(f, t, nx, ny);
+    int L = leaf_near(f, t, nx, ny, q->level);
     if (L < 0) return -1;
     if (f->q[L].level == q->level) { nb[0] = L; return 1; }
     if (f->q[L].level < q->level) { nb[0] = L; return 2; }
@@ -217,8 +289,8 @@ static int classify_side(const amr_forest_t *f, int k, int s, int nb[2])
     case 2: ay = ny + h; by = ny + h; bx = nx + h; break;   /* neighbour's yhi children */
     case 3: bx = nx + h; break;                             /* neighbour's ylo children */
     }
-    nb[0] = amr_forest_find_leaf(f, t, ax, ay);
-    nb[1] = amr_forest_find_leaf(f, t, bx, by);
+    nb[0] = leaf_near(f, t, ax, ay, q->level + 1);
+    nb[1] = leaf_near(f, t, bx, by, q->level + 1);
     if (nb[0] < 0 || nb[1] < 0) return -1;
     if (f->q[nb[0]].level != q->level + 1 || f->q[nb[1]].level != q->level + 1) return -1;
     return 3;
@@ -312,6 +384,7 @@ int amr_forest_refine(amr_forest_t *f, const int *flags)
     f->q = q; f->key = key; f->soln = soln; f->tag = tag; f->type = type;
     if (src) { free(f->src); f->src = src; }
     f->nquad = nn; f->cap = nn;
+    f->hvalid = 0;
     return cnt;
 }
 
@@ -378,10 +451,11 @@ int amr_forest_coarsen(amr_forest_t *f)
         }
     }
     f->nquad = j;
+    f->hvalid = 0;
     return cnt;
 }
 
-static int balance_flags(const amr_forest_t *f, int *flags)
+static int balance_flags(amr_forest_t *f, int *flags)
 {
     int any = 0;
     memset(flags, 0, sizeof(int) * (size_t)f->nquad);
@@ -393,7 +467,7 @@ static int balance_flags(const amr_forest_t *f, int *flags)
                 if (!dx && !dy) continue;
                 int t, nx, ny;
                 if (!neighbour_region(f, q, dx, dy, &t, &nx, &ny)) continue;
-                int L = amr_forest_find_leaf(f, t, nx, ny);
+                int L = leaf_near(f, t, nx, ny, q->level);
                 if (L >= 0 && f->q[L].level < q->level - 1) { flags[L] = 1; any = 1; }
             }
     }
@@ -420,7 +494,7 @@ int amr_forest_balance(amr_forest_t *f)
     return total;
 }
 
-int amr_forest_is_balanced(const amr_forest_t *f)
+int amr_forest_is_balanced(amr_forest_t *f)
 {
     int *flags = (int *)malloc(sizeof(int) * (size_t)(f->nquad > 0 ? f->nquad : 1));
     int any = balance_flags(f, flags);

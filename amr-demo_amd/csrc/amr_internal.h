@@ -20,9 +20,12 @@ AMR_HIDDEN void amr_set_error(const char *fmt, ...);
 /* Device context for the whole forest with the current physics settings
  * (one instance; h_base = coarse_grid_h, walls bc[0..3]). */
 AMR_HIDDEN int amr_device_setup(struct amr_forest *f, struct amrgpu_ctx **out);
-/* Regrid the forest by tags (amr_regrid_tags) and remap the device state. */
+/* Regrid the forest by tags (amr_regrid_tags) and remap the device state;
+ * wall seconds of the host forest part and of the device remesh are added to
+ * *t_forest and *t_remesh (either may be NULL). */
 AMR_HIDDEN int amr_device_regrid(struct amr_forest *f, struct amrgpu_ctx *g, const int *tags,
-                                 int allow_coarsening);
+                                 int allow_coarsening, double *t_forest, double *t_remesh);
+AMR_HIDDEN double amr_now(void);
 
 #ifdef __cplusplus
 }

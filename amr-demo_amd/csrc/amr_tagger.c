@@ -109,7 +109,7 @@ int amr_tagger_collect(amr_forest_t *f, const amr_collect_opts_t *o, amr_tagset_
         if (finished(&p, istep, t)) break;
         /* regrid with the plain indicator of the current state */
         for (int k = 0; k < nq; ++k) b.tags[k] = b.indt[k] > o->tol;
-        rc = amr_device_regrid(f, g, b.tags, o->allow_coarsening);
+        rc = amr_device_regrid(f, g, b.tags, o->allow_coarsening, NULL, NULL);
     }
     if (!rc) rc = amrgpu_download_soln(g, amr_forest_soln(f));
     amrgpu_destroy(g);

@@ -138,6 +138,7 @@ def learned_experiment(amrhip, args):
     m, losses = L.train(ts, epochs=args.learned_epochs)
     t_train = time.perf_counter() - t0
     rec = L.recall_report(m, held, args.tol)
+    rec["at_equal_refine_fraction"] = L.recall_at_equal_fraction(m, held, args.tol)
     log(f"tagger: {ts.size} training rows ({t_collect:.1f} s to collect), trained in {t_train:.2f} s, "
         f"loss {losses[0]:.3f} -> {losses[-1]:.3f}; held-out recall mlp {rec['recall_mlp']:.3f} "
         f"vs indicator-now {rec['recall_indicator_now']:.3f}")
@@ -152,7 +153,6 @@ def learned_experiment(amrhip, args):
             f"L1 {r['l1_density']:.5e}")
     base = runs["plain_every_10"]
     lr = runs[f"learned_every_{iv}"]
-    m.close()
     return {
         "deployed_workload": f"riemann4 seed {args.seed}, levels {args.lmin}-{lmax}, 8x8 FP64, t={args.t_end}",
         "reference": f"plain FP64 on the uniform level-{lmax} forest ({ref.shape[0]}^2 cells)",
@@ -164,7 +164,69 @@ def learned_experiment(amrhip, args):
         "runs": runs,
         "speedup_end_to_end": lr["end_to_end_Mcell_upd_s"] / base["end_to_end_Mcell_upd_s"],
         "l1_ratio": lr["l1_density"] / base["l1_density"],
-    }
+    }, m
+
+
+def batched_experiment(amrhip, args, model):
+    """K independent Riemann problems (seeds 1000+k, max level drawn from
+    {5,6,7,8}, levels lmin..max) advanced together in one device context
+    (amr_evolve_batch, one launch per RK stage for all), against the same
+    problems run one after another on the plain path; per-instance final
+    states compared bit for bit; then the batch with the learned tagger
+    every --learned-interval steps.  Rates count the cells of instances that
+    have not reached their stop."""
+    from amrhip import learned as L
+    rng = np.random.default_rng(2024)
+    K = args.batch_instances
+    lmaxs = [int(x) for x in rng.choice([5, 6, 7, 8], size=K)]
+    seeds = [1000 + k for k in range(K)]
+    t_end = args.batch_t_end
+
+    def forests():
+        return [L.riemann_forest(s, args.lmin, lm, args.tol) for s, lm in zip(seeds, lmaxs)]
+
+    insts = [amrhip.make_inst(1.0, (0, 0, 0, 0), 0.3, 1.0, t_end) for _ in seeds]
+    out = {"instances": K, "max_levels": lmaxs, "t_end": t_end}
+    # one by one, plain every 10 steps
+    L.riemann_params(t_end, regrid=10)
+    alone, wall_alone, cs_alone = [], 0.0, 0.0
+    for fa in forests():
+        t0 = time.perf_counter()
+        st = fa.evolve_device(args.tol, 10)
+        wall_alone += time.perf_counter() - t0
+        cs_alone += st.cell_stages
+        alone.append(fa)
+    out["sequential_plain_every_10"] = {"wall_s": round(wall_alone, 3), "cell_stages": cs_alone,
+                                        "end_to_end_Mcell_upd_s": cs_alone / wall_alone / 1e6}
+    log(f"batch: {K} instances one by one: {wall_alone:.2f} s, {cs_alone / wall_alone / 1e6:.0f} Mcell-upd/s")
+    runs = [("batched_plain_every_10", 10, None)]
+    if model is not None:
+        runs.append((f"batched_learned_every_{args.learned_interval}", args.learned_interval,
+                     amrhip.MlpTagger(model, args.tol, True)))
+    for name, iv, tagger in runs:
+        fb = forests()
+        t0 = time.perf_counter()
+        per, tot = amrhip.evolve_batch(fb, insts, args.tol, iv, tagger=tagger)
+        wall = time.perf_counter() - t0
+        useful = sum(p.cell_stages for p in per)
+        r = {"wall_s": round(wall, 3), "steps": tot.steps, "regrids": tot.regrids,
+             "peak_cells": tot.peak_quads * 64, "gpu_s": round(tot.wall_gpu, 3),
+             "regrid_s": round(tot.wall_regrid, 3),
+             "regrid_split_s": {"tags": round(tot.wall_tags, 3), "host_forest": round(tot.wall_forest, 3),
+                                "device_remesh": round(tot.wall_remesh, 3)},
+             "cell_stages": useful,
+             "end_to_end_Mcell_upd_s": useful / wall / 1e6,
+             "launched_Mcell_upd_s": tot.cell_stages / wall / 1e6}
+        if tagger is None:
+            same = [bool(a.quads() == b.quads() and np.array_equal(a.soln(), b.soln())) for a, b in zip(alone, fb)]
+            r["instances_bitwise_equal_to_alone"] = int(sum(same))
+        out[name] = r
+        log(f"{name}: {wall:.2f} s, {r['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s end to end, "
+            f"{tot.regrids} regrids, peak {tot.peak_quads * 64} cells")
+    base = out["sequential_plain_every_10"]["end_to_end_Mcell_upd_s"]
+    for name, _, _ in runs:
+        out[name]["speedup_vs_sequential_plain"] = out[name]["end_to_end_Mcell_upd_s"] / base
+    return out
 
 
 def main():
@@ -192,7 +254,10 @@ def main():
     ap.add_argument("--learned-seeds", type=int, default=8, help="training runs (seeds 1..k)")
     ap.add_argument("--learned-lmax", type=int, default=8, help="finest level of the deployed run")
     ap.add_argument("--learned-interval", type=int, default=40)
-    ap.add_argument("--learned-epochs", type=int, default=12)
+    ap.add_argument("--learned-epochs", type=int, default=30)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances comparison")
+    ap.add_argument("--batch-instances", type=int, default=16)
+    ap.add_argument("--batch-t-end", type=float, default=0.25)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (the whole forest lives on one "
@@ -304,6 +369,8 @@ def main():
             "steps": st.steps, "regrids": st.regrids, "t": round(st.t, 6),
             "cell_stages": st.cell_stages, "wall_s": round(wall, 3),
             "gpu_s": round(st.wall_gpu, 3), "regrid_s": round(st.wall_regrid, 3),
+            "regrid_split_s": {"tags": round(st.wall_tags, 3), "host_forest": round(st.wall_forest, 3),
+                               "device_remesh": round(st.wall_remesh, 3)},
             "end_to_end_Mcell_updates_per_s": round(st.cell_stages / wall / 1e6, 3),
         }
         if host_run is not None:
@@ -328,8 +395,13 @@ def main():
                           f"C oracle {label}, {threads} threads on contiguous quadrant ranges",
             }
             out["gpu_over_cpu"] = round(value / v, 1)
+    model = None
     if args.workload == "riemann4" and args.n == 8 and not args.no_learned:
-        out["learned"] = learned_experiment(amrhip, args)
+        out["learned"], model = learned_experiment(amrhip, args)
+    if args.workload == "riemann4" and args.n == 8 and not args.no_batch:
+        out["batched"] = batched_experiment(amrhip, args, model)
+    if model is not None:
+        model.close()
     print(json.dumps(out), flush=True)
 
 

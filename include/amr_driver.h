@@ -75,8 +75,11 @@ typedef struct {
     int    peak_quads;       /* largest quadrant count seen */
     double cell_stages;      /* sum over steps of sub-cells x RK stages */
     double t;                /* physical time reached */
-    double wall_gpu;         /* seconds inside physics_go_gpu */
-    double wall_regrid;      /* seconds in host regrid + boundary build */
+    double wall_gpu;         /* seconds inside physics_go_gpu / time stepping */
+    double wall_regrid;      /* seconds in regrids (the three parts below) + setup */
+    double wall_tags;        /* of which: indicator / tagger and tag download */
+    double wall_forest;      /* of which: host forest regrid + boundary arrays */
+    double wall_remesh;      /* of which: device face table + state remap */
 } amr_evolve_stats_t;
 
 /* The evolve loop of src/amr/src/amr_evolve.c:15-73 on the host forest: per

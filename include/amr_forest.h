@@ -73,7 +73,7 @@ int amr_forest_coarsen(amr_forest_t *f);
  * quadrants copy their parent patch.  Returns quadrants refined. */
 int amr_forest_balance(amr_forest_t *f);
 /* 1 if the forest is 2:1 balanced across faces and corners. */
-int amr_forest_is_balanced(const amr_forest_t *f);
+int amr_forest_is_balanced(amr_forest_t *f);
 
 /* Regrid tracking.  Between track_begin and track_end, refine / coarsen /
  * balance record where each quadrant's patch comes from; with keep_data = 0
