@@ -185,10 +185,12 @@ def batched_experiment(amrhip, args, model):
     def forests():
         return [L.riemann_forest(s, args.lmin, lm, args.tol) for s, lm in zip(seeds, lmaxs)]
 
-    insts = [amrhip.make_inst(1.0, (0, 0, 0, 0), 0.3, 1.0, t_end) for _ in seeds]
-    out = {"instances": K, "max_levels": lmaxs, "t_end": t_end}
-    # one by one, plain every 10 steps
+    # one by one, plain every 10 steps; the batch instances take the same
+    # settings (effective CFL 0.25 * 0.4 * 3 as computed from the input)
     L.riemann_params(t_end, regrid=10)
+    p = amrhip.get_params()
+    insts = [amrhip.make_inst(p.coarse_grid_h, tuple(p.bc[:4]), p.cfl, p.dt, t_end) for _ in seeds]
+    out = {"instances": K, "max_levels": lmaxs, "t_end": t_end}
     alone, wall_alone, cs_alone = [], 0.0, 0.0
     for fa in forests():
         t0 = time.perf_counter()
