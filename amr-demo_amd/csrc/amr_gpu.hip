@@ -398,6 +398,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     using TL = Tile<N, PPB>;
     constexpr int N2 = TL::N2, T = TL::T;
     __shared__ double lds[TL::LDS_DOUBLES];
+    __shared__ double ldsU[PPB * NFLD * N2];     // own state, kept out of registers
     __shared__ double red[(TL::BLK + 63) / 64];
     __shared__ int pquad[PPB], plevel[PPB], pinst[PPB];
     __shared__ int pnbr[PPB][8];
@@ -445,10 +446,6 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     if (active) {
 #pragma unroll
         for (int f = 0; f < 4; ++f) U[f] = a.Us[base + f * N2];
-        if (needB) {
-#pragma unroll
-            for (int f = 0; f < 4; ++f) B[f] = a.U0[base + f * N2];
-        }
     }
     // ghost ring: each thread fetches its neighbour ids itself, so the halo
     // loads do not wait for the block barrier
@@ -476,7 +473,11 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         }
         store_rec<N, PPB>(lds, p, tpos, q0, q1, q2, q3);
     }
-    if (active) store_rec<N, PPB>(lds, p, (cj + 1) * T + (ci + 1), U[0], U[1], U[2], U[3]);
+    if (active) {
+        store_rec<N, PPB>(lds, p, (cj + 1) * T + (ci + 1), U[0], U[1], U[2], U[3]);
+#pragma unroll
+        for (int f = 0; f < 4; ++f) ldsU[(p * NFLD + f) * N2 + c] = U[f];
+    }
     __syncthreads();
 
     // ---- face fluxes into registers --------------------------------------
@@ -555,6 +556,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 }
                 dC = tp * TL::FLUX_PER_PATCH + (xs ? k * (N + 1) : NFLD * N * (N + 1) + k);
             }
+        }
+        if (active && needB) {   // base state: loaded late, its latency overlaps the flux gather
+#pragma unroll
+            for (int f = 0; f < 4; ++f) B[f] = a.U0[base + f * N2];
         }
         __syncthreads();   // all reads of the primitive tile are done
         const int stride = N * (N + 1);
@@ -668,6 +673,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             // little ILP and costs the registers that bound occupancy
             __builtin_amdgcn_sched_barrier(0);
         }
+        if (active && needB) {   // base state: loaded late, its latency overlaps the flux gather
+#pragma unroll
+            for (int f = 0; f < 4; ++f) B[f] = a.U0[base + f * N2];
+        }
         __syncthreads();   // all reads of the primitive tile are done
     #pragma unroll
         for (int r = 0; r < TL::ROUNDS; ++r) {
@@ -680,6 +689,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     }
 
     // ---- per-cell residual and stage update ------------------------------
+    if (active) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) U[f] = ldsU[(p * NFLD + f) * N2 + c];
+    }
     double k4[4] = {0.0, 0.0, 0.0, 0.0};
     double out[4] = {0.0, 0.0, 0.0, 0.0};
     double cand = INFINITY;
@@ -1004,6 +1017,7 @@ const TileVariant kVariants[] = {
     {4, 8, 1, 0}, {4, 8, 1, 1},
     {8, 2, 1, 0}, {8, 4, 1, 0}, {8, 4, 4, 0}, {8, 2, 1, 1}, {8, 4, 1, 1}, {8, 4, 4, 1},
     {16, 1, 1, 0}, {16, 1, 4, 0}, {16, 1, 1, 1}, {16, 1, 4, 1},
+    {8, 1, 1, 1}, {8, 1, 4, 1}, {8, 2, 4, 1}, {8, 1, 5, 1},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -1050,6 +1064,10 @@ int launch_stage(amrgpu_ctx *c, const StageArgs &a)
     case 11: return launch_v<16, 1, 4, 0>(c, a);
     case 12: return launch_v<16, 1, 1, 1>(c, a);
     case 13: return launch_v<16, 1, 4, 1>(c, a);
+    case 14: return launch_v<8, 1, 1, 1>(c, a);
+    case 15: return launch_v<8, 1, 4, 1>(c, a);
+    case 16: return launch_v<8, 2, 4, 1>(c, a);
+    case 17: return launch_v<8, 1, 5, 1>(c, a);
     default: set_err("bad tile variant %d", c->variant); return AMRGPU_ERR_ARG;
     }
 }
