@@ -67,6 +67,12 @@ EXPORTS = {
     "amr_forest_balance": (_i, [_vp]),
     "amr_forest_is_balanced": (_i, [_vp]),
     "amr_forest_find_leaf": (_i, [_vp, _i, _i, _i]),
+    "amr_forest_brick": (None, [_vp, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pi, _pi]),
+    "amr_forest_from_quads": (_vp, [_i, _i, _i, _i, _d, _d, _d, _i, _i, _i, _i, _vp]),
+    "amr_forest_save": (_i, [_vp, C.c_char_p, _i, _d]),
+    "amr_forest_load": (_vp, [C.c_char_p, _pi, _pd]),
+    "amr_forest_write_vtk": (_i, [_vp, C.c_char_p, _i]),
+    "amr_set_output_dir": (None, [C.c_char_p]),
     # include/amr_driver.h
     "amr_ic_patch": (None, [_vp, _i, _pd, _pd]),
     "amr_ic_apply": (None, [_vp, _vp]),
@@ -228,6 +234,45 @@ class Forest:
         self.n = n
         self.scale = scale
         self.ntx, self.nty, self.xlo, self.ylo = ntx, nty, xlo, ylo
+
+    @classmethod
+    def _wrap(cls, h):
+        f = cls.__new__(cls)
+        f.L = lib()
+        f.h = h
+        a = [_i() for _ in range(6)]
+        d = [_d() for _ in range(3)]
+        f.L.amr_forest_brick(h, C.byref(a[0]), C.byref(a[1]), C.byref(a[2]), C.byref(a[3]), C.byref(d[0]),
+                             C.byref(d[1]), C.byref(d[2]), C.byref(a[4]), C.byref(a[5]))
+        f.ntx, f.nty, f.xlo, f.ylo, f.scale = a[0].value, a[1].value, d[0].value, d[1].value, d[2].value
+        f.n = f.L.amr_forest_nelem_subgrid(h)
+        return f
+
+    @classmethod
+    def load(cls, path):
+        """forest from an amr_forest_save checkpoint -> (forest, istep, t)"""
+        istep, t = _i(), _d()
+        h = lib().amr_forest_load(os.fsencode(path), C.byref(istep), C.byref(t))
+        if not h:
+            raise ValueError(f"{path}: not a checkpoint of this format")
+        return cls._wrap(h), istep.value, t.value
+
+    @classmethod
+    def from_quads(cls, ntx, nty, periodic, xlo, ylo, scale, n, min_level, max_level, quads):
+        arr = (Quad * len(quads))(*[Quad(*q) for q in quads])
+        h = lib().amr_forest_from_quads(ntx, nty, int(periodic[0]), int(periodic[1]), xlo, ylo, scale, n,
+                                        min_level, max_level, len(quads), arr)
+        if not h:
+            raise ValueError("amr_forest_from_quads rejected the quadrants")
+        return cls._wrap(h)
+
+    def save(self, path, istep=0, t=0.0):
+        if self.L.amr_forest_save(self.h, os.fsencode(path), istep, t):
+            raise OSError(f"amr_forest_save({path}) failed")
+
+    def write_vtk(self, path, subcells=False):
+        if self.L.amr_forest_write_vtk(self.h, os.fsencode(path), int(subcells)):
+            raise OSError(f"amr_forest_write_vtk({path}) failed")
 
     def __del__(self):
         try:
@@ -547,6 +592,11 @@ class Gpu:
         X = np.zeros((self.nquad(), 384), np.uint16)
         check(self.L.amrgpu_tagger_features_host(self.h, X.ctypes.data), "amrgpu_tagger_features_host")
         return X
+
+
+def set_output_dir(path):
+    """amr_set_output_dir: where the evolve loops write VTK and checkpoints (None: nowhere)"""
+    lib().amr_set_output_dir(os.fsencode(path) if path else None)
 
 
 def set_params(**kw):

@@ -8,6 +8,8 @@
 #include "amr_internal.h"
 
 #include <pthread.h>
+#include <stdio.h>
+#include <sys/stat.h>
 #include <time.h>
 #include <unistd.h>
 
@@ -277,6 +279,33 @@ static double now_s(void)
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+static char g_outdir[1024];
+
+void amr_set_output_dir(const char *dir)
+{
+    g_outdir[0] = '\0';
+    if (dir && dir[0]) snprintf(g_outdir, sizeof(g_outdir), "%s", dir);
+}
+
+/* vtk_write_solution / p4est_save of the reference evolve loop */
+static int write_output(amr_forest_t *f, int istep, double t, int visualize, int checkpoint)
+{
+    if (!g_outdir[0]) return 0;
+    char path[1200];
+    int rc = 0;
+    if (visualize) {
+        snprintf(path, sizeof(path), "%s/soln_%07d.vtu", g_outdir, istep);
+        if (amr_forest_write_vtk(f, path, 0)) rc = AMRGPU_ERR_ARG;
+    }
+    if (checkpoint) {
+        snprintf(path, sizeof(path), "%s/checkpoint", g_outdir);
+        mkdir(path, 0777);
+        snprintf(path, sizeof(path), "%s/checkpoint/amr_soln_%07d.bin", g_outdir, istep);
+        if (amr_forest_save(f, path, istep, t)) rc = AMRGPU_ERR_ARG;
+    }
+    return rc;
+}
+
 int amr_evolve(amr_forest_t *f, double tol, int allow_coarsening, amr_evolve_stats_t *st)
 {
     physics_gpu_params_t p;
@@ -312,6 +341,9 @@ int amr_evolve(amr_forest_t *f, double tol, int allow_coarsening, amr_evolve_sta
         int nst = istep - before;
         s.steps += nst;
         s.cell_stages += (double)nst * ncell * n * n * stages[p.time_scheme];
+        if (!evolve) visualize = checkpoint = 1;     /* end of the run */
+        rc = write_output(f, istep, physics_gpu_time(), visualize, checkpoint);
+        if (rc) break;
         if (regrid && evolve) {
             double t3 = now_s();
             amr_regrid(f, tol, allow_coarsening);
@@ -406,6 +438,11 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
         long long nsteps = (long long)p.time_steps - istep;
         long long r = steps_to(istep, o ? o->regrid_interval : p.regrid_interval);
         if (r < nsteps) nsteps = r;
+        if (g_outdir[0]) {        /* stop at the output intervals too */
+            long long v = steps_to(istep, p.visualization_interval), c = steps_to(istep, p.checkpoint_interval);
+            if (v < nsteps) nsteps = v;
+            if (c < nsteps) nsteps = c;
+        }
         if (nsteps <= 0) break;
         if (p.t_end > 0.0 && nsteps > 16) nsteps = 16;       /* poll the physical time */
         double t1 = now_s();
@@ -415,9 +452,19 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
         istep += (int)nsteps;
         s.steps += (int)nsteps;
         s.cell_stages += (double)nsteps * ncell * n * n * stages[p.time_scheme];
-        if (p.t_end > 0.0) amrgpu_get_time(g, &t, NULL);
+        if (p.t_end > 0.0 || g_outdir[0]) amrgpu_get_time(g, &t, NULL);
         s.wall_gpu += now_s() - t1;
-        if (istep >= p.time_steps || (p.t_end > 0.0 && t >= p.t_end)) break;
+        const int done = istep >= p.time_steps || (p.t_end > 0.0 && t >= p.t_end);
+        if (g_outdir[0] && !done) {
+            const int vis = p.visualization_interval > 0 && istep % p.visualization_interval == 0;
+            const int ckp = p.checkpoint_interval > 0 && istep % p.checkpoint_interval == 0;
+            if (vis || ckp) {
+                rc = amrgpu_download_soln(g, amr_forest_soln(f));
+                if (!rc) rc = write_output(f, istep, t, vis, ckp);
+                if (rc) break;
+            }
+        }
+        if (done) break;
         int interval = o ? o->regrid_interval : p.regrid_interval;
         if (interval <= 0 || istep % interval != 0) continue;
         /* regrid: tags on the device, topology on the host, data on the device */
@@ -447,6 +494,7 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
         amrgpu_get_time(g, &t, NULL);
         rc = amrgpu_download_soln(g, amr_forest_soln(f));
     }
+    if (!rc) rc = write_output(f, istep, t, 1, 1);      /* end of the run */
     s.t = t;
     amrgpu_destroy(g);
     free(ind);

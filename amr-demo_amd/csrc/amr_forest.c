@@ -520,3 +520,59 @@ int amr_forest_track_end(amr_forest_t *f)
     f->track_data = 1;
     return f->track_ok;
 }
+
+void amr_forest_brick(const amr_forest_t *f, int *ntx, int *nty, int *periodic_x, int *periodic_y,
+                      double *xlo, double *ylo, double *scale, int *min_level, int *max_level)
+{
+    if (ntx) *ntx = f->ntx;
+    if (nty) *nty = f->nty;
+    if (periodic_x) *periodic_x = f->periodic[0];
+    if (periodic_y) *periodic_y = f->periodic[1];
+    if (xlo) *xlo = f->xlo;
+    if (ylo) *ylo = f->ylo;
+    if (scale) *scale = f->scale;
+    if (min_level) *min_level = f->min_level;
+    if (max_level) *max_level = f->max_level;
+}
+
+amr_forest_t *amr_forest_from_quads(int ntx, int nty, int periodic_x, int periodic_y, double xlo,
+                                    double ylo, double scale, int n, int min_level, int max_level,
+                                    int nquad, const amr_quad_t *quads)
+{
+    if (nquad < ntx * nty || !quads) return NULL;
+    /* validate: order, alignment, and each tree tiled exactly once (the
+     * leaves of a complete quadtree in Morton order cover consecutive
+     * Morton ranges: each starts where the previous ended) */
+    const uint64_t full = (uint64_t)1 << (2 * AMR_MAXLEVEL);
+    int tree = 0;
+    uint64_t next = 0;
+    for (int k = 0; k < nquad; ++k) {
+        const amr_quad_t *q = &quads[k];
+        if (q->level < 0 || q->level > max_level || q->tree < 0 || q->tree >= ntx * nty) return NULL;
+        const int len = qlen(q->level);
+        if (q->x < 0 || q->y < 0 || q->x % len || q->y % len || q->x >= AMR_ROOT_LEN || q->y >= AMR_ROOT_LEN)
+            return NULL;
+        if (q->tree != tree) {
+            if (q->tree != tree + 1 || next != full) return NULL;
+            tree = q->tree;
+            next = 0;
+        }
+        if (morton(q->x, q->y) != next) return NULL;
+        next += (uint64_t)len * (uint64_t)len;
+    }
+    if (tree != ntx * nty - 1 || next != full) return NULL;
+    amr_forest_t *f = amr_forest_new(ntx, nty, periodic_x, periodic_y, xlo, ylo, scale, n, 0, max_level);
+    if (!f) return NULL;
+    f->min_level = min_level;
+    reserve(f, nquad);
+    for (int k = 0; k < nquad; ++k) {
+        f->q[k] = quads[k];
+        f->key[k] = morton(quads[k].x, quads[k].y);
+        f->tag[k] = 0;
+        f->type[k] = 0;
+    }
+    f->nquad = nquad;
+    f->hvalid = 0;
+    memset(f->soln, 0, sizeof(double) * (size_t)patch_size(f) * (size_t)nquad);
+    return f;
+}

@@ -68,6 +68,14 @@ int amr_regrid_tags(amr_forest_t *f, const int *tags, int allow_coarsening);
  * (initialize_quadrant_data), then balance. */
 int amr_regrid_initial(amr_forest_t *f, const amr_ic_t *ic, double tol);
 
+/* Output of the evolve loops (as src/amr/src/amr_evolve.c:45-66): when a
+ * directory is set, the visualization and checkpoint flags of the interval
+ * settings (and the end of the run) write <dir>/soln_<step>.vtu
+ * (amr_forest_write_vtk, per quadrant as the reference) and
+ * <dir>/checkpoint/amr_soln_<step>.bin (amr_forest_save).  NULL or "" (the
+ * default) writes nothing. */
+void amr_set_output_dir(const char *dir);
+
 /* Statistics of an adaptive run. */
 typedef struct {
     int    steps;            /* time steps taken */

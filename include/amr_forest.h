@@ -90,6 +90,32 @@ int amr_forest_track_end(amr_forest_t *f);
 /* Leaf index containing the point (tree, x, y) or -1. */
 int amr_forest_find_leaf(const amr_forest_t *f, int tree, int x, int y);
 
+/* Construction parameters of the forest (any pointer may be NULL). */
+void amr_forest_brick(const amr_forest_t *f, int *ntx, int *nty, int *periodic_x, int *periodic_y,
+                      double *xlo, double *ylo, double *scale, int *min_level, int *max_level);
+/* Forest with the given leaves (tree-major Morton order, each tree tiled
+ * exactly once, levels within [0, max_level]); patches zero, tags 0.  NULL if
+ * the quadrants do not form such a forest. */
+amr_forest_t *amr_forest_from_quads(int ntx, int nty, int periodic_x, int periodic_y, double xlo,
+                                    double ylo, double scale, int n, int min_level, int max_level,
+                                    int nquad, const amr_quad_t *quads);
+
+/* ---- output in this project's formats (amr_io.c) ----------------------
+ * The reference checkpoints with p4est_save and visualizes through p4est's
+ * VTK writer (src/amr/src/amr_evolve.c:55-64, src/amr/src/amr_vtk.c:23-106);
+ * without p4est both are self-contained files. */
+
+/* Binary checkpoint (header with brick, patch size, levels, step and time;
+ * quadrants; patches in the reference layout).  0 on success. */
+int amr_forest_save(amr_forest_t *f, const char *path, int istep, double t);
+/* Forest restored from amr_forest_save's file (NULL on a bad file). */
+amr_forest_t *amr_forest_load(const char *path, int *istep, double *t);
+/* XML VTK unstructured grid (.vtu, raw appended binary).  subcells = 0: one
+ * cell per quadrant with tree and level, and the patch-averaged rho, rhou,
+ * rhov, rhoe at its corners (the reference's fields); subcells = 1: one cell
+ * per sub-grid cell with its state.  0 on success. */
+int amr_forest_write_vtk(amr_forest_t *f, const char *path, int subcells);
+
 #ifdef __cplusplus
 }
 #endif

@@ -270,3 +270,64 @@ def test_device_evolve_equals_host_regrid_loop(allow_coarsening):
     assert sa.steps == sb.steps == 60 and sa.regrids == sb.regrids
     assert fa.quads() == fb.quads()
     assert np.array_equal(fa.soln(), fb.soln())
+
+
+def vortex_l1(f, soln, t):
+    """L1 density error per unit area against the exactly advected vortex
+    (amr_ic_vortex_density at the sub-cell centres, nearest periodic image)"""
+    ic = amrhip.vortex_ic(5.0, 5.0, 5.0)
+    L = amrhip.lib()
+    n = f.n
+    _, _, geom, _ = f.external()
+    q = np.asarray(soln).reshape(f.nquad, n, n, 4)
+    err, area = 0.0, 0.0
+    for k in range(f.nquad):
+        g = geom[12 * k:12 * k + 12].reshape(4, 3)
+        x0, y0, h = g[0, 0], g[0, 1], (g[1, 0] - g[0, 0]) / n
+        for j in range(n):
+            for i in range(n):
+                ex = L.amr_ic_vortex_density(C_IC(ic), x0 + (i + 0.5) * h, y0 + (j + 0.5) * h, t,
+                                             0.0, 10.0, 0.0, 10.0)
+                err += abs(q[k, j, i, 0] - ex) * h * h
+                area += h * h
+    return err / area
+
+
+def C_IC(ic):
+    import ctypes
+    return ctypes.byref(ic)
+
+
+def test_config1_vortex_200_steps_l1(gpu, oracle):
+    """config 1: 200 SSP-RK3 steps of the strength-5 vortex on the uniform
+    level-4 forest (128^2 cells, CFL 0.4 input = 0.3 effective): the GPU
+    trajectory matches the oracle's, and both have the same L1 density error
+    against the exactly advected vortex; halving h (level 5, same physical
+    time) lowers it as a first-order scheme must"""
+    log("config1 200 steps")
+    f = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 10.0, 8, 4, 4)
+    f.apply_ic(amrhip.vortex_ic(5.0, 5.0, 5.0))
+    ci, fi, geom, soln = f.external()
+    setup(gpu, 8, 3, 10.0, (0, 0, 0, 0), 0.3, 1.0, ci, fi, geom, soln)
+    gpu.run(200)
+    t4 = float(gpu.time()[0][0])
+    q_gpu = gpu.soln()
+    p = oracle.params(8, cfl=0.3, h_base=10.0)
+    q_orc, _, _ = oracle.run(p, ci, fi, geom, soln, 200, threads=8)
+    assert parity_err(q_gpu, q_orc) <= TRAJ_TOL
+    e_gpu = vortex_l1(f, q_gpu, t4)
+    e_orc = vortex_l1(f, q_orc, t4)
+    log(f"L1 level 4: gpu {e_gpu:.6e} oracle {e_orc:.6e} at t={t4:.4f}")
+    assert abs(e_gpu - e_orc) <= 1e-9 * e_orc
+    assert 1e-5 < e_gpu < 5e-2
+    # level 5 to the same physical time
+    f5 = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 10.0, 8, 5, 5)
+    f5.apply_ic(amrhip.vortex_ic(5.0, 5.0, 5.0))
+    ci5, fi5, g5, s5 = f5.external()
+    setup(gpu, 8, 3, 10.0, (0, 0, 0, 0), 0.3, 1.0, ci5, fi5, g5, s5, t_end=t4)
+    while gpu.time()[0][0] < t4:
+        gpu.run(50)
+    assert gpu.time()[0][0] == pytest.approx(t4, abs=1e-14)
+    e5 = vortex_l1(f5, gpu.soln(), t4)
+    log(f"L1 level 5: {e5:.6e} (ratio {e_gpu / e5:.2f})")
+    assert e5 < 0.75 * e_gpu
