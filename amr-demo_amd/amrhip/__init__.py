@@ -89,6 +89,7 @@ EXPORTS = {
     "amr_forest_track_end": (_i, [_vp]),
     "amrgpu_indicator": (_i, [_vp, _pd]),
     "amrgpu_remesh": (_i, [_vp, _i, _pi, _pi, _i, _pi, _pi]),
+    "amrgpu_set_geometry": (_i, [_vp, _pd]),
     "amrgpu_device_state": (_i, [_vp, _vp, _vp, _pi, _pi, _vp]),
     # include/amr_tagger.h
     "amr_mlp_create": (_vp, [C.c_ulonglong]),
@@ -117,6 +118,15 @@ MLP_NPARAM = 64 * 384 + 64 + 64 * 64 + 64 + 3 * 64 + 3
 TAG_COARSEN, TAG_KEEP, TAG_REFINE = 0, 1, 2
 
 
+class InstParams(C.Structure):
+    """amrgpu_inst_t"""
+    _fields_ = [("h_base", _d), ("bc", _i * 4), ("cfl", _d), ("dt_cap", _d), ("t_end", _d),
+                ("bstate", (_d * 4) * 2), ("shock_x0", _d), ("shock_angle", _d), ("shock_speed", _d)]
+
+
+BC_COPY, BC_SLIP, BC_NOSLIP, BC_INFLOW, BC_SHOCK_WALL, BC_SHOCK_TOP = 0, 1, 2, 3, 4, 5
+
+
 class EvolveStats(C.Structure):
     """amr_evolve_stats_t"""
     _fields_ = [("steps", _i), ("regrids", _i), ("peak_quads", _i), ("cell_stages", _d), ("t", _d),
@@ -142,7 +152,8 @@ class MlpTaggerStruct(C.Structure):
 class EvolveOpts(C.Structure):
     """amr_evolve_opts_t"""
     _fields_ = [("tol", _d), ("allow_coarsening", _i), ("regrid_interval", _i), ("tagger", _vp),
-                ("tagger_ctx", _vp), ("tile_ppb", _i), ("tile_minw", _i), ("tile_design", _i)]
+                ("tagger_ctx", _vp), ("tile_ppb", _i), ("tile_minw", _i), ("tile_design", _i),
+                ("inst", C.POINTER(InstParams))]
 
 
 class PhysicsParams(C.Structure):
@@ -154,11 +165,6 @@ class PhysicsParams(C.Structure):
         ("checkpoint_interval", _i), ("visualization_interval", _i), ("regrid_interval", _i),
         ("bc", _i * 6), ("cfl", _d), ("coarse_grid_h", _d), ("t_end", _d), ("verbose", _i),
     ]
-
-
-class InstParams(C.Structure):
-    """amrgpu_inst_t"""
-    _fields_ = [("h_base", _d), ("bc", _i * 4), ("cfl", _d), ("dt_cap", _d), ("t_end", _d)]
 
 
 class Quad(C.Structure):
@@ -370,7 +376,7 @@ class Forest:
         return st
 
     def evolve_device(self, tol, regrid_interval, allow_coarsening=True, tile=None, tagger=None,
-                      tagger_ctx=None):
+                      tagger_ctx=None, inst=None):
         """device-resident evolve loop; tagger: optional TAGGER_FN or MlpTagger"""
         o = EvolveOpts()
         o.tol, o.allow_coarsening, o.regrid_interval = tol, int(allow_coarsening), regrid_interval
@@ -382,6 +388,8 @@ class Forest:
             o.tagger_ctx = tagger_ctx
         if tile:
             o.tile_ppb, o.tile_minw, o.tile_design = tile
+        if inst is not None:
+            o.inst = C.pointer(inst)
         st = EvolveStats()
         rc = self.L.amr_evolve_device(self.h, C.byref(o), C.byref(st))
         check(rc, "amr_evolve_device")
@@ -460,13 +468,44 @@ def dmr_ic(gamma=1.4):
 # ---------------------------------------------------------------------------
 # device context
 # ---------------------------------------------------------------------------
-def make_inst(h_base, bc=(0, 0, 0, 0), cfl=0.3, dt_cap=1.0, t_end=0.0):
+def make_inst(h_base, bc=(0, 0, 0, 0), cfl=0.3, dt_cap=1.0, t_end=0.0, bstate=None, shock=None):
     ip = InstParams()
     ip.h_base = h_base
     for s in range(4):
         ip.bc[s] = int(bc[s])
     ip.cfl, ip.dt_cap, ip.t_end = cfl, dt_cap, t_end
+    if bstate is not None:
+        for a in range(2):
+            for f in range(4):
+                ip.bstate[a][f] = float(bstate[a][f])
+    if shock is not None:
+        ip.shock_x0, ip.shock_angle, ip.shock_speed = (float(v) for v in shock)
     return ip
+
+
+def dmr_states(gamma=1.4, mach=10.0, rho1=1.4, p1=1.0, angle=60.0):
+    """post- and pre-shock conservative states of the double-Mach reflection
+    (Rankine-Hugoniot for a shock of Mach `mach` into gas at rest, moving
+    along the normal of a line at `angle` degrees to the x axis), its speed"""
+    th = np.radians(angle)
+    c1 = np.sqrt(gamma * p1 / rho1)
+    w = mach * c1
+    rho2 = rho1 * (gamma + 1.0) * mach * mach / ((gamma - 1.0) * mach * mach + 2.0)
+    p2 = p1 * (2.0 * gamma * mach * mach - (gamma - 1.0)) / (gamma + 1.0)
+    un = w * (1.0 - rho1 / rho2)
+    u2, v2 = un * np.sin(th), -un * np.cos(th)
+    post = (rho2, rho2 * u2, rho2 * v2, p2 / (gamma - 1.0) + 0.5 * rho2 * (u2 * u2 + v2 * v2))
+    pre = (rho1, 0.0, 0.0, p1 / (gamma - 1.0))
+    return post, pre, w
+
+
+def dmr_inst(h_base=1.0, cfl=0.3, dt_cap=1.0, t_end=0.0, x0=1.0 / 6.0, angle=60.0):
+    """walls of the double-Mach reflection: post-shock inflow on the left,
+    outflow on the right, post-shock then reflecting wall (from x0) along the
+    bottom, the moving shock along the top"""
+    post, pre, w = dmr_states(angle=angle)
+    return make_inst(h_base, (BC_INFLOW, BC_COPY, BC_SHOCK_WALL, BC_SHOCK_TOP), cfl, dt_cap, t_end,
+                     bstate=(post, pre), shock=(x0, angle, w))
 
 
 class Gpu:

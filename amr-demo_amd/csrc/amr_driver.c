@@ -365,7 +365,7 @@ static long long steps_to(int istep, int interval)
     return (long long)interval - (istep % interval);
 }
 
-int amr_device_setup(amr_forest_t *f, amrgpu_ctx **out)
+int amr_device_setup(amr_forest_t *f, const amrgpu_inst_t *override, amrgpu_ctx **out)
 {
     physics_gpu_params_t p;
     physics_gpu_get_params(&p);
@@ -378,11 +378,13 @@ int amr_device_setup(amr_forest_t *f, amrgpu_ctx **out)
     amrgpu_ctx *g = amrgpu_create();
     if (!g) return AMRGPU_ERR_HIP;
     amrgpu_inst_t ip;
+    memset(&ip, 0, sizeof(ip));
     ip.h_base = p.coarse_grid_h;
     for (int k = 0; k < 4; ++k) ip.bc[k] = p.bc[k];
     ip.cfl = p.cfl;
     ip.dt_cap = p.dt;
     ip.t_end = p.t_end;
+    if (override) ip = *override;
     int rc = amrgpu_setup(g, amr_forest_nelem_subgrid(f), p.time_scheme, 1, &ip, ncell, cell_info, NULL, nface,
                           face_info, geom, soln);
     if (rc) {
@@ -409,6 +411,7 @@ int amr_device_regrid(amr_forest_t *f, amrgpu_ctx *g, const int *tags, int allow
         return AMRGPU_ERR_MESH;
     double t1 = now_s();
     int rc = amrgpu_remesh(g, ncell, cell_info, NULL, nface, face_info, amr_forest_track_src(f));
+    if (!rc) rc = amrgpu_set_geometry(g, geom);
     if (t_forest) *t_forest += t1 - t0;
     if (t_remesh) *t_remesh += now_s() - t1;
     return rc;
@@ -426,7 +429,7 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
     int ncell = amr_forest_num_quads(f), rc = 0;
     double t0 = now_s();
     amrgpu_ctx *g = NULL;
-    rc = amr_device_setup(f, &g);
+    rc = amr_device_setup(f, o ? o->inst : NULL, &g);
     if (rc) return rc;
     if (o && o->tile_ppb > 0) rc = amrgpu_set_tile(g, o->tile_ppb, o->tile_minw, o->tile_design);
     double *ind = NULL;
@@ -772,6 +775,7 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
         double t4 = now_s();
         tot.wall_forest += t4 - t3;
         if (!rc) rc = amrgpu_remesh(g, ncell, b.ci, b.cinst, nface, b.fi, b.src);
+        if (!rc) rc = amrgpu_set_geometry(g, b.geom);
         tot.wall_remesh += now_s() - t4;
         tot.regrids++;
         if (ncell > tot.peak_quads) tot.peak_quads = ncell;

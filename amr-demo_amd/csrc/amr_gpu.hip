@@ -43,6 +43,8 @@ struct InstDev {
     double cfl;
     double dt_cap;
     double t_end;
+    double bstate[2][4];              // boundary states of wall types 3-5
+    double shock_x0, shock_cot, shock_vx;   // x_s(y,t) = x0 + y cot(a) + vx t
 };
 
 // Per-quadrant face table entry for side s (0 xlo, 1 xhi, 2 ylo, 3 yhi):
@@ -57,6 +59,8 @@ struct MeshDev {
     const int *level;    // nquad
     const int *inst;     // nquad
     const InstDev *ip;   // ninst
+    const double *org;   // nquad*2 lower-left corner (position-dependent walls)
+    const double *t_step;// ninst: instance time at the start of the step
 };
 
 enum { MODE_SO = 0, MODE_RK4_ACC = 1, MODE_RK4_FINAL = 2, MODE_LSERK = 3, MODE_RHS = 4 };
@@ -75,6 +79,7 @@ struct StageArgs {
     int use_cfl;         // stage-1 dt includes the CFL slot
     double *dt_cur;
     double *t_inst;
+    double *t_step;      // written at the step's first stage (equal values)
     unsigned long long *slot;
     double *norm_part;
 };
@@ -196,9 +201,35 @@ __device__ __forceinline__ void flux_face(const Prim &L, const Prim &R, double h
 
 // Physical boundary flux: bc_2d (bc_2d_module.F90:14-75) then flux_bc_2d
 // (flux_2d_module.F90:13-60) with outward normal nrm = (+-h, 0) or (0, +-h),
-// operation for operation.
-__device__ __forceinline__ void flux_boundary(int bc, const double nrm[2], const double q[4], double F[4])
+// operation for operation.  Types 3-5 (project additions) take a prescribed
+// state that depends on the face centre (org + (fx, fy) h) and time t.
+__device__ __forceinline__ void flux_boundary(const InstDev &ip, int side, const double nrm[2], const double q[4],
+                                              const double *org, double ofx, double ofy, double h, double t,
+                                              double F[4])
 {
+    int bc = ip.bc[side];
+    if (bc >= 3) {
+        const double *st = ip.bstate[0];
+        if (bc >= 4) {
+            const double xf = org[0] + ofx * h, yf = org[1] + ofy * h;
+            if (bc == 4) {
+                if (!(xf < ip.shock_x0)) bc = 1;             // slip wall beyond x0
+            } else if (!(xf < ip.shock_x0 + yf * ip.shock_cot + ip.shock_vx * t)) {
+                st = ip.bstate[1];                           // ahead of the shock
+            }
+        }
+        if (bc != 1) {
+            const double r = st[0];
+            const double oneOr = 1.0 / r;
+            const double ub = st[1] * oneOr, vb = st[2] * oneOr;
+            const double p = (1.4 - 1.0) * (st[3] - 0.5 * r * (ub * ub + vb * vb));
+            const double fx4[4] = {st[1], st[1] * ub + p, st[1] * vb, ub * (st[3] + p)};
+            const double fy4[4] = {st[2], st[2] * ub, st[2] * vb + p, vb * (st[3] + p)};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) F[k] = fx4[k] * nrm[0] + fy4[k] * nrm[1];
+            return;
+        }
+    }
     double rho = q[0];
     double oneOrho = 1.0 / rho;
     double u = q[1] * oneOrho;
@@ -403,6 +434,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     __shared__ int pquad[PPB], plevel[PPB], pinst[PPB];
     __shared__ int pnbr[PPB][8];
     __shared__ double pdt[PPB];
+    __shared__ double ptime[PPB];
 
     const int tid = threadIdx.x;
     const int nblk = gridDim.x;
@@ -435,10 +467,20 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             if (in != last_inst) {
                 last_dt = step_dt(a.m.ip[in], a.slot, a.t_inst, in, a.use_cfl);
                 last_inst = in;
-                if ((tid & 63) == 0) a.dt_cur[in] = last_dt;   // identical values from every writer
+                if ((tid & 63) == 0) {                          // identical values from every writer
+                    a.dt_cur[in] = last_dt;
+                    a.t_step[in] = a.t_inst[in];
+                }
             }
-            if ((tid & 63) == 0) pdt[q] = last_dt;
+            if ((tid & 63) == 0) {
+                pdt[q] = last_dt;
+                ptime[q] = a.t_inst[in];
+            }
         }
+    } else if (tid < PPB) {
+        // later stages: the step's start time, stable while the time advances
+        const int qq = lb * PPB + tid;
+        ptime[tid] = qq < a.m.nquad ? a.m.t_step[a.m.inst[qq]] : 0.0;
     }
     double U[4] = {1.0, 0.0, 0.0, 1.0};
     double B[4] = {0.0, 0.0, 0.0, 0.0};     // base state U0 of the step
@@ -504,7 +546,8 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 if (na1 < 0) {
                     const double nrm[2] = {h, 0.0};
                     const double q[4] = {L.r, L.ru, L.rv, L.rE};
-                    flux_boundary(a.m.ip[pinst[p]].bc[1], nrm, q, FA);
+                    flux_boundary(a.m.ip[pinst[p]], 1, nrm, q, a.m.org + 2 * (size_t)quad, (double)N,
+                                  cj + 0.5, h, ptime[p], FA);
                 } else {
                     Prim F2;
                     load_prim<N, PPB>(lds, p, TL::TT + 1 * N + cj, F2);
@@ -517,7 +560,8 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 if (na3 < 0) {
                     const double nrm[2] = {0.0, h};
                     const double q[4] = {L.r, L.ru, L.rv, L.rE};
-                    flux_boundary(a.m.ip[pinst[p]].bc[3], nrm, q, FB);
+                    flux_boundary(a.m.ip[pinst[p]], 3, nrm, q, a.m.org + 2 * (size_t)quad, ci + 0.5,
+                                  (double)N, h, ptime[p], FB);
                 } else {
                     Prim F2;
                     load_prim<N, PPB>(lds, p, TL::TT + 3 * N + ci, F2);
@@ -545,7 +589,8 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 if (na < 0) {
                     const double nrm[2] = {xs ? -h : 0.0, xs ? 0.0 : -h};
                     const double q[4] = {R.r, R.ru, R.rv, R.rE};
-                    flux_boundary(a.m.ip[pinst[tp]].bc[side], nrm, q, FC);
+                    flux_boundary(a.m.ip[pinst[tp]], side, nrm, q, a.m.org + 2 * (size_t)pquad[tp],
+                                  xs ? 0.0 : k + 0.5, xs ? k + 0.5 : 0.0, h, ptime[tp], FC);
                     FC[0] = -FC[0]; FC[1] = -FC[1]; FC[2] = -FC[2]; FC[3] = -FC[3];
                 } else if (nb >= 0) {
                     Prim F2;
@@ -627,13 +672,15 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 if (na < 0) {
                     // physical boundary: outward normal; the face value is stored
                     // with the +x/+y orientation of the interior fluxes
-                    int bc = a.m.ip[pinst[tp]].bc[side];
                     double nrm[2] = {0.0, 0.0};
                     if (side == 0) nrm[0] = -h; else if (side == 1) nrm[0] = h;
                     else if (side == 2) nrm[1] = -h; else nrm[1] = h;
                     const double *b = lds + tp * TL::PRIM_PER_PATCH + town;
                     double q[4] = {b[0], b[TL::NPOS], b[2 * TL::NPOS], b[3 * TL::NPOS]};
-                    flux_boundary(bc, nrm, q, F);
+                    const double fxo = side == 0 ? 0.0 : (side == 1 ? (double)N : fi + 0.5);
+                    const double fyo = side == 2 ? 0.0 : (side == 3 ? (double)N : fj + 0.5);
+                    flux_boundary(a.m.ip[pinst[tp]], side, nrm, q, a.m.org + 2 * (size_t)pquad[tp], fxo, fyo, h,
+                                  ptime[tp], F);
                     if (lowside) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; F[3] = -F[3]; }
                 } else if (nb < 0) {
                     Prim L, R;
@@ -819,7 +866,7 @@ __global__ void slot_reset_kernel(unsigned long long *slot, int ninst)
 
 // RK1 has a single stage: the step's dt and time update run in this tiny
 // kernel ahead of it.
-__global__ void dt_prep_kernel(MeshDev m, int use_cfl, double *dt_cur, double *t_inst,
+__global__ void dt_prep_kernel(MeshDev m, int use_cfl, double *dt_cur, double *t_inst, double *t_step,
                                unsigned long long *slot)
 {
     // one wave
@@ -827,6 +874,7 @@ __global__ void dt_prep_kernel(MeshDev m, int use_cfl, double *dt_cur, double *t
         double dt = step_dt(m.ip[i], slot, t_inst, i, use_cfl);
         if (threadIdx.x == 0) {
             dt_cur[i] = dt;
+            t_step[i] = t_inst[i];
             t_inst[i] += dt;
         }
         for (int k = threadIdx.x; k < NSLOT; k += 64) slot[(size_t)i * NSLOT + k] = DBL_INF_BITS;
@@ -986,6 +1034,9 @@ struct amrgpu_ctx {
     double *d_stage = nullptr;      // AoS staging for up/download
     int *d_map = nullptr;           // regrid source map
     double *d_ind = nullptr;        // per-quadrant indicator
+    double *d_org = nullptr;        // per-quadrant lower-left corner
+    double *d_tstep = nullptr;      // per-instance time at the step start
+    bool org_ok = false, need_org = false;
     int nblocks = 0;
     int variant = -1;
     int cap_quad = 0, cap_inst = 0, cap_blocks = 0;
@@ -1003,6 +1054,7 @@ MeshDev mesh_of(const amrgpu_ctx *c)
     MeshDev m;
     m.nquad = c->nquad; m.ninst = c->ninst;
     m.nbr = c->d_nbr; m.level = c->d_level; m.inst = c->d_inst; m.ip = c->d_ip;
+    m.org = c->d_org; m.t_step = c->d_tstep;
     return m;
 }
 
@@ -1112,6 +1164,7 @@ StageArgs base_args(amrgpu_ctx *c)
     a.m = mesh_of(c);
     a.dt_cur = c->d_dt_cur;
     a.t_inst = c->d_t;
+    a.t_step = c->d_tstep;
     a.slot = c->d_slot;
     a.use_cfl = (c->scheme != 2);
     return a;
@@ -1187,7 +1240,7 @@ int enqueue_step(amrgpu_ctx *c, int cur, bool with_norm, int *next)
     if (nx < 0) { set_err("bad time scheme %d", c->scheme); return AMRGPU_ERR_ARG; }
     if (c->scheme == 1) {
         hipLaunchKernelGGL(dt_prep_kernel, dim3(1), dim3(64), 0, c->stream, mesh_of(c), 1,
-                           c->d_dt_cur, c->d_t, c->d_slot);
+                           c->d_dt_cur, c->d_t, c->d_tstep, c->d_slot);
         HIPCHK(hipGetLastError());
     }
     for (auto &a : st) {
@@ -1211,8 +1264,9 @@ void free_mesh(amrgpu_ctx *c)
     for (int i = 0; i < 3; ++i) (void)hipFree(c->U[i]);
     (void)hipFree(c->Acc); (void)hipFree(c->d_dt_cur); (void)hipFree(c->d_t); (void)hipFree(c->d_slot);
     (void)hipFree(c->d_norm); (void)hipFree(c->d_stage);
-    (void)hipFree(c->d_map); (void)hipFree(c->d_ind);
-    c->d_map = nullptr; c->d_ind = nullptr;
+    (void)hipFree(c->d_map); (void)hipFree(c->d_ind); (void)hipFree(c->d_org); (void)hipFree(c->d_tstep);
+    c->d_map = nullptr; c->d_ind = nullptr; c->d_org = nullptr; c->d_tstep = nullptr;
+    c->org_ok = false;
     c->d_nbr = c->d_level = c->d_inst = nullptr; c->d_ip = nullptr;
     c->U[0] = c->U[1] = c->U[2] = nullptr; c->Acc = nullptr;
     c->d_dt_cur = c->d_t = nullptr; c->d_slot = nullptr; c->d_norm = nullptr;
@@ -1363,6 +1417,8 @@ static int ensure_capacity(amrgpu_ctx *c, int n, int ncell, int ninst, bool need
     HIPCHK(hipMalloc(&c->d_stage, sizeof(double) * cs));
     HIPCHK(hipMalloc(&c->d_map, sizeof(int) * cq));
     HIPCHK(hipMalloc(&c->d_ind, sizeof(double) * cq));
+    HIPCHK(hipMalloc(&c->d_org, sizeof(double) * 2 * cq));
+    HIPCHK(hipMalloc(&c->d_tstep, sizeof(double) * c->cap_inst));
     return AMRGPU_OK;
 }
 
@@ -1394,7 +1450,6 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
                  const int *cell_inst, int nface, const int *face_info, const double *geom,
                  const double *soln)
 {
-    (void)geom;
     if (!c || !inst || !cell_info || !face_info || ncell <= 0 || ninst <= 0 || nface < 0) {
         set_err("amrgpu_setup: bad arguments");
         return AMRGPU_ERR_ARG;
@@ -1407,6 +1462,12 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
         set_err("amrgpu_setup: time_scheme %d", time_scheme);
         return AMRGPU_ERR_ARG;
     }
+    for (int i = 0; i < ninst; ++i)
+        for (int s = 0; s < 4; ++s)
+            if (inst[i].bc[s] < 0 || inst[i].bc[s] > AMRGPU_BC_SHOCK_TOP) {
+                set_err("amrgpu_setup: instance %d side %d wall type %d", i, s, inst[i].bc[s]);
+                return AMRGPU_ERR_ARG;
+            }
     const int n2 = nelem_subgrid * nelem_subgrid;
     int rc = validate_cells(n2, ninst, ncell, cell_info, cell_inst, "amrgpu_setup");
     if (rc) return rc;
@@ -1422,20 +1483,51 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
     c->n = nelem_subgrid; c->ninst = ninst; c->scheme = time_scheme;
     c->cur = 0;
     c->h_ip.resize(ninst);
+    c->need_org = false;
     for (int i = 0; i < ninst; ++i) {
-        c->h_ip[i].h_base = inst[i].h_base;
-        for (int s = 0; s < 4; ++s) c->h_ip[i].bc[s] = inst[i].bc[s];
-        c->h_ip[i].cfl = inst[i].cfl;
-        c->h_ip[i].dt_cap = inst[i].dt_cap;
-        c->h_ip[i].t_end = inst[i].t_end;
+        InstDev &d = c->h_ip[i];
+        memset(&d, 0, sizeof(d));
+        d.h_base = inst[i].h_base;
+        for (int s = 0; s < 4; ++s) {
+            d.bc[s] = inst[i].bc[s];
+            if (d.bc[s] >= AMRGPU_BC_SHOCK_WALL) c->need_org = true;
+        }
+        d.cfl = inst[i].cfl;
+        d.dt_cap = inst[i].dt_cap;
+        d.t_end = inst[i].t_end;
+        memcpy(d.bstate, inst[i].bstate, sizeof(d.bstate));
+        const double a = inst[i].shock_angle * 3.141592653589793238462643383279502884197 / 180.0;
+        d.shock_x0 = inst[i].shock_x0;
+        d.shock_cot = a != 0.0 ? cos(a) / sin(a) : 0.0;
+        d.shock_vx = a != 0.0 ? inst[i].shock_speed / sin(a) : 0.0;
     }
     HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice, c->stream));
     std::vector<double> zeros(ninst, 0.0);
     HIPCHK(hipMemcpyAsync(c->d_t, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_tstep, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_dt_cur, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
     rc = upload_mesh(c, ncell, cell_info, cell_inst, nbr);
     if (rc) return rc;
+    c->org_ok = false;
+    if (geom) {
+        rc = amrgpu_set_geometry(c, geom);
+        if (rc) return rc;
+    }
     if (soln) return amrgpu_upload_soln(c, soln);
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_geometry(amrgpu_ctx *c, const double *geom)
+{
+    if (!c || !c->nquad || !geom) { set_err("amrgpu_set_geometry: no mesh"); return AMRGPU_ERR_ARG; }
+    std::vector<double> org(2 * (size_t)c->nquad);
+    for (int q = 0; q < c->nquad; ++q) {
+        org[2 * (size_t)q] = geom[12 * (size_t)q];          // corner 0: (x, y) of the lower-left node
+        org[2 * (size_t)q + 1] = geom[12 * (size_t)q + 1];
+    }
+    HIPCHK(hipMemcpyAsync(c->d_org, org.data(), sizeof(double) * org.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->org_ok = true;
     return AMRGPU_OK;
 }
 
@@ -1452,6 +1544,7 @@ int amrgpu_indicator(amrgpu_ctx *c, double *ind)
 int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst, int nface,
                   const int *face_info, const int *src)
 {
+    if (c) c->org_ok = false;          // the new quadrants' coordinates come with amrgpu_set_geometry
     if (!c || !c->nquad || !cell_info || !face_info || !src || ncell <= 0 || nface < 0) {
         set_err("amrgpu_remesh: bad arguments");
         return AMRGPU_ERR_ARG;
@@ -1539,9 +1632,20 @@ int amrgpu_download_soln(amrgpu_ctx *c, double *soln)
     return AMRGPU_OK;
 }
 
+// position-dependent walls need the quadrant coordinates of the current mesh
+static int geometry_ready(amrgpu_ctx *c)
+{
+    if (c->need_org && !c->org_ok) {
+        set_err("wall types 4-5 need amrgpu_set_geometry after a remesh");
+        return AMRGPU_ERR_MESH;
+    }
+    return AMRGPU_OK;
+}
+
 int amrgpu_rhs(amrgpu_ctx *c, double *rhs)
 {
     if (!c || !rhs || !c->nquad) { set_err("amrgpu_rhs: no mesh"); return AMRGPU_ERR_ARG; }
+    if (int rc = geometry_ready(c)) return rc;
     StageArgs a = base_args(c);
     int other = (c->cur + 1) % 3;
     a.Us = c->U[c->cur]; a.Uout = c->U[other]; a.mode = MODE_RHS; a.role = 0;
@@ -1577,6 +1681,7 @@ int amrgpu_cfl(amrgpu_ctx *c, double *dt_inst)
 
 static int prime_run(amrgpu_ctx *c)
 {
+    if (int rc = geometry_ready(c)) return rc;
     // CFL slot of the current state for the first step's dt
     hipLaunchKernelGGL(slot_reset_kernel, dim3(1), dim3(256), 0, c->stream, c->d_slot, c->ninst);
     HIPCHK(hipGetLastError());
@@ -1666,6 +1771,7 @@ int amrgpu_set_time(amrgpu_ctx *c, const double *t_inst)
 {
     if (!c || !c->nquad || !t_inst) { set_err("amrgpu_set_time: no mesh"); return AMRGPU_ERR_ARG; }
     HIPCHK(hipMemcpyAsync(c->d_t, t_inst, sizeof(double) * c->ninst, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_tstep, t_inst, sizeof(double) * c->ninst, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return AMRGPU_OK;
 }
@@ -1710,7 +1816,7 @@ int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_
         int nx = build_step(c, c->cur, st, false);
         if (c->scheme == 1) {
             hipLaunchKernelGGL(dt_prep_kernel, dim3(1), dim3(64), 0, c->stream, mesh_of(c), 1,
-                               c->d_dt_cur, c->d_t, c->d_slot);
+                               c->d_dt_cur, c->d_t, c->d_tstep, c->d_slot);
         }
         HIPCHK(hipEventRecord(ev[(size_t)s * (ns + 1)], c->stream));
         for (int k = 0; k < ns; ++k) {

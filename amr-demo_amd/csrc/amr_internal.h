@@ -5,6 +5,8 @@
 #ifndef AMR_INTERNAL_H
 #define AMR_INTERNAL_H
 
+#include "amr_hip.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -18,8 +20,9 @@ struct amr_forest;
 AMR_HIDDEN void amr_set_error(const char *fmt, ...);
 
 /* Device context for the whole forest with the current physics settings
- * (one instance; h_base = coarse_grid_h, walls bc[0..3]). */
-AMR_HIDDEN int amr_device_setup(struct amr_forest *f, struct amrgpu_ctx **out);
+ * (one instance; h_base = coarse_grid_h, walls bc[0..3]), or with the given
+ * instance settings when override is not NULL. */
+AMR_HIDDEN int amr_device_setup(struct amr_forest *f, const amrgpu_inst_t *override, struct amrgpu_ctx **out);
 /* Regrid the forest by tags (amr_regrid_tags) and remap the device state;
  * wall seconds of the host forest part and of the device remesh are added to
  * *t_forest and *t_remesh (either may be NULL). */

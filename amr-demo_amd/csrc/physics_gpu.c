@@ -144,6 +144,7 @@ static int check_structured(const int *dim, const int *nfields)
 static amrgpu_inst_t inst_from_params(void)
 {
     amrgpu_inst_t ip;
+    memset(&ip, 0, sizeof(ip));
     ip.h_base = g_p.coarse_grid_h;
     for (int s = 0; s < 4; ++s) ip.bc[s] = g_p.bc[s];
     ip.cfl = g_p.cfl;

@@ -111,6 +111,8 @@ typedef struct {
     amr_tagger_fn tagger;        /* NULL: device density-gradient indicator */
     void  *tagger_ctx;
     int    tile_ppb, tile_minw, tile_design;   /* stage-kernel tile (ppb <= 0: default) */
+    const amrgpu_inst_t *inst;   /* NULL: instance from the physics parameters; else these
+                                    settings (e.g. the double-Mach walls and states) */
 } amr_evolve_opts_t;
 
 /* The evolve loop with the state resident in HBM across regrids: tags are

@@ -108,12 +108,29 @@ int physics_time_step_gpu(int *dim, int *nfields, int *nelem_subgrid, int *ncell
 typedef struct amrgpu_ctx amrgpu_ctx;
 
 /* Per-instance settings (one forest = one instance). */
+/* Wall types per domain side.  0-2 are the reference's (bc_2d_module.F90:
+ * 44-72); 3-5 are project additions for the double-Mach reflection, whose
+ * boundary states depend on position and time (boundary flux = physical flux
+ * of the boundary state with the outward normal, as flux_bc_2d):
+ *   3 inflow      the prescribed state bstate[0]
+ *   4 shock wall  bstate[0] for x < shock_x0, slip wall beyond
+ *   5 shock top   bstate[0] behind the moving shock line
+ *                 x_s(y, t) = shock_x0 + y / tan(angle) + speed / sin(angle) t,
+ *                 bstate[1] ahead of it
+ * Types 4 and 5 need the quadrant coordinates (amrgpu_setup's geom, or
+ * amrgpu_set_geometry after a remesh); t is the instance time at the start
+ * of the step. */
+enum { AMRGPU_BC_COPY = 0, AMRGPU_BC_SLIP = 1, AMRGPU_BC_NOSLIP = 2, AMRGPU_BC_INFLOW = 3,
+       AMRGPU_BC_SHOCK_WALL = 4, AMRGPU_BC_SHOCK_TOP = 5 };
+
 typedef struct {
     double h_base;      /* coarse_grid_h of this forest */
     int    bc[4];       /* xlo, xhi, ylo, yhi */
     double cfl;         /* effective CFL */
     double dt_cap;      /* input dt */
     double t_end;       /* physical-time stop, <= 0: none */
+    double bstate[2][4];/* boundary states (conservative) of types 3-5 */
+    double shock_x0, shock_angle, shock_speed;   /* types 4-5: degrees, speed */
 } amrgpu_inst_t;
 
 amrgpu_ctx *amrgpu_create(void);
@@ -140,6 +157,9 @@ int amrgpu_indicator(amrgpu_ctx *c, double *ind);
  * settings and time are kept. */
 int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst, int nface,
                   const int *face_info, const int *src);
+/* Quadrant coordinates of the current mesh (geom as amrgpu_setup) for the
+ * position-dependent wall types; a remesh drops them until this is called. */
+int amrgpu_set_geometry(amrgpu_ctx *c, const double *geom);
 /* Residual rhs = -R/V of the current state (reference layout, host out). */
 int amrgpu_rhs(amrgpu_ctx *c, double *rhs);
 /* CFL minimum of the current state per instance (no dt cap). */

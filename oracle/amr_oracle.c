@@ -382,6 +382,46 @@ void orc_residual_bc_2d(int nfields, int n, int side, int type, int level, int b
     }
 }
 
+/* project wall types 3-5 (no reference counterpart) */
+void orc_residual_bc_ext(const orc_params *p, int n, int side, int level, const double geom[12],
+                         const double *soln, double *res)
+{
+    const double pi = 3.141592653589793238462643383279502884197;
+    double normal[2] = {0.0, 0.0}, qbc[4], flux[4];
+    const double h = cell_h_of(p->coarse_grid_h, level, n);
+    const double a = p->shock_angle * pi / 180.0;
+    const double cot = a != 0.0 ? cos(a) / sin(a) : 0.0;
+    const double vx = a != 0.0 ? p->shock_speed / sin(a) : 0.0;
+    const double x0 = geom[0], y0 = geom[1];
+    switch (side) {
+    case 0: normal[0] = -h; break;
+    case 1: normal[0] = h; break;
+    case 2: normal[1] = -h; break;
+    case 3: normal[1] = h; break;
+    }
+    for (int k = 0; k < n; ++k) {
+        int i = 0, j = 0;
+        double xf = x0, yf = y0;
+        switch (side) {
+        case 0: i = 0; j = k; yf = y0 + (k + 0.5) * h; break;
+        case 1: i = n - 1; j = k; xf = x0 + (double)n * h; yf = y0 + (k + 0.5) * h; break;
+        case 2: i = k; j = 0; xf = x0 + (k + 0.5) * h; break;
+        case 3: i = k; j = n - 1; xf = x0 + (k + 0.5) * h; yf = y0 + (double)n * h; break;
+        }
+        const double *q = &soln[QIDX(0, i, j, n)];
+        const int bc = p->bc[side];
+        const double *st = p->bstate[0];
+        if (bc == 4 && !(xf < p->shock_x0)) {
+            orc_bc_2d(1, normal, q, qbc);                      /* slip wall beyond x0 */
+        } else {
+            if (bc == 5 && !(xf < p->shock_x0 + yf * cot + vx * p->time)) st = p->bstate[1];
+            for (int f = 0; f < 4; ++f) qbc[f] = st[f];
+        }
+        orc_flux_bc_2d(qbc, normal, flux);
+        add4(&res[QIDX(0, i, j, n)], flux);
+    }
+}
+
 /* a11  src/solver/src/solver_time_step_2d_module.F90:26-122 */
 void orc_time_step_2d(int nfields, int n, int type, int level, const double geom[12],
                       const double *Q, double cfl, double *dt)
@@ -456,9 +496,13 @@ static void face_residual(const orc_params *p, const int *cell_info, const doubl
 
     switch (facetype) {
     case 1:                                                     /* :142-155 */
-        if (owned(ind, lo, hi))
-            orc_residual_bc_2d(4, n, side, type, level, p->bc[side], p->coarse_grid_h,
-                               &soln[ind], &rhs[ind]);
+        if (owned(ind, lo, hi)) {
+            if (p->bc[side] >= 3)
+                orc_residual_bc_ext(p, n, side, level, &geom[12 * cell], &soln[ind], &rhs[ind]);
+            else
+                orc_residual_bc_2d(4, n, side, type, level, p->bc[side], p->coarse_grid_h,
+                                   &soln[ind], &rhs[ind]);
+        }
         break;
     case 2: {                                                   /* :157-176 */
         int side_plus = (side % 2 == 0) ? side + 1 : side;

@@ -44,6 +44,11 @@ typedef struct {
     double cfl;                  /* effective CFL (already 0.25*cfl*scheme) */
     double dt;                   /* input dt cap */
     double coarse_grid_h;        /* h_base = mesh_scale */
+    /* wall types 3-5 (project additions, no reference counterpart; see
+     * include/amr_hip.h): boundary states, shock line, time of the residual */
+    double bstate[2][4];
+    double shock_x0, shock_angle, shock_speed;
+    double time;
 } orc_params;
 
 /* a1: src/solver/src/flux_2d_module.F90:63-182 */
@@ -79,6 +84,12 @@ void orc_residual_face_hang_full_2d(int nfields, int n, int side, int level_l, i
 /* a6: src/solver/src/solver_residual_bc_structured_2d_module.F90:31-88 */
 void orc_residual_bc_2d(int nfields, int n, int side, int type, int level, int bc,
                         double h_base, const double *soln, double *res);
+/* Wall types 3-5 on side `side` of a quadrant with corners geom (project
+ * additions: prescribed inflow state, post-shock / slip bottom wall, moving
+ * shock along the top; face centres from the quadrant's lower-left corner).
+ * Boundary flux = flux_bc_2d of the boundary state, as the reference's walls. */
+void orc_residual_bc_ext(const orc_params *p, int n, int side, int level, const double geom[12],
+                         const double *soln, double *res);
 /* a11: src/solver/src/solver_time_step_2d_module.F90:26-122 */
 void orc_time_step_2d(int nfields, int n, int type, int level, const double geom[12],
                       const double *Q, double cfl, double *dt);

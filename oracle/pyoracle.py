@@ -30,7 +30,8 @@ def dpt(a):
 
 class OrcParams(C.Structure):
     _fields_ = [("nelem_subgrid", _i), ("time_scheme", _i), ("bc", _i * 6), ("cfl", _d),
-                ("dt", _d), ("coarse_grid_h", _d)]
+                ("dt", _d), ("coarse_grid_h", _d), ("bstate", (_d * 4) * 2), ("shock_x0", _d),
+                ("shock_angle", _d), ("shock_speed", _d), ("time", _d)]
 
 
 RHS_FN = C.CFUNCTYPE(None, _vp, _pd, _pd)
@@ -70,12 +71,21 @@ class Oracle:
         self.L = L
 
     @staticmethod
-    def params(n, bc=(0, 0, 0, 0), cfl=0.3, dt=1.0, h_base=1.0, scheme=3):
+    def params(n, bc=(0, 0, 0, 0), cfl=0.3, dt=1.0, h_base=1.0, scheme=3, bstate=None, shock=None, time=0.0):
+        """bstate: two conservative states, shock: (x0, angle_deg, speed) for
+        the project wall types 3-5; time: instance time of the residual"""
         p = OrcParams()
         p.nelem_subgrid, p.time_scheme = n, scheme
         for s in range(6):
             p.bc[s] = int(bc[s]) if s < len(bc) else 1
         p.cfl, p.dt, p.coarse_grid_h = cfl, dt, h_base
+        if bstate is not None:
+            for a in range(2):
+                for f in range(4):
+                    p.bstate[a][f] = float(bstate[a][f])
+        if shock is not None:
+            p.shock_x0, p.shock_angle, p.shock_speed = (float(v) for v in shock)
+        p.time = time
         return p
 
     def flux(self, ql, qr, nrm):

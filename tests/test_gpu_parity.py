@@ -331,3 +331,73 @@ def test_config1_vortex_200_steps_l1(gpu, oracle):
     e5 = vortex_l1(f5, gpu.soln(), t4)
     log(f"L1 level 5: {e5:.6e} (ratio {e_gpu / e5:.2f})")
     assert e5 < 0.75 * e_gpu
+
+
+def dmr_forest(seed=4):
+    rng = np.random.default_rng(seed)
+    f = amrhip.Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, 8, 1, 3)
+    f.refine((rng.uniform(size=f.nquad) < 0.5).astype(np.int32))
+    f.balance()
+    f.refine((rng.uniform(size=f.nquad) < 0.3).astype(np.int32))
+    f.balance()
+    f.apply_ic(amrhip.dmr_ic())
+    return f
+
+
+def dmr_oracle_params(oracle, t):
+    post, pre, w = amrhip.dmr_states()
+    return oracle.params(8, bc=(3, 0, 4, 5), cfl=0.3, bstate=(post, pre), shock=(1.0 / 6.0, 60.0, w), time=t)
+
+
+@pytest.mark.parametrize("t", [0.0, 0.07])
+def test_rhs_dmr_walls_every_tile(gpu, oracle, t):
+    """wall types 3-5 (position- and time-dependent states) on a hanging-face
+    mesh of [0,4] x [0,1], every tile variant, against the oracle"""
+    f = dmr_forest()
+    ci, fi, geom, soln = f.external()
+    gpu.setup(8, 3, [amrhip.dmr_inst(1.0, 0.3, 1.0)], ci, fi, geom, soln)
+    gpu.set_time([t])
+    ref = oracle.rhs(dmr_oracle_params(oracle, t), ci, fi, geom, soln)
+    for ppb, minw, design in amrhip.tile_variants(8):
+        log(f"dmr walls t={t} tile {(ppb, minw, design)}")
+        gpu.set_tile(ppb, minw, design)
+        assert parity_err(gpu.rhs(), ref) <= RHS_TOL
+
+
+def test_dmr_trajectory_against_oracle(gpu, oracle):
+    """10 SSP-RK3 steps with the moving-shock walls; the boundary time of
+    every stage is the step's start time on both sides"""
+    import ctypes
+    import pyoracle
+    f = dmr_forest(5)
+    ci, fi, geom, soln = f.external()
+    gpu.setup(8, 3, [amrhip.dmr_inst(1.0, 0.3, 1.0)], ci, fi, geom, soln)
+    gpu.run(10)
+    q_gpu = gpu.soln()
+    q = np.array(soln, dtype=float, copy=True)
+    work = np.zeros(5 * q.size)
+    t = 0.0
+    cii = np.ascontiguousarray(ci, np.int32)
+    fii = np.ascontiguousarray(fi, np.int32)
+    g = np.ascontiguousarray(geom, float)
+    for _ in range(10):
+        p = dmr_oracle_params(oracle, t)
+        dt = oracle.time_step(p, ci, geom, q, 1.0)
+        oracle.L.orc_rk_step(ctypes.byref(p), cii.size // 3, pyoracle.ipt(cii), pyoracle.dpt(g),
+                             fii.size // 11, pyoracle.ipt(fii), q.size, dt, pyoracle.dpt(q),
+                             pyoracle.dpt(work))
+        t += dt
+    assert gpu.time()[0][0] == pytest.approx(t, rel=1e-13)
+    assert parity_err(q_gpu, q) <= TRAJ_TOL
+
+
+def test_position_walls_need_geometry_after_remesh(gpu):
+    f = dmr_forest(6)
+    ci, fi, geom, soln = f.external()
+    gpu.setup(8, 3, [amrhip.dmr_inst(1.0, 0.3, 1.0)], ci, fi, geom, soln)
+    src = np.arange(f.nquad, dtype=np.int32)
+    gpu.remesh(ci, fi, src)
+    with pytest.raises(amrhip.GpuError):
+        gpu.run(1)
+    amrhip.check(amrhip.lib().amrgpu_set_geometry(gpu.h, amrhip.dpt(np.ascontiguousarray(geom))), "geom")
+    gpu.run(1)
