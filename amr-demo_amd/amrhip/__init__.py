@@ -140,7 +140,7 @@ TAGGER_FN = C.CFUNCTYPE(_i, _vp, _vp, _vp, _pi)
 class CollectOpts(C.Structure):
     """amr_collect_opts_t"""
     _fields_ = [("tol", _d), ("coarsen_frac", _d), ("window", _i), ("label_stride", _i),
-                ("allow_coarsening", _i)]
+                ("allow_coarsening", _i), ("inst", C.POINTER(InstParams))]
 
 
 class MlpTaggerStruct(C.Structure):
@@ -395,11 +395,13 @@ class Forest:
         check(rc, "amr_evolve_device")
         return st
 
-    def collect(self, tagset, tol, window=40, label_stride=5, coarsen_frac=0.5, allow_coarsening=True):
+    def collect(self, tagset, tol, window=40, label_stride=5, coarsen_frac=0.5, allow_coarsening=True, inst=None):
         """run to the physical stop collecting labelled snapshots (amr_tagger_collect)"""
         o = CollectOpts()
         o.tol, o.coarsen_frac, o.window, o.label_stride = tol, coarsen_frac, window, label_stride
         o.allow_coarsening = int(allow_coarsening)
+        if inst is not None:
+            o.inst = C.pointer(inst)
         rows = C.c_int(0)
         check(self.L.amr_tagger_collect(self.h, C.byref(o), tagset.h, C.byref(rows)), "amr_tagger_collect")
         return rows.value

@@ -16,7 +16,8 @@ import time
 
 import numpy as np
 
-from . import (Forest, Mlp, MlpTagger, Tagset, riemann4_ic, set_params, vortex_ic, TAG_REFINE)
+from . import (Forest, Mlp, MlpTagger, Tagset, dmr_ic, dmr_inst, get_params, riemann4_ic, set_params, vortex_ic,
+               TAG_REFINE)
 
 ROOT_LEN = 1 << 30
 
@@ -123,10 +124,10 @@ def recall_at_equal_fraction(mlp, ts, tol=0.1, biases=np.arange(0.0, 4.01, 0.25)
 
 def density_on_grid(f, level):
     """piecewise-constant density of the forest's patches sampled on the
-    uniform grid of `level` quadrants (8x8 cells each): (N, N) array, N =
-    8 * 2**level (tree 0 of a one-tree brick)"""
+    uniform grid of `level` quadrants (8x8 cells each): (N, ntx * N) array,
+    N = 8 * 2**level, for a brick of ntx x 1 trees"""
     N = 8 * (1 << level)
-    out = np.zeros((N, N))
+    out = np.zeros((N, f.ntx * N))
     q = f.soln()                         # (nquad, 8, 8, 4) [quad][j][i][field]
     quads = f.quads()
     lv = np.array([qq[1] for qq in quads])
@@ -137,7 +138,8 @@ def density_on_grid(f, level):
             raise ValueError("forest finer than the sampling grid")
         idx = np.nonzero(lv == L)[0]
         r = 1 << (level - L)             # fine cells per patch cell
-        x0 = (xs[idx] >> (30 - level)) * 8
+        trees = np.array([quads[i][0] for i in idx], np.int64)
+        x0 = (xs[idx] >> (30 - level)) * 8 + trees * N
         y0 = (ys[idx] >> (30 - level)) * 8
         rho = q[idx, :, :, 0]            # (k, 8, 8) [j][i]
         blk = np.repeat(np.repeat(rho, r, axis=1), r, axis=2)   # (k, 8r, 8r)
@@ -161,8 +163,67 @@ def uniform_reference(seed, level, t_end, log=None):
 
 
 def l1_error(rho, ref):
-    N = ref.shape[0]
-    return float(np.abs(rho - ref).sum() / (N * N))
+    """mean |rho - ref| over the grid (L1 per unit area of a unit-size tree)"""
+    return float(np.abs(rho - ref).mean())
+
+
+# ---- double-Mach reflection (config 5): [0,4] x [0,1], four unit trees ----
+def dmr_params(t_end, regrid=10):
+    set_params(nelem_subgrid=8, time_scheme=3, time_steps=10 ** 9, dt=1.0, cfl=0.25 * 0.4 * 3,
+               coarse_grid_h=1.0, regrid_interval=regrid, checkpoint_interval=2 ** 31 - 1,
+               visualization_interval=2 ** 31 - 1, bc=(0, 0, 0, 0, 1, 1), t_end=t_end, verbose=0)
+    p = get_params()
+    return dmr_inst(1.0, p.cfl, p.dt, t_end)
+
+
+def dmr_forest(lmin, lmax, tol=0.1):
+    ic = dmr_ic()
+    f = Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, 8, lmin, lmax)
+    f.apply_ic(ic)
+    f.regrid_initial(ic, tol)
+    return f
+
+
+def dmr_uniform_reference(level, t_end, log=None):
+    inst = dmr_params(t_end, regrid=2 ** 31 - 1)
+    f = Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, 8, level, level)
+    f.apply_ic(dmr_ic())
+    t0 = time.perf_counter()
+    st = f.evolve_device(0.1, 0, inst=inst)
+    if log:
+        log(f"uniform level-{level} double-Mach reference: {f.nquad * 64} cells, {st.steps} steps, "
+            f"{time.perf_counter() - t0:.1f} s")
+    return density_on_grid(f, level), st
+
+
+def dmr_collect(lmin, lmax, t_end, tol=0.1, window=40, label_stride=5, tagset=None, log=None):
+    ts = tagset if tagset is not None else Tagset()
+    inst = dmr_params(t_end)
+    f = dmr_forest(lmin, lmax, tol)
+    t0 = time.perf_counter()
+    rows = f.collect(ts, tol, window=window, label_stride=label_stride, inst=inst)
+    if log:
+        log(f"collect double-Mach levels {lmin}-{lmax} to t={t_end}: {rows} rows in {time.perf_counter() - t0:.2f} s")
+    return ts
+
+
+def dmr_deploy(lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None):
+    inst = dmr_params(t_end, regrid=interval)
+    f = dmr_forest(lmin, lmax, tol)
+    t0 = time.perf_counter()
+    st = f.evolve_device(tol, interval, tagger=tagger, inst=inst)
+    wall = time.perf_counter() - t0
+    out = {
+        "regrid_interval": interval, "steps": st.steps, "regrids": st.regrids, "t": st.t,
+        "peak_cells": st.peak_quads * 64, "final_cells": f.nquad * 64, "cell_stages": st.cell_stages,
+        "wall_s": wall, "gpu_s": st.wall_gpu, "regrid_s": st.wall_regrid,
+        "end_to_end_Mcell_upd_s": st.cell_stages / wall / 1e6,
+    }
+    if ref is not None:
+        out["l1_density"] = l1_error(density_on_grid(f, ref_level), ref)
+    if isinstance(tagger, MlpTagger):
+        out["tagger"] = tagger.stats()
+    return out, f
 
 
 def deploy(seed, lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None):

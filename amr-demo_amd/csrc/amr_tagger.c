@@ -66,7 +66,7 @@ int amr_tagger_collect(amr_forest_t *f, const amr_collect_opts_t *o, amr_tagset_
     physics_gpu_params_t p;
     physics_gpu_get_params(&p);
     amrgpu_ctx *g = NULL;
-    int rc = amr_device_setup(f, NULL, &g);
+    int rc = amr_device_setup(f, o->inst, &g);
     if (rc) return rc;
     host_bufs_t b;
     memset(&b, 0, sizeof(b));

@@ -167,6 +167,40 @@ def learned_experiment(amrhip, args):
     }, m
 
 
+def dmr_experiment(amrhip, args, model):
+    """config 5 at reduced depth: the Mach-10 double-Mach reflection on
+    [0,4] x [0,1] (levels 2..--dmr-lmax, 8x8 FP64 patches, to --dmr-t-end),
+    plain indicator every 10 steps vs the learned tagger (fine-tuned on
+    double-Mach snapshots of a run one level coarser) every
+    --learned-interval steps; L1 density error against the plain solution on
+    the uniform finest level"""
+    from amrhip import learned as L
+    lmax, iv, t_end = args.dmr_lmax, args.learned_interval, args.dmr_t_end
+    out = {"workload": f"double-Mach reflection, Mach 10 at 60 deg, [0,4]x[0,1], levels 2-{lmax}, "
+                       f"8x8 FP64 patches, t={t_end}"}
+    if model is not None:
+        t0 = time.perf_counter()
+        ts = L.dmr_collect(2, lmax - 1, t_end, tol=args.tol, window=iv, log=log)
+        losses = model.train(ts, epochs=10, batch=4096, lr=1e-3, cw=(1.0, 1.0, 2.0), seed=7)
+        out["fine_tune"] = {"rows": ts.size, "epochs": 10, "loss_first_last": [float(losses[0]), float(losses[-1])],
+                            "wall_s": round(time.perf_counter() - t0, 2)}
+        ts.close()
+    ref, _ = L.dmr_uniform_reference(lmax, t_end, log=log)
+    runs = [("plain_every_10", 10, None)]
+    if model is not None:
+        runs.append((f"learned_every_{iv}", iv, amrhip.MlpTagger(model, args.tol, True)))
+    for name, interval, tagger in runs:
+        r, _ = L.dmr_deploy(2, lmax, t_end, interval, tagger, args.tol, ref, lmax)
+        out[name] = r
+        log(f"double-Mach {name}: {r['steps']} steps, {r['regrids']} regrids, peak {r['peak_cells']} cells, "
+            f"{r['wall_s']:.2f} s, {r['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s, L1 {r['l1_density']:.4e}")
+    if model is not None:
+        a, b = out["plain_every_10"], out[f"learned_every_{iv}"]
+        out["speedup_end_to_end"] = b["end_to_end_Mcell_upd_s"] / a["end_to_end_Mcell_upd_s"]
+        out["l1_ratio"] = b["l1_density"] / a["l1_density"]
+    return out
+
+
 def batched_experiment(amrhip, args, model):
     """K independent Riemann problems (seeds 1000+k, max level drawn from
     {5,6,7,8}, levels lmin..max) advanced together in one device context
@@ -257,6 +291,9 @@ def main():
     ap.add_argument("--learned-lmax", type=int, default=8, help="finest level of the deployed run")
     ap.add_argument("--learned-interval", type=int, default=40)
     ap.add_argument("--learned-epochs", type=int, default=30)
+    ap.add_argument("--no-dmr", action="store_true", help="skip the double-Mach reflection comparison")
+    ap.add_argument("--dmr-lmax", type=int, default=7)
+    ap.add_argument("--dmr-t-end", type=float, default=0.2)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances comparison")
     ap.add_argument("--batch-instances", type=int, default=16)
     ap.add_argument("--batch-t-end", type=float, default=0.25)
@@ -402,6 +439,8 @@ def main():
         out["learned"], model = learned_experiment(amrhip, args)
     if args.workload == "riemann4" and args.n == 8 and not args.no_batch:
         out["batched"] = batched_experiment(amrhip, args, model)
+    if args.workload == "riemann4" and args.n == 8 and not args.no_dmr:
+        out["double_mach"] = dmr_experiment(amrhip, args, model)
     if model is not None:
         model.close()
     print(json.dumps(out), flush=True)

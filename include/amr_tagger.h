@@ -38,6 +38,8 @@
 
 #include <stdint.h>
 
+#include "amr_hip.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -109,6 +111,7 @@ typedef struct {
     int window;              /* steps between regrids = label horizon (40)       */
     int label_stride;        /* label times every label_stride steps (5)         */
     int allow_coarsening;
+    const amrgpu_inst_t *inst;   /* NULL: instance from the physics parameters */
 } amr_collect_opts_t;
 
 /* Advances the forest (initial condition already applied) to the physical
