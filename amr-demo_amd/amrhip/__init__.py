@@ -153,7 +153,7 @@ class EvolveOpts(C.Structure):
     """amr_evolve_opts_t"""
     _fields_ = [("tol", _d), ("allow_coarsening", _i), ("regrid_interval", _i), ("tagger", _vp),
                 ("tagger_ctx", _vp), ("tile_ppb", _i), ("tile_minw", _i), ("tile_design", _i),
-                ("inst", C.POINTER(InstParams))]
+                ("inst", C.POINTER(InstParams)), ("regrid_lag", _i)]
 
 
 class PhysicsParams(C.Structure):
@@ -376,7 +376,7 @@ class Forest:
         return st
 
     def evolve_device(self, tol, regrid_interval, allow_coarsening=True, tile=None, tagger=None,
-                      tagger_ctx=None, inst=None):
+                      tagger_ctx=None, inst=None, regrid_lag=0):
         """device-resident evolve loop; tagger: optional TAGGER_FN or MlpTagger"""
         o = EvolveOpts()
         o.tol, o.allow_coarsening, o.regrid_interval = tol, int(allow_coarsening), regrid_interval
@@ -390,6 +390,7 @@ class Forest:
             o.tile_ppb, o.tile_minw, o.tile_design = tile
         if inst is not None:
             o.inst = C.pointer(inst)
+        o.regrid_lag = regrid_lag
         st = EvolveStats()
         rc = self.L.amr_evolve_device(self.h, C.byref(o), C.byref(st))
         check(rc, "amr_evolve_device")
@@ -407,7 +408,8 @@ class Forest:
         return rows.value
 
 
-def evolve_batch(forests, insts, tol, regrid_interval, allow_coarsening=True, tile=None, tagger=None):
+def evolve_batch(forests, insts, tol, regrid_interval, allow_coarsening=True, tile=None, tagger=None,
+                 regrid_lag=0):
     """amr_evolve_batch: every forest advanced in one device context;
     insts[k] = make_inst(...) of forest k.  Returns (per-instance stats, total)"""
     L = lib()
@@ -421,6 +423,7 @@ def evolve_batch(forests, insts, tol, regrid_interval, allow_coarsening=True, ti
         o.tagger_ctx = C.cast(C.pointer(tagger.s), _vp)
     if tile:
         o.tile_ppb, o.tile_minw, o.tile_design = tile
+    o.regrid_lag = regrid_lag
     per = (EvolveStats * k)()
     tot = EvolveStats()
     check(L.amr_evolve_batch(fa, ia, k, C.byref(o), per, C.byref(tot)), "amr_evolve_batch")

@@ -207,14 +207,14 @@ def dmr_collect(lmin, lmax, t_end, tol=0.1, window=40, label_stride=5, tagset=No
     return ts
 
 
-def dmr_deploy(lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None):
+def dmr_deploy(lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None, lag=0):
     inst = dmr_params(t_end, regrid=interval)
     f = dmr_forest(lmin, lmax, tol)
     t0 = time.perf_counter()
-    st = f.evolve_device(tol, interval, tagger=tagger, inst=inst)
+    st = f.evolve_device(tol, interval, tagger=tagger, inst=inst, regrid_lag=lag)
     wall = time.perf_counter() - t0
     out = {
-        "regrid_interval": interval, "steps": st.steps, "regrids": st.regrids, "t": st.t,
+        "regrid_interval": interval, "regrid_lag": lag, "steps": st.steps, "regrids": st.regrids, "t": st.t,
         "peak_cells": st.peak_quads * 64, "final_cells": f.nquad * 64, "cell_stages": st.cell_stages,
         "wall_s": wall, "gpu_s": st.wall_gpu, "regrid_s": st.wall_regrid,
         "end_to_end_Mcell_upd_s": st.cell_stages / wall / 1e6,
@@ -226,15 +226,16 @@ def dmr_deploy(lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_
     return out, f
 
 
-def deploy(seed, lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None):
-    """one adaptive run on the device loop; end-to-end rate and L1 error"""
+def deploy(seed, lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None, lag=0):
+    """one adaptive run on the device loop; end-to-end rate and L1 error;
+    lag > 0 overlaps the host regrid with `lag` steps on the old mesh"""
     riemann_params(t_end, regrid=interval)
     f = riemann_forest(seed, lmin, lmax, tol)
     t0 = time.perf_counter()
-    st = f.evolve_device(tol, interval, tagger=tagger)
+    st = f.evolve_device(tol, interval, tagger=tagger, regrid_lag=lag)
     wall = time.perf_counter() - t0
     out = {
-        "regrid_interval": interval,
+        "regrid_interval": interval, "regrid_lag": lag,
         "steps": st.steps, "regrids": st.regrids, "t": st.t,
         "peak_cells": st.peak_quads * 64, "final_cells": f.nquad * 64,
         "cell_stages": st.cell_stages, "wall_s": wall, "gpu_s": st.wall_gpu, "regrid_s": st.wall_regrid,

@@ -395,6 +395,44 @@ int amr_device_setup(amr_forest_t *f, const amrgpu_inst_t *override, amrgpu_ctx 
     return AMRGPU_OK;
 }
 
+typedef struct {
+    int ncell, nface, rc;
+    int *cell_info, *face_info;
+    double *geom, *soln;
+} ext_t;
+
+static int ext_of(amr_forest_t *f, ext_t *e)
+{
+    int nghost, dof;
+    e->rc = amr_forest_build_external(f, &e->ncell, &nghost, &e->nface, &dof, &e->cell_info, &e->face_info,
+                                      &e->geom, &e->soln)
+                ? AMRGPU_ERR_MESH
+                : AMRGPU_OK;
+    return e->rc;
+}
+
+/* Host half of a lagged regrid: topology and boundary arrays of one forest on
+ * a thread of its own while the GPU keeps stepping the old mesh. */
+typedef struct {
+    amr_forest_t *f;
+    const int *tags;
+    int allow, ok;
+    ext_t ext;
+    double wall;
+} lag_job_t;
+
+static void *lag_regrid_thread(void *arg)
+{
+    lag_job_t *j = (lag_job_t *)arg;
+    double t0 = now_s();
+    amr_forest_track_begin(j->f, 0);
+    amr_regrid_tags(j->f, j->tags, j->allow);
+    j->ok = amr_forest_track_end(j->f);
+    ext_of(j->f, &j->ext);
+    j->wall = now_s() - t0;
+    return NULL;
+}
+
 double amr_now(void) { return now_s(); }
 
 int amr_device_regrid(amr_forest_t *f, amrgpu_ctx *g, const int *tags, int allow_coarsening, double *t_forest,
@@ -487,11 +525,47 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
         }
         s.wall_tags += now_s() - t2;
         if (rc) break;
-        rc = amr_device_regrid(f, g, tags, o ? o->allow_coarsening : 1, &s.wall_forest, &s.wall_remesh);
+        const int lag = o ? o->regrid_lag : 0;
+        if (lag > 0 && lag < interval) {
+            lag_job_t job;
+            memset(&job, 0, sizeof(job));
+            job.f = f; job.tags = tags; job.allow = o->allow_coarsening;
+            pthread_t th;
+            const int threaded = pthread_create(&th, NULL, lag_regrid_thread, &job) == 0;
+            if (!threaded) lag_regrid_thread(&job);
+            /* the GPU advances the old mesh meanwhile */
+            int nl = lag;
+            if (istep + nl > p.time_steps) nl = p.time_steps - istep;
+            double t4 = now_s();
+            double resnorm;
+            if (nl > 0) rc = amrgpu_run(g, nl, 1, &resnorm);
+            if (!rc && nl > 0) {
+                istep += nl;
+                s.steps += nl;
+                s.cell_stages += (double)nl * ncell * n * n * stages[p.time_scheme];
+                if (p.t_end > 0.0 || g_outdir[0]) amrgpu_get_time(g, &t, NULL);
+            }
+            const double t5 = now_s();
+            s.wall_gpu += t5 - t4;
+            if (threaded) pthread_join(th, NULL);
+            s.wall_forest += job.wall;
+            if (!rc && (!job.ok || job.ext.rc)) rc = AMRGPU_ERR_MESH;
+            const double t6 = now_s();
+            if (!rc) rc = amrgpu_remesh(g, job.ext.ncell, job.ext.cell_info, NULL, job.ext.nface,
+                                        job.ext.face_info, amr_forest_track_src(f));
+            if (!rc) rc = amrgpu_set_geometry(g, job.ext.geom);
+            s.wall_remesh += now_s() - t6;
+            /* the regrid wall counts what the host made the GPU wait for:
+             * tagging, the join after the overlapped steps, the remesh */
+            t2 += t5 - t4;
+        } else {
+            rc = amr_device_regrid(f, g, tags, o ? o->allow_coarsening : 1, &s.wall_forest, &s.wall_remesh);
+        }
         ncell = amr_forest_num_quads(f);
         s.regrids++;
         if (ncell > s.peak_quads) s.peak_quads = ncell;
         s.wall_regrid += now_s() - t2;
+        if (istep >= p.time_steps || (p.t_end > 0.0 && t >= p.t_end)) break;   /* stop reached while lagging */
     }
     if (!rc) {
         amrgpu_get_time(g, &t, NULL);
@@ -545,22 +619,6 @@ static int batch_grow(batch_bufs_t *b, size_t nc, size_t nf, int n2)
         b->cap_f = c;
     }
     return 0;
-}
-
-typedef struct {
-    int ncell, nface, rc;
-    int *cell_info, *face_info;
-    double *geom, *soln;
-} ext_t;
-
-static int ext_of(amr_forest_t *f, ext_t *e)
-{
-    int nghost, dof;
-    e->rc = amr_forest_build_external(f, &e->ncell, &nghost, &e->nface, &dof, &e->cell_info, &e->face_info,
-                                      &e->geom, &e->soln)
-                ? AMRGPU_ERR_MESH
-                : AMRGPU_OK;
-    return e->rc;
 }
 
 /* Concatenated boundary arrays of all forests (quadrants of instance k at
@@ -644,6 +702,8 @@ static int host_threads(int work)
     return n < 1 ? 1 : (int)n;
 }
 
+static void *regrid_all_thread(void *arg);
+
 static void regrid_all(regrid_job_t *j)
 {
     const int nt = host_threads(j->ninst);
@@ -656,6 +716,12 @@ static void regrid_all(regrid_job_t *j)
     regrid_worker(j);
     for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
     pthread_mutex_destroy(&j->lock);
+}
+
+static void *regrid_all_thread(void *arg)
+{
+    regrid_all((regrid_job_t *)arg);
+    return NULL;
 }
 
 int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, const amr_evolve_opts_t *o,
@@ -755,7 +821,37 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
         job.f = f; job.done = done; job.tags = tags; job.obase = obase;
         job.allow = o ? o->allow_coarsening : 1;
         job.ninst = ninst; job.ext = ext; job.ok = ok;
-        regrid_all(&job);
+        const int lag = o ? o->regrid_lag : 0;
+        double t_gpu_lag = 0.0;
+        if (lag > 0 && lag < interval) {
+            /* the host regrids every instance while the GPU advances the old
+             * meshes lag more steps; instances that reached their stop before
+             * this regrid stay frozen and unregridded */
+            pthread_t th;
+            const int threaded = pthread_create(&th, NULL, regrid_all_thread, &job) == 0;
+            if (!threaded) regrid_all(&job);
+            int nl = lag;
+            if (istep + nl > p.time_steps) nl = p.time_steps - istep;
+            const double t4 = now_s();
+            double resnorm;
+            if (nl > 0) rc = amrgpu_run(g, nl, 1, &resnorm);
+            if (!rc && nl > 0) {
+                for (int k = 0; k < ninst; ++k) {
+                    if (done[k]) continue;
+                    st[k].steps += nl;
+                    st[k].cell_stages += (double)nl * (base[k + 1] - base[k]) * n2 * stages[p.time_scheme];
+                }
+                istep += nl;
+                tot.steps += nl;
+                tot.cell_stages += (double)nl * ncell * n2 * stages[p.time_scheme];
+                rc = amrgpu_get_time(g, t, NULL);
+            }
+            t_gpu_lag = now_s() - t4;
+            tot.wall_gpu += t_gpu_lag;
+            if (threaded) pthread_join(th, NULL);
+        } else {
+            regrid_all(&job);
+        }
         for (int k = 0; k < ninst; ++k) {
             if (!ok[k]) rc = AMRGPU_ERR_MESH;
             if (!done[k]) st[k].regrids++;
@@ -773,13 +869,21 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
             if (nk > st[k].peak_quads) st[k].peak_quads = nk;
         }
         double t4 = now_s();
-        tot.wall_forest += t4 - t3;
+        tot.wall_forest += t4 - t3 - t_gpu_lag;
         if (!rc) rc = amrgpu_remesh(g, ncell, b.ci, b.cinst, nface, b.fi, b.src);
         if (!rc) rc = amrgpu_set_geometry(g, b.geom);
         tot.wall_remesh += now_s() - t4;
         tot.regrids++;
         if (ncell > tot.peak_quads) tot.peak_quads = ncell;
-        tot.wall_regrid += now_s() - t2;
+        tot.wall_regrid += now_s() - t2 - t_gpu_lag;
+        if (t_gpu_lag > 0.0) {       /* stops reached during the overlapped steps */
+            int all = 1;
+            for (int k = 0; k < ninst; ++k) {
+                if (inst[k].t_end > 0.0 && t[k] >= inst[k].t_end) done[k] = 1;
+                if (!done[k]) all = 0;
+            }
+            if (istep >= p.time_steps || all) break;
+        }
     }
     if (!rc) {
         rc = amrgpu_get_time(g, t, NULL);

@@ -144,9 +144,10 @@ def learned_experiment(amrhip, args):
         f"vs indicator-now {rec['recall_indicator_now']:.3f}")
     ref, _ = L.uniform_reference(args.seed, lmax, args.t_end, log=log)
     runs = {}
-    for name, interval, tagger in (("plain_every_10", 10, None), (f"plain_every_{iv}", iv, None),
-                                   (f"learned_every_{iv}", iv, amrhip.MlpTagger(m, args.tol, True))):
-        r, _ = L.deploy(args.seed, args.lmin, lmax, args.t_end, interval, tagger, args.tol, ref, lmax)
+    for name, interval, tagger, lag in (("plain_every_10", 10, None, 0), (f"plain_every_{iv}", iv, None, 0),
+                                        (f"learned_every_{iv}", iv, amrhip.MlpTagger(m, args.tol, True),
+                                         args.learned_lag)):
+        r, _ = L.deploy(args.seed, args.lmin, lmax, args.t_end, interval, tagger, args.tol, ref, lmax, lag)
         runs[name] = r
         log(f"{name}: {r['steps']} steps, {r['regrids']} regrids, peak {r['peak_cells']} cells, "
             f"{r['wall_s']:.2f} s, {r['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s end to end, "
@@ -190,7 +191,8 @@ def dmr_experiment(amrhip, args, model):
     if model is not None:
         runs.append((f"learned_every_{iv}", iv, amrhip.MlpTagger(model, args.tol, True)))
     for name, interval, tagger in runs:
-        r, _ = L.dmr_deploy(2, lmax, t_end, interval, tagger, args.tol, ref, lmax)
+        r, _ = L.dmr_deploy(2, lmax, t_end, interval, tagger, args.tol, ref, lmax,
+                            args.learned_lag if tagger is not None else 0)
         out[name] = r
         log(f"double-Mach {name}: {r['steps']} steps, {r['regrids']} regrids, peak {r['peak_cells']} cells, "
             f"{r['wall_s']:.2f} s, {r['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s, L1 {r['l1_density']:.4e}")
@@ -242,7 +244,8 @@ def batched_experiment(amrhip, args, model):
     for name, iv, tagger in runs:
         fb = forests()
         t0 = time.perf_counter()
-        per, tot = amrhip.evolve_batch(fb, insts, args.tol, iv, tagger=tagger)
+        per, tot = amrhip.evolve_batch(fb, insts, args.tol, iv, tagger=tagger,
+                                       regrid_lag=args.learned_lag if tagger is not None else 0)
         wall = time.perf_counter() - t0
         useful = sum(p.cell_stages for p in per)
         r = {"wall_s": round(wall, 3), "steps": tot.steps, "regrids": tot.regrids,
@@ -291,6 +294,8 @@ def main():
     ap.add_argument("--learned-lmax", type=int, default=8, help="finest level of the deployed run")
     ap.add_argument("--learned-interval", type=int, default=40)
     ap.add_argument("--learned-epochs", type=int, default=30)
+    ap.add_argument("--learned-lag", type=int, default=10,
+                    help="learned runs: steps the GPU advances while the host regrids (0: none)")
     ap.add_argument("--no-dmr", action="store_true", help="skip the double-Mach reflection comparison")
     ap.add_argument("--dmr-lmax", type=int, default=7)
     ap.add_argument("--dmr-t-end", type=float, default=0.2)

@@ -113,6 +113,10 @@ typedef struct {
     int    tile_ppb, tile_minw, tile_design;   /* stage-kernel tile (ppb <= 0: default) */
     const amrgpu_inst_t *inst;   /* NULL: instance from the physics parameters; else these
                                     settings (e.g. the double-Mach walls and states) */
+    int    regrid_lag;           /* 0: regrid at the tagging step (the reference's order);
+                                    k in (0, regrid_interval): the host regrids the topology on
+                                    a thread while the GPU advances k more steps on the old mesh,
+                                    then the state k steps later is remapped (tags k steps old) */
 } amr_evolve_opts_t;
 
 /* The evolve loop with the state resident in HBM across regrids: tags are

@@ -30,18 +30,20 @@ def forest(seed, lmax, lmin=2):
 CASES = [(0, 4), (1, 5), (2, 5), (3, 6)]
 
 
-@pytest.mark.parametrize("allow_coarsening", [True, False])
-def test_batch_equals_each_instance_alone(allow_coarsening):
+@pytest.mark.parametrize("allow_coarsening,lag", [(True, 0), (False, 0), (True, 4)])
+def test_batch_equals_each_instance_alone(allow_coarsening, lag):
+    """lag > 0: the host regrids while the GPU advances `lag` more steps
+    (alone and batched alike)"""
     params()
     batch = [forest(s, lm) for s, lm in CASES]
     insts = [amrhip.make_inst(1.0, (0, 0, 0, 0), 0.3, 1.0, T_END) for _ in CASES]
-    print(f"batch sizes {[f.nquad for f in batch]}", flush=True)
-    per, tot = amrhip.evolve_batch(batch, insts, 0.1, 10, allow_coarsening=allow_coarsening)
+    print(f"batch sizes {[f.nquad for f in batch]} lag {lag}", flush=True)
+    per, tot = amrhip.evolve_batch(batch, insts, 0.1, 10, allow_coarsening=allow_coarsening, regrid_lag=lag)
     assert tot.regrids > 0
     steps = []
     for (s, lm), fb, st in zip(CASES, batch, per):
         fa = forest(s, lm)
-        sa = fa.evolve_device(0.1, 10, allow_coarsening=allow_coarsening)
+        sa = fa.evolve_device(0.1, 10, allow_coarsening=allow_coarsening, regrid_lag=lag)
         print(f"seed {s}: alone {sa.steps} steps {sa.regrids} regrids; batch {st.steps} / {st.regrids}",
               flush=True)
         assert st.t == sa.t == pytest.approx(T_END, abs=0)
@@ -67,3 +69,27 @@ def test_batch_with_learned_tagger_runs():
         assert st.t == pytest.approx(T_END, abs=0)
         assert f.is_balanced()
     m.close()
+
+
+def test_lagged_regrid_conserves_and_differs_from_immediate():
+    """the lagged loop is a different (legitimate) schedule: it conserves
+    mass on a periodic box to round-off and ends on a balanced mesh"""
+    amrhip.set_params(nelem_subgrid=8, time_scheme=3, time_steps=60, dt=1.0, cfl=0.3, coarse_grid_h=10.0,
+                      regrid_interval=10, checkpoint_interval=2 ** 31 - 1, visualization_interval=2 ** 31 - 1,
+                      bc=(0, 0, 0, 0, 1, 1), t_end=0.0, verbose=0)
+    ic = amrhip.vortex_ic(5.0, 5.0, 5.0)
+    out = []
+    for lag in (0, 5):
+        f = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 10.0, 8, 2, 4)
+        f.apply_ic(ic)
+        f.regrid_initial(ic, 0.02)
+        s0 = f.soln().reshape(f.nquad, -1, 4)
+        lv0 = f.levels()
+        m0 = float(np.sum(s0[:, :, 0].sum(axis=1) * (10.0 / 2.0 ** lv0 / 8) ** 2))
+        st = f.evolve_device(0.02, 10, regrid_lag=lag)
+        assert st.steps == 60 and f.is_balanced()
+        s1 = f.soln().reshape(f.nquad, -1, 4)
+        m1 = float(np.sum(s1[:, :, 0].sum(axis=1) * (10.0 / 2.0 ** f.levels() / 8) ** 2))
+        assert abs(m1 - m0) <= 1e-11 * abs(m0)
+        out.append((f.quads(), s1.copy()))
+    assert out[0][0] != out[1][0] or not np.array_equal(out[0][1], out[1][1])
