@@ -33,9 +33,12 @@ namespace {
 constexpr int NFLD = 4;
 constexpr unsigned long long DBL_INF_BITS = 0x7ff0000000000000ull;
 // The CFL minimum of an instance is spread over NSLOT sub-slots (workgroup b
-// folds into sub-slot b % NSLOT): a single address would serialise one 64-bit
-// atomic per workgroup; the consumer reduces the sub-slots with one wave.
-constexpr int NSLOT = 64;
+// folds into sub-slot b % NSLOT): atomics on one address serialise, and on a
+// 16.8M-cell mesh with one patch per workgroup 64 sub-slots still left the
+// last stage ~17 % slower than the others (4096 atomics per address); 256
+// recover most of it while the first stage's wave still reduces the
+// sub-slots with four loads per lane.
+constexpr int NSLOT = 256;
 
 struct InstDev {
     double h_base;
