@@ -300,7 +300,7 @@ def main():
     ap.add_argument("--dmr-lmax", type=int, default=7)
     ap.add_argument("--dmr-t-end", type=float, default=0.2)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances comparison")
-    ap.add_argument("--batch-instances", type=int, default=16)
+    ap.add_argument("--batch-instances", type=int, default=64)
     ap.add_argument("--batch-t-end", type=float, default=0.25)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
