@@ -227,10 +227,13 @@ int amr_regrid_tags(amr_forest_t *f, const int *tags, int allow_coarsening)
         tag[k] = tags[k] == 1 ? 1 : (tags[k] ? 2 : 0);
         flags[k] = tags[k] == 1;
     }
+    /* the forest is balanced on entry (every regrid leaves it so): balance
+     * only around what this regrid refines and coarsens */
+    amr_forest_clear_marks(f);
     amr_forest_refine(f, flags);
     free(flags);
     if (allow_coarsening) amr_forest_coarsen(f);
-    amr_forest_balance(f);
+    amr_forest_balance_marked(f);
     return amr_forest_num_quads(f);
 }
 

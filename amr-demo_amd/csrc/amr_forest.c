@@ -18,14 +18,18 @@ struct amr_forest {
     double xlo, ylo, scale;
     int n, min_level, max_level;
     int nquad, cap;
+    int soln_cap;         /* patches the soln array holds (>= nquad when read) */
     amr_quad_t *q;
     uint64_t *key;        /* Morton key per quadrant (lower-left corner) */
     double *soln;         /* nquad * 4*n*n */
     int *tag, *type;
+    /* 1: created by the last refine, 2: by the last coarsen (worklist of the
+     * regrid's balance) */
+    unsigned char *mark;
     /* boundary arrays (rebuilt by amr_forest_build_external) */
     int *cell_info, *face_info;
     double *geom;
-    int nface;
+    int nface, ext_cap;
     /* regrid tracking: source of every quadrant's patch (see track_begin) */
     int *src;
     int track, track_data, track_ok;
@@ -111,7 +115,9 @@ static void reserve(amr_forest_t *f, int cap)
     f->soln = (double *)realloc(f->soln, sizeof(double) * (size_t)ps * (size_t)cap);
     f->tag = (int *)realloc(f->tag, sizeof(int) * (size_t)cap);
     f->type = (int *)realloc(f->type, sizeof(int) * (size_t)cap);
+    f->mark = (unsigned char *)realloc(f->mark, (size_t)cap);
     f->cap = cap;
+    f->soln_cap = cap;
 }
 
 amr_forest_t *amr_forest_new(int ntx, int nty, int periodic_x, int periodic_y,
@@ -143,7 +149,7 @@ amr_forest_t *amr_forest_new(int ntx, int nty, int periodic_x, int periodic_y,
             f->q[k].tree = t; f->q[k].level = min_level;
             f->q[k].x = x * len; f->q[k].y = y * len;
             f->key[k] = morton(f->q[k].x, f->q[k].y);
-            f->tag[k] = 0; f->type[k] = 0;
+            f->tag[k] = 0; f->type[k] = 0; f->mark[k] = 0;
             ++k;
         }
     }
@@ -155,7 +161,7 @@ amr_forest_t *amr_forest_new(int ntx, int nty, int periodic_x, int periodic_y,
 void amr_forest_destroy(amr_forest_t *f)
 {
     if (!f) return;
-    free(f->q); free(f->key); free(f->soln); free(f->tag); free(f->type);
+    free(f->q); free(f->key); free(f->soln); free(f->tag); free(f->type); free(f->mark);
     free(f->cell_info); free(f->face_info); free(f->geom); free(f->src); free(f->hash);
     free(f);
 }
@@ -163,7 +169,16 @@ void amr_forest_destroy(amr_forest_t *f)
 int amr_forest_num_quads(const amr_forest_t *f) { return f->nquad; }
 int amr_forest_nelem_subgrid(const amr_forest_t *f) { return f->n; }
 const amr_quad_t *amr_forest_quads(const amr_forest_t *f) { return f->q; }
-double *amr_forest_soln(amr_forest_t *f) { return f->soln; }
+double *amr_forest_soln(amr_forest_t *f)
+{
+    /* regrids that leave the data on the device do not move patches; the
+     * array only grows here, contents of new entries undefined */
+    if (f->soln_cap < f->nquad) {
+        f->soln = (double *)realloc(f->soln, sizeof(double) * (size_t)patch_size(f) * (size_t)f->nquad);
+        f->soln_cap = f->nquad;
+    }
+    return f->soln;
+}
 int *amr_forest_tags(amr_forest_t *f) { return f->tag; }
 double amr_forest_scale(const amr_forest_t *f) { return f->scale; }
 
@@ -300,10 +315,14 @@ int amr_forest_build_external(amr_forest_t *f, int *ncell, int *nghost, int *nfa
                               int **cell_info, int **face_info, double **geom, double **soln)
 {
     int nq = f->nquad, ps = patch_size(f);
-    free(f->cell_info); free(f->face_info); free(f->geom);
-    f->cell_info = (int *)malloc(sizeof(int) * 3 * (size_t)nq);
-    f->face_info = (int *)malloc(sizeof(int) * 11 * 4 * (size_t)nq);
-    f->geom = (double *)malloc(sizeof(double) * 12 * (size_t)nq);
+    if (nq > f->ext_cap) {
+        const int c = nq + nq / 4 + 16;
+        free(f->cell_info); free(f->face_info); free(f->geom);
+        f->cell_info = (int *)malloc(sizeof(int) * 3 * (size_t)c);
+        f->face_info = (int *)malloc(sizeof(int) * 11 * 4 * (size_t)c);
+        f->geom = (double *)malloc(sizeof(double) * 12 * (size_t)c);
+        f->ext_cap = c;
+    }
     for (int k = 0; k < nq; ++k) {
         f->cell_info[3 * k + 0] = f->type[k];
         f->cell_info[3 * k + 1] = f->q[k].level;
@@ -337,24 +356,26 @@ int amr_forest_build_external(amr_forest_t *f, int *ncell, int *nghost, int *nfa
     }
     f->nface = nf;
     *ncell = nq; *nghost = 0; *nface = nf; *dof = ps * nq;
-    *cell_info = f->cell_info; *face_info = f->face_info; *geom = f->geom; *soln = f->soln;
+    *cell_info = f->cell_info; *face_info = f->face_info; *geom = f->geom; *soln = amr_forest_soln(f);
     return rc;
 }
 
 /* Replace the quadrant array by applying refine flags (one level). */
 int amr_forest_refine(amr_forest_t *f, const int *flags)
 {
+    if (!f->track || f->track_data) (void)amr_forest_soln(f);    /* patches are copied: full array */
     int nq = f->nquad, ps = patch_size(f), cnt = 0;
     for (int k = 0; k < nq; ++k) if (flags[k] && f->q[k].level < f->max_level) ++cnt;
     if (!cnt) return 0;
     int nn = nq + 3 * cnt;
     amr_quad_t *q = (amr_quad_t *)malloc(sizeof(amr_quad_t) * (size_t)nn);
     uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)nn);
-    double *soln = (double *)malloc(sizeof(double) * (size_t)ps * (size_t)nn);
+    const int move = !f->track || f->track_data;
+    double *soln = move ? (double *)malloc(sizeof(double) * (size_t)ps * (size_t)nn) : NULL;
     int *tag = (int *)malloc(sizeof(int) * (size_t)nn);
     int *type = (int *)malloc(sizeof(int) * (size_t)nn);
+    unsigned char *mark = (unsigned char *)malloc((size_t)nn);
     int *src = f->track ? (int *)malloc(sizeof(int) * (size_t)nn) : NULL;
-    const int move = !f->track || f->track_data;
     int j = 0;
     for (int k = 0; k < nq; ++k) {
         if (flags[k] && f->q[k].level < f->max_level) {
@@ -370,6 +391,7 @@ int amr_forest_refine(amr_forest_t *f, const int *flags)
                 if (src) src[j] = f->src[k];
                 tag[j] = f->tag[k];
                 type[j] = f->type[k];
+                mark[j] = 1;
                 ++j;
             }
         } else {
@@ -377,11 +399,17 @@ int amr_forest_refine(amr_forest_t *f, const int *flags)
             if (move) memcpy(&soln[(size_t)ps * j], &f->soln[(size_t)ps * k], sizeof(double) * ps);
             if (src) src[j] = f->src[k];
             tag[j] = f->tag[k]; type[j] = f->type[k];
+            mark[j] = f->mark[k];
             ++j;
         }
     }
-    free(f->q); free(f->key); free(f->soln); free(f->tag); free(f->type);
-    f->q = q; f->key = key; f->soln = soln; f->tag = tag; f->type = type;
+    free(f->q); free(f->key); free(f->tag); free(f->type); free(f->mark);
+    f->q = q; f->key = key; f->tag = tag; f->type = type; f->mark = mark;
+    if (move) {
+        free(f->soln);
+        f->soln = soln;
+        f->soln_cap = nn;
+    }
     if (src) { free(f->src); f->src = src; }
     f->nquad = nn; f->cap = nn;
     f->hvalid = 0;
@@ -405,6 +433,7 @@ static int is_family(const amr_forest_t *f, int k)
 
 int amr_forest_coarsen(amr_forest_t *f)
 {
+    if (!f->track || f->track_data) (void)amr_forest_soln(f);    /* patches are moved: full array */
     int nq = f->nquad, ps = patch_size(f), j = 0, cnt = 0;
     for (int k = 0; k < nq;) {
         int fam = is_family(f, k) && f->q[k].level > f->min_level;
@@ -438,6 +467,7 @@ int amr_forest_coarsen(amr_forest_t *f)
             f->q[j] = p; f->key[j] = morton(p.x, p.y);
             f->tag[j] = 0;
             f->type[j] = f->type[k + 3];  /* last child's type (amr_regrid.c:264-268) */
+            f->mark[j] = 2;
             ++j; k += 4; ++cnt;
         } else {
             if (j != k) {
@@ -445,7 +475,7 @@ int amr_forest_coarsen(amr_forest_t *f)
                 if (!f->track || f->track_data)
                     memmove(&f->soln[(size_t)ps * j], &f->soln[(size_t)ps * k], sizeof(double) * ps);
                 if (f->track) f->src[j] = f->src[k];
-                f->tag[j] = f->tag[k]; f->type[j] = f->type[k];
+                f->tag[j] = f->tag[k]; f->type[j] = f->type[k]; f->mark[j] = f->mark[k];
             }
             ++j; ++k;
         }
@@ -500,6 +530,76 @@ int amr_forest_is_balanced(amr_forest_t *f)
     int any = balance_flags(f, flags);
     free(flags);
     return !any;
+}
+
+/* Balance after a regrid of a balanced forest: only quadrants this regrid
+ * created can take part in a violation.  Refined children (mark 1) look for
+ * neighbours more than one level coarser; coarsened parents (mark 2) for
+ * neighbours two or more levels finer, sampled at the quarter points of each
+ * side and the four corners (a leaf two levels finer is a quarter side
+ * long).  Each round refines what it flagged; the next round's worklist is
+ * the new children and the quadrants that flagged a neighbour (one
+ * refinement may not be enough).  The result is the same unique balanced
+ * closure as amr_forest_balance's full passes. */
+int amr_forest_balance_marked(amr_forest_t *f)
+{
+    int total = 0;
+    for (;;) {
+        const int nq = f->nquad;
+        int *flags = (int *)calloc((size_t)(nq > 0 ? nq : 1), sizeof(int));
+        int any = 0;
+        for (int k = 0; k < nq; ++k) {
+            if (!f->mark[k]) continue;
+            const amr_quad_t q = f->q[k];
+            int flagged = 0;
+            if (q.level >= 2) {
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        if (!dx && !dy) continue;
+                        int t, nx, ny;
+                        if (!neighbour_region(f, &q, dx, dy, &t, &nx, &ny)) continue;
+                        const int L = leaf_near(f, t, nx, ny, q.level);
+                        if (L >= 0 && f->q[L].level < q.level - 1) { flags[L] = 1; any = 1; flagged = 1; }
+                    }
+            }
+            if (f->mark[k] == 2 && !flags[k]) {
+                const int len = qlen(q.level);
+                int found = 0;
+                for (int dy = -1; dy <= 1 && !found; ++dy)
+                    for (int dx = -1; dx <= 1 && !found; ++dx) {
+                        if (!dx && !dy) continue;
+                        int t, nx, ny;
+                        if (!neighbour_region(f, &q, dx, dy, &t, &nx, &ny)) continue;
+                        const int px = dx > 0 ? nx : nx + len - 1, py = dy > 0 ? ny : ny + len - 1;
+                        const int ns = (dx && dy) ? 1 : 4;       /* corner: one point; side: four */
+                        for (int i = 0; i < ns && !found; ++i) {
+                            const int off = (2 * i + 1) * (len / 8);
+                            const int sx = dx ? px : nx + off, sy = dy ? py : ny + off;
+                            const int L = leaf_near(f, t, sx, sy, q.level + 2);
+                            if (L >= 0 && f->q[L].level >= q.level + 2) found = 1;
+                        }
+                    }
+                if (found) { flags[k] = 1; any = 1; }
+            }
+            /* settled unless a neighbour it flagged may still be too coarse
+             * after one refinement (refined quadrants' children are marked) */
+            if (!flagged) f->mark[k] = 0;
+        }
+        if (any) {
+            int save = f->max_level;
+            f->max_level = AMR_MAXLEVEL - 1;
+            total += amr_forest_refine(f, flags);    /* marks its children: the next worklist */
+            f->max_level = save;
+        }
+        free(flags);
+        if (!any) break;
+    }
+    return total;
+}
+
+void amr_forest_clear_marks(amr_forest_t *f)
+{
+    memset(f->mark, 0, (size_t)(f->nquad > 0 ? f->nquad : 1));
 }
 
 void amr_forest_track_begin(amr_forest_t *f, int keep_data)
@@ -570,6 +670,7 @@ amr_forest_t *amr_forest_from_quads(int ntx, int nty, int periodic_x, int period
         f->key[k] = morton(quads[k].x, quads[k].y);
         f->tag[k] = 0;
         f->type[k] = 0;
+        f->mark[k] = 0;
     }
     f->nquad = nquad;
     f->hvalid = 0;

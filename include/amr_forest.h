@@ -72,6 +72,11 @@ int amr_forest_coarsen(amr_forest_t *f);
 /* 2:1 balance across faces and corners (P4EST_CONNECT_FULL); refined
  * quadrants copy their parent patch.  Returns quadrants refined. */
 int amr_forest_balance(amr_forest_t *f);
+/* The same closure after a regrid of a balanced forest, examining only the
+ * quadrants created by refine / coarsen since amr_forest_clear_marks (their
+ * neighbours are the only possible violations).  Returns quadrants refined. */
+int amr_forest_balance_marked(amr_forest_t *f);
+void amr_forest_clear_marks(amr_forest_t *f);
 /* 1 if the forest is 2:1 balanced across faces and corners. */
 int amr_forest_is_balanced(amr_forest_t *f);
 

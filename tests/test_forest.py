@@ -182,3 +182,28 @@ def test_tracking_without_data_gives_same_topology():
     fa.track_begin(keep_data=True); fa.regrid_tags(tags); oka, sa = fa.track_end()
     fb.track_begin(keep_data=False); fb.regrid_tags(tags); okb, sb = fb.track_end()
     assert oka and okb and np.array_equal(sa, sb) and fa.quads() == fb.quads()
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_worklist_balance_equals_full_balance(seed):
+    """amr_regrid_tags balances only around what it refined / coarsened; the
+    result must be the same closure as full passes over every quadrant"""
+    rng = np.random.default_rng(500 + seed)
+    per = (bool(seed % 2), bool(seed % 3 == 0))
+    f = amrhip.Forest(3, 2, per, 0.0, 0.0, 1.0, 4, 1, 7)
+    for _ in range(4):
+        f.refine((rng.uniform(size=f.nquad) < 0.25).astype(np.int32))
+        f.balance()
+    L = amrhip.lib()
+    for step in range(4):
+        tags = rng.choice([0, 1, 2], size=f.nquad, p=[0.6, 0.15, 0.25]).astype(np.int32)
+        g = amrhip.Forest.from_quads(3, 2, per, 0.0, 0.0, 1.0, 4, 1, 7, f.quads())
+        # reference: the same refine / coarsen, then full-pass balance
+        gt = g.tags()
+        gt[:] = np.where(tags == 1, 1, np.where(tags != 0, 2, 0))
+        g.refine((tags == 1).astype(np.int32))
+        L.amr_forest_coarsen(g.h)
+        g.balance()
+        f.regrid_tags(tags, allow_coarsening=True)
+        assert f.is_balanced()
+        assert f.quads() == g.quads(), (seed, step)
