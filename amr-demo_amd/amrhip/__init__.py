@@ -68,6 +68,8 @@ EXPORTS = {
     "amr_forest_is_balanced": (_i, [_vp]),
     "amr_forest_find_leaf": (_i, [_vp, _i, _i, _i]),
     "amr_forest_brick": (None, [_vp, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pi, _pi]),
+    "amr_forest_balance_marked": (_i, [_vp]),
+    "amr_forest_clear_marks": (None, [_vp]),
     "amr_forest_from_quads": (_vp, [_i, _i, _i, _i, _d, _d, _d, _i, _i, _i, _i, _vp]),
     "amr_forest_save": (_i, [_vp, C.c_char_p, _i, _d]),
     "amr_forest_load": (_vp, [C.c_char_p, _pi, _pd]),
