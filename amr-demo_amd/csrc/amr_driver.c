@@ -680,6 +680,7 @@ typedef struct {
 static void *regrid_worker(void *arg)
 {
     regrid_job_t *j = (regrid_job_t *)arg;
+    amr_forest_host_parallel(0);       /* the instances are the parallelism here */
     for (;;) {
         pthread_mutex_lock(&j->lock);
         const int k = j->next++;

@@ -7,6 +7,7 @@
  * Neighbour queries are binary searches for the leaf containing a point.
  */
 #include "amr_forest.h"
+#include "amr_internal.h"
 
 #include <stdint.h>
 #include <stdio.h>
@@ -30,6 +31,8 @@ struct amr_forest {
     int *cell_info, *face_info;
     double *geom;
     int nface, ext_cap;
+    int *ext_rec;         /* per quadrant up to 4 face records (parallel build) */
+    int ext_rec_cap;
     /* regrid tracking: source of every quadrant's patch (see track_begin) */
     int *src;
     int track, track_data, track_ok;
@@ -46,6 +49,13 @@ struct hentry {
 };
 
 static inline int qlen(int level) { return 1 << (AMR_MAXLEVEL - level); }
+
+/* Host threads for the per-quadrant loops (OpenMP), off inside the batch's
+ * own per-instance threads. */
+static _Thread_local int t_parallel = 1;
+#define PAR_MIN 1024
+
+void amr_forest_host_parallel(int on) { t_parallel = on; }
 
 static inline uint64_t spread(uint32_t v)
 {
@@ -162,7 +172,7 @@ void amr_forest_destroy(amr_forest_t *f)
 {
     if (!f) return;
     free(f->q); free(f->key); free(f->soln); free(f->tag); free(f->type); free(f->mark);
-    free(f->cell_info); free(f->face_info); free(f->geom); free(f->src); free(f->hash);
+    free(f->cell_info); free(f->face_info); free(f->geom); free(f->src); free(f->hash); free(f->ext_rec);
     free(f);
 }
 
@@ -323,21 +333,30 @@ int amr_forest_build_external(amr_forest_t *f, int *ncell, int *nghost, int *nfa
         f->geom = (double *)malloc(sizeof(double) * 12 * (size_t)c);
         f->ext_cap = c;
     }
+    if (nq + 1 > f->ext_rec_cap) {
+        const int c = nq + nq / 4 + 16;
+        free(f->ext_rec);
+        f->ext_rec = (int *)malloc(sizeof(int) * 45 * (size_t)c);    /* 4 records of 11 + count */
+        f->ext_rec_cap = c;
+    }
+    if (!f->hvalid) hash_build(f);               /* shared read-only below */
+    int err = 0;
+    /* each quadrant's sides, in side order, into its own slots */
+#pragma omp parallel for schedule(static) reduction(| : err) if (nq >= PAR_MIN && t_parallel)
     for (int k = 0; k < nq; ++k) {
         f->cell_info[3 * k + 0] = f->type[k];
         f->cell_info[3 * k + 1] = f->q[k].level;
         f->cell_info[3 * k + 2] = ps * k;
         amr_forest_quad_geom(f, k, &f->geom[12 * k]);
-    }
-    int nf = 0, rc = 0;
-    for (int k = 0; k < nq; ++k) {
+        int *rec = &f->ext_rec[45 * k];
+        int cnt = 0;
         for (int s = 0; s < 4; ++s) {
             int nb[2] = {-1, -1};
             int c = classify_side(f, k, s, nb);
-            int *fi = &f->face_info[11 * nf];
-            if (c < 0) { rc = -1; continue; }
+            if (c < 0) { err = 1; continue; }
             if (c == 2) continue;                      /* emitted by the coarse side */
             if (c == 1 && (s % 2) == 0) continue;      /* emitted from the odd side */
+            int *fi = &rec[1 + 11 * cnt];
             for (int e = 0; e < 11; ++e) fi[e] = -1;
             fi[1] = s;
             fi[2] = k;
@@ -351,13 +370,27 @@ int amr_forest_build_external(amr_forest_t *f, int *ncell, int *nghost, int *nfa
                 fi[3] = nb[0]; fi[4] = nb[1];
                 fi[7] = s ^ 1; fi[8] = s ^ 1;
             }
-            ++nf;
+            ++cnt;
         }
+        rec[0] = cnt;
+    }
+    /* quadrant order, as the serial loop */
+    int nf = 0;
+    for (int k = 0; k < nq; ++k) {
+        const int cnt = f->ext_rec[45 * k];
+        f->ext_rec[45 * k] = nf;                      /* now: first face of quadrant k */
+        nf += cnt;
+    }
+#pragma omp parallel for schedule(static) if (nq >= PAR_MIN && t_parallel)
+    for (int k = 0; k < nq; ++k) {
+        const int first = f->ext_rec[45 * k];
+        const int cnt = (k + 1 < nq ? f->ext_rec[45 * (k + 1)] : nf) - first;
+        memcpy(&f->face_info[11 * (size_t)first], &f->ext_rec[45 * k + 1], sizeof(int) * 11 * (size_t)cnt);
     }
     f->nface = nf;
     *ncell = nq; *nghost = 0; *nface = nf; *dof = ps * nq;
     *cell_info = f->cell_info; *face_info = f->face_info; *geom = f->geom; *soln = amr_forest_soln(f);
-    return rc;
+    return err ? -1 : 0;
 }
 
 /* Replace the quadrant array by applying refine flags (one level). */
@@ -489,6 +522,8 @@ static int balance_flags(amr_forest_t *f, int *flags)
 {
     int any = 0;
     memset(flags, 0, sizeof(int) * (size_t)f->nquad);
+    if (!f->hvalid) hash_build(f);
+#pragma omp parallel for schedule(static) reduction(| : any) if (f->nquad >= PAR_MIN && t_parallel)
     for (int k = 0; k < f->nquad; ++k) {
         const amr_quad_t *q = &f->q[k];
         if (q->level < 2) continue;
@@ -498,7 +533,7 @@ static int balance_flags(amr_forest_t *f, int *flags)
                 int t, nx, ny;
                 if (!neighbour_region(f, q, dx, dy, &t, &nx, &ny)) continue;
                 int L = leaf_near(f, t, nx, ny, q->level);
-                if (L >= 0 && f->q[L].level < q->level - 1) { flags[L] = 1; any = 1; }
+                if (L >= 0 && f->q[L].level < q->level - 1) { __atomic_store_n(&flags[L], 1, __ATOMIC_RELAXED); any = 1; }
             }
     }
     return any;
@@ -532,69 +567,286 @@ int amr_forest_is_balanced(amr_forest_t *f)
     return !any;
 }
 
-/* Balance after a regrid of a balanced forest: only quadrants this regrid
- * created can take part in a violation.  Refined children (mark 1) look for
- * neighbours more than one level coarser; coarsened parents (mark 2) for
- * neighbours two or more levels finer, sampled at the quarter points of each
- * side and the four corners (a leaf two levels finer is a quarter side
- * long).  Each round refines what it flagged; the next round's worklist is
- * the new children and the quadrants that flagged a neighbour (one
- * refinement may not be enough).  The result is the same unique balanced
- * closure as amr_forest_balance's full passes. */
+/* ---- balance after a regrid, as a worklist over a dynamic node set ------
+ * Only quadrants this regrid created (marks: 1 refined child, 2 coarsened
+ * parent) and what balance refines can take part in a violation of a forest
+ * that was balanced before.  The tree nodes -- leaves and their ancestors --
+ * live in an open-addressing set keyed by (tree, level, corner) with a leaf
+ * flag, so each neighbour question is one lookup: the leaf containing a point
+ * is coarser than level l exactly when no node of level l contains it, and
+ * finer than l exactly when the node of level l containing it is internal.
+ * Refining a leaf flips its flag and inserts its four children, so the ripple
+ * needs no array rebuilds.  The Morton-ordered arrays are emitted once at the
+ * end by expanding every refined original quadrant into its leaves (children
+ * in z order).  The result is the same unique balanced closure as
+ * amr_forest_balance's full passes. */
+typedef struct {
+    uint64_t key;
+    int meta;          /* tree * 32 + level; -1 empty */
+    int leaf;
+} nodeent_t;
+
+typedef struct {
+    nodeent_t *e;
+    uint64_t mask;
+    int used;
+} nodeset_t;
+
+static nodeent_t *ns_find(const nodeset_t *S, int meta, uint64_t key)
+{
+    for (uint64_t h = hmix(key, meta) & S->mask;; h = (h + 1) & S->mask) {
+        nodeent_t *e = &S->e[h];
+        if (e->meta == -1) return NULL;
+        if (e->meta == meta && e->key == key) return e;
+    }
+}
+
+static void ns_alloc(nodeset_t *S, uint64_t cap)
+{
+    S->e = (nodeent_t *)malloc(sizeof(nodeent_t) * (size_t)cap);
+    for (uint64_t i = 0; i < cap; ++i) S->e[i].meta = -1;
+    S->mask = cap - 1;
+    S->used = 0;
+}
+
+static void ns_put(nodeset_t *S, int meta, uint64_t key, int leaf)
+{
+    uint64_t h = hmix(key, meta) & S->mask;
+    while (S->e[h].meta != -1) h = (h + 1) & S->mask;
+    S->e[h].key = key;
+    S->e[h].meta = meta;
+    S->e[h].leaf = leaf;
+    S->used++;
+}
+
+/* insert a node known to be absent, keeping the load at most 1/2 */
+static void ns_insert(nodeset_t *S, int meta, uint64_t key, int leaf)
+{
+    if (2 * (uint64_t)(S->used + 1) > S->mask + 1) {
+        nodeset_t T;
+        ns_alloc(&T, 2 * (S->mask + 1));
+        for (uint64_t i = 0; i <= S->mask; ++i)
+            if (S->e[i].meta != -1) ns_put(&T, S->e[i].meta, S->e[i].key, S->e[i].leaf);
+        free(S->e);
+        *S = T;
+    }
+    ns_put(S, meta, key, leaf);
+}
+
+/* the node of level l containing (tree, x, y), or NULL */
+static inline nodeent_t *ns_at(const nodeset_t *S, int tree, int l, int x, int y)
+{
+    const int len = qlen(l);
+    return ns_find(S, tree * 32 + l, morton(x & ~(len - 1), y & ~(len - 1)));
+}
+
+typedef struct {
+    amr_quad_t *q;
+    unsigned char *kind;   /* 1: check for too-coarse neighbours, 2: also too-fine */
+    int n, cap;
+} qqueue_t;
+
+static void qq_push(qqueue_t *Q, amr_quad_t q, unsigned char kind)
+{
+    if (Q->n == Q->cap) {
+        Q->cap = Q->cap ? 2 * Q->cap : 256;
+        Q->q = (amr_quad_t *)realloc(Q->q, sizeof(amr_quad_t) * (size_t)Q->cap);
+        Q->kind = (unsigned char *)realloc(Q->kind, (size_t)Q->cap);
+    }
+    Q->q[Q->n] = q;
+    Q->kind[Q->n] = kind;
+    Q->n++;
+}
+
+/* neighbour_region with q's tree coordinates (tx, ty) already known */
+static inline int near_region(const amr_forest_t *f, const amr_quad_t *q, int tx, int ty, int dx, int dy,
+                              int *tree, int *nx, int *ny)
+{
+    const int len = qlen(q->level);
+    int x = q->x + dx * len, y = q->y + dy * len;
+    if (x < 0) { x += AMR_ROOT_LEN; tx -= 1; }
+    else if (x >= AMR_ROOT_LEN) { x -= AMR_ROOT_LEN; tx += 1; }
+    if (y < 0) { y += AMR_ROOT_LEN; ty -= 1; }
+    else if (y >= AMR_ROOT_LEN) { y -= AMR_ROOT_LEN; ty += 1; }
+    if (tx < 0 || tx >= f->ntx) {
+        if (!f->periodic[0]) return 0;
+        tx = tx < 0 ? tx + f->ntx : tx - f->ntx;
+    }
+    if (ty < 0 || ty >= f->nty) {
+        if (!f->periodic[1]) return 0;
+        ty = ty < 0 ? ty + f->nty : ty - f->nty;
+    }
+    *tree = ty * f->ntx + tx;
+    *nx = x; *ny = y;
+    return 1;
+}
+
+static void refine_leaf(nodeset_t *S, qqueue_t *Q, nodeent_t *e, amr_quad_t q, int *count)
+{
+    e->leaf = 0;     /* before the inserts, which may move the table */
+    const int h = qlen(q.level + 1);
+    for (int c = 0; c < 4; ++c) {
+        amr_quad_t ch = q;
+        ch.level = q.level + 1;
+        ch.x = q.x + (c & 1) * h;
+        ch.y = q.y + (c >> 1) * h;
+        ns_insert(S, ch.tree * 32 + ch.level, morton(ch.x, ch.y), 1);
+        qq_push(Q, ch, 1);
+    }
+    (*count)++;
+}
+
+/* emission of the balanced arrays */
+typedef struct {
+    amr_quad_t *q;
+    uint64_t *key;
+    int *tag, *type, *src;
+    unsigned char *mark;
+    double *soln;
+    int n, cap, move, ps;
+} emit_t;
+
+static void emit_leaves(const amr_forest_t *f, const nodeset_t *S, emit_t *E, amr_quad_t q, int k)
+{
+    if (!ns_find(S, q.tree * 32 + q.level, morton(q.x, q.y))->leaf) {
+        const int h = qlen(q.level + 1);
+        for (int c = 0; c < 4; ++c) {
+            amr_quad_t ch = q;
+            ch.level = q.level + 1;
+            ch.x = q.x + (c & 1) * h;
+            ch.y = q.y + (c >> 1) * h;
+            emit_leaves(f, S, E, ch, k);
+        }
+        return;
+    }
+    if (E->n == E->cap) {
+        E->cap *= 2;
+        E->q = (amr_quad_t *)realloc(E->q, sizeof(amr_quad_t) * (size_t)E->cap);
+        E->key = (uint64_t *)realloc(E->key, sizeof(uint64_t) * (size_t)E->cap);
+        E->tag = (int *)realloc(E->tag, sizeof(int) * (size_t)E->cap);
+        E->type = (int *)realloc(E->type, sizeof(int) * (size_t)E->cap);
+        E->mark = (unsigned char *)realloc(E->mark, (size_t)E->cap);
+        if (E->src) E->src = (int *)realloc(E->src, sizeof(int) * (size_t)E->cap);
+        if (E->move) E->soln = (double *)realloc(E->soln, sizeof(double) * (size_t)E->ps * (size_t)E->cap);
+    }
+    const int j = E->n++;
+    E->q[j] = q;
+    E->key[j] = morton(q.x, q.y);
+    E->tag[j] = f->tag[k];            /* balance-refined children inherit, as refine */
+    E->type[j] = f->type[k];
+    E->mark[j] = 0;
+    if (E->src) E->src[j] = f->src[k];
+    if (E->move) memcpy(&E->soln[(size_t)E->ps * j], &f->soln[(size_t)E->ps * k], sizeof(double) * (size_t)E->ps);
+}
+
 int amr_forest_balance_marked(amr_forest_t *f)
 {
-    int total = 0;
-    for (;;) {
-        const int nq = f->nquad;
-        int *flags = (int *)calloc((size_t)(nq > 0 ? nq : 1), sizeof(int));
-        int any = 0;
-        for (int k = 0; k < nq; ++k) {
-            if (!f->mark[k]) continue;
-            const amr_quad_t q = f->q[k];
-            int flagged = 0;
-            if (q.level >= 2) {
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        if (!dx && !dy) continue;
-                        int t, nx, ny;
-                        if (!neighbour_region(f, &q, dx, dy, &t, &nx, &ny)) continue;
-                        const int L = leaf_near(f, t, nx, ny, q.level);
-                        if (L >= 0 && f->q[L].level < q.level - 1) { flags[L] = 1; any = 1; flagged = 1; }
-                    }
-            }
-            if (f->mark[k] == 2 && !flags[k]) {
-                const int len = qlen(q.level);
-                int found = 0;
-                for (int dy = -1; dy <= 1 && !found; ++dy)
-                    for (int dx = -1; dx <= 1 && !found; ++dx) {
-                        if (!dx && !dy) continue;
-                        int t, nx, ny;
-                        if (!neighbour_region(f, &q, dx, dy, &t, &nx, &ny)) continue;
-                        const int px = dx > 0 ? nx : nx + len - 1, py = dy > 0 ? ny : ny + len - 1;
-                        const int ns = (dx && dy) ? 1 : 4;       /* corner: one point; side: four */
-                        for (int i = 0; i < ns && !found; ++i) {
-                            const int off = (2 * i + 1) * (len / 8);
-                            const int sx = dx ? px : nx + off, sy = dy ? py : ny + off;
-                            const int L = leaf_near(f, t, sx, sy, q.level + 2);
-                            if (L >= 0 && f->q[L].level >= q.level + 2) found = 1;
+    const int nq = f->nquad;
+    nodeset_t S;
+    uint64_t cap = 16;
+    while (cap < 4 * (uint64_t)nq) cap <<= 1;
+    ns_alloc(&S, cap);
+    qqueue_t Q = {NULL, NULL, 0, 0};
+    for (int k = 0; k < nq; ++k) {
+        const amr_quad_t *q = &f->q[k];
+        ns_insert(&S, q->tree * 32 + q->level, f->key[k], 1);
+        /* ancestors up to the first one already present; in Morton order
+         * the parent of a child other than the first is present already */
+        const int own = q->level > 0 ? qlen(q->level) : 0;
+        for (int l = (q->x | q->y) & own ? -1 : q->level - 1; l >= 0; --l) {
+            const int len = qlen(l);
+            const int meta = q->tree * 32 + l;
+            const uint64_t key = morton(q->x & ~(len - 1), q->y & ~(len - 1));
+            if (ns_find(&S, meta, key)) break;
+            ns_insert(&S, meta, key, 0);
+        }
+        if (f->mark[k]) qq_push(&Q, *q, f->mark[k] == 2 ? 2 : 1);
+    }
+    int refined = 0;
+    for (int it = 0; it < Q.n; ++it) {
+        const amr_quad_t q = Q.q[it];
+        const unsigned char kind = Q.kind[it];
+        const nodeent_t *self = ns_find(&S, q.tree * 32 + q.level, morton(q.x, q.y));
+        if (!self->leaf) continue;                                             /* refined since */
+        int recheck = 0, too_coarse_self = 0;
+        const int len = qlen(q.level);
+        /* q's parent is not a leaf, so every leaf inside it is at q's level
+         * or finer: the directions towards the siblings are never too coarse */
+        const int sx_dir = (q.x & len) ? -1 : 1, sy_dir = (q.y & len) ? -1 : 1;
+        const int qtx = q.tree % f->ntx, qty = q.tree / f->ntx;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (!dx && !dy) continue;
+                const int sibling = q.level > 0 && (!dx || dx == sx_dir) && (!dy || dy == sy_dir);
+                if (sibling && kind != 2) continue;
+                int t, nx, ny;
+                if (!near_region(f, &q, qtx, qty, dx, dy, &t, &nx, &ny)) continue;
+                if (!sibling && q.level >= 2 && !ns_at(&S, t, q.level - 1, nx, ny)) {
+                    /* the neighbour leaf containing (nx, ny) is coarser than
+                     * level - 1: find it and refine it */
+                    for (int l = q.level - 2; l >= 0; --l) {
+                        nodeent_t *e = ns_at(&S, t, l, nx, ny);
+                        if (e && e->leaf) {
+                            const int ln = qlen(l);
+                            amr_quad_t L = {t, l, nx & ~(ln - 1), ny & ~(ln - 1)};
+                            refine_leaf(&S, &Q, e, L, &refined);
+                            recheck = 1;
+                            break;
                         }
                     }
-                if (found) { flags[k] = 1; any = 1; }
+                }
+                if (kind == 2 && !too_coarse_self) {
+                    /* a neighbour two or more levels finer touches q when one
+                     * of the level + 1 nodes along the side (two; one at a
+                     * corner) is internal: one point in each */
+                    const int px = dx > 0 ? nx : nx + len - 1, py = dy > 0 ? ny : ny + len - 1;
+                    const int ns = (dx && dy) ? 1 : 2;
+                    for (int i = 0; i < ns && !too_coarse_self; ++i) {
+                        const int off = i * (len / 2);
+                        const int sx = dx ? px : nx + off, sy = dy ? py : ny + off;
+                        const nodeent_t *e = ns_at(&S, t, q.level + 1, sx, sy);
+                        if (e && !e->leaf) too_coarse_self = 1;
+                    }
+                }
             }
-            /* settled unless a neighbour it flagged may still be too coarse
-             * after one refinement (refined quadrants' children are marked) */
-            if (!flagged) f->mark[k] = 0;
+        if (too_coarse_self) {
+            nodeent_t *e = ns_find(&S, q.tree * 32 + q.level, morton(q.x, q.y));
+            refine_leaf(&S, &Q, e, q, &refined);
+        } else if (recheck) {
+            qq_push(&Q, q, 1);
         }
-        if (any) {
-            int save = f->max_level;
-            f->max_level = AMR_MAXLEVEL - 1;
-            total += amr_forest_refine(f, flags);    /* marks its children: the next worklist */
-            f->max_level = save;
-        }
-        free(flags);
-        if (!any) break;
     }
-    return total;
+    if (refined) {
+        emit_t E;
+        memset(&E, 0, sizeof(E));
+        E.cap = nq + 3 * refined + 16;
+        E.q = (amr_quad_t *)malloc(sizeof(amr_quad_t) * (size_t)E.cap);
+        E.key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)E.cap);
+        E.tag = (int *)malloc(sizeof(int) * (size_t)E.cap);
+        E.type = (int *)malloc(sizeof(int) * (size_t)E.cap);
+        E.mark = (unsigned char *)malloc((size_t)E.cap);
+        E.src = f->track ? (int *)malloc(sizeof(int) * (size_t)E.cap) : NULL;
+        E.move = !f->track || f->track_data;
+        E.ps = patch_size(f);
+        if (E.move) {
+            (void)amr_forest_soln(f);
+            E.soln = (double *)malloc(sizeof(double) * (size_t)E.ps * (size_t)E.cap);
+        }
+        for (int k = 0; k < nq; ++k) emit_leaves(f, &S, &E, f->q[k], k);
+        free(f->q); free(f->key); free(f->tag); free(f->type); free(f->mark);
+        f->q = E.q; f->key = E.key; f->tag = E.tag; f->type = E.type; f->mark = E.mark;
+        if (E.src) { free(f->src); f->src = E.src; }
+        if (E.move) { free(f->soln); f->soln = E.soln; f->soln_cap = E.cap; }
+        f->nquad = E.n;
+        f->cap = E.cap;
+        f->hvalid = 0;
+    } else {
+        memset(f->mark, 0, (size_t)(nq > 0 ? nq : 1));
+    }
+    free(S.e);
+    free(Q.q);
+    free(Q.kind);
+    return refined;
 }
 
 void amr_forest_clear_marks(amr_forest_t *f)
