@@ -151,27 +151,43 @@ __device__ __forceinline__ void make_prim(double r, double ru, double rv, double
     P.sr = sr;
 }
 
+// A face's primitive record oriented along the face normal: n is the normal
+// direction (x for an x face), t the tangential one.
+struct OPrim {
+    double r, rn, rt, rE, un, ut, p, H, sr;
+};
+
+__device__ __forceinline__ OPrim orient(const Prim &P, bool xdir)
+{
+    OPrim o;
+    o.r = P.r; o.rE = P.rE; o.p = P.p; o.H = P.H; o.sr = P.sr;
+    o.rn = xdir ? P.ru : P.rv;
+    o.rt = xdir ? P.rv : P.ru;
+    o.un = xdir ? P.u : P.v;
+    o.ut = xdir ? P.v : P.u;
+    return o;
+}
+
 // HLLE-type flux of flux_2d_module.F90:63-182 on an axis-aligned face of
-// length h with unit normal +x (XDIR) or +y.  sqrt(rho_l*rho_r) is taken as
+// length h, in the face's own frame (unit normal +n): F[0] mass, F[1] normal
+// and F[2] tangential momentum, F[3] energy.  x and y faces are the same
+// arithmetic on oriented records, so both sides of a face and both
+// directions round alike.  sqrt(rho_l*rho_r) is taken as
 // sqrt(rho_l)*sqrt(rho_r), abar and 1/abar come from one rsq, and the
 // divisions become products with refined reciprocals: all within a few ulp
 // of the reference's correctly rounded operations.
-template <bool XDIR>
-__device__ __forceinline__ void flux_face(const Prim &L, const Prim &R, double h, double F[4])
+__device__ __forceinline__ void flux_normal(const OPrim &L, const OPrim &R, double h, double F[4])
 {
-    double unl = XDIR ? L.u : L.v;
-    double unr = XDIR ? R.u : R.v;
     double inv_rf = frcp(L.sr + R.sr);
     double rhobar = L.sr * R.sr;
-    double ubar = (L.u * L.sr + R.u * R.sr) * inv_rf;
-    double vbar = (L.v * L.sr + R.v * R.sr) * inv_rf;
+    double unbar = (L.un * L.sr + R.un * R.sr) * inv_rf;
+    double utbar = (L.ut * L.sr + R.ut * R.sr) * inv_rf;
     double Hbar = (L.H * L.sr + R.H * R.sr) * inv_rf;
     double abar, inva;
-    fsqrt_rsq(0.4 * (Hbar - 0.5 * (ubar * ubar + vbar * vbar)), abar, inva);
-    double unbar = XDIR ? ubar : vbar;
+    fsqrt_rsq(0.4 * (Hbar - 0.5 * (unbar * unbar + utbar * utbar)), abar, inva);
     double Mn = unbar * inva;
     double eigp = unbar + abar, eigm = unbar - abar;
-    double eigp_max = unr + abar, eigm_min = unl - abar;
+    double eigp_max = R.un + abar, eigm_min = L.un - abar;
     double b1 = (eigp > 0.0 || eigp_max > 0.0) ? fmax(eigp, eigp_max) : 0.0;
     double b2 = (eigm < 0.0 || eigm_min < 0.0) ? fmin(eigm, eigm_min) : 0.0;
     double invb = frcp(b1 - b2);
@@ -180,26 +196,41 @@ __device__ __forceinline__ void flux_face(const Prim &L, const Prim &R, double h
     double Mc = frcp(1.0 + fabs(Mn));
     double bM = bcoeff * Mc;
     double dH = R.H - L.H;
-    double fl0, fr0, fl1, fr1, fl2, fr2, d1, d2;
-    if (XDIR) {
-        fl0 = L.ru;               fr0 = R.ru;
-        fl1 = L.ru * L.u + L.p;   fr1 = R.ru * R.u + R.p;
-        fl2 = L.ru * L.v;         fr2 = R.ru * R.v;
-        d1 = delta1 * ubar;                          // shear term vanishes for n = +x
-        d2 = delta1 * vbar + rhobar * (R.v - L.v);
-    } else {
-        fl0 = L.rv;               fr0 = R.rv;
-        fl1 = L.rv * L.u;         fr1 = R.rv * R.u;
-        fl2 = L.rv * L.v + L.p;   fr2 = R.rv * R.v + R.p;
-        d1 = delta1 * ubar + rhobar * (R.u - L.u);
-        d2 = delta1 * vbar;
-    }
-    double fl3 = (L.rE + L.p) * unl, fr3 = (R.rE + R.p) * unr;
-    F[0] = ((b1 * fl0 - b2 * fr0) * invb + (bcoeff * (R.r - L.r) - bM * delta1)) * h;
-    F[1] = ((b1 * fl1 - b2 * fr1) * invb + (bcoeff * (R.ru - L.ru) - bM * d1)) * h;
-    F[2] = ((b1 * fl2 - b2 * fr2) * invb + (bcoeff * (R.rv - L.rv) - bM * d2)) * h;
+    double fl1 = L.rn * L.un + L.p, fr1 = R.rn * R.un + R.p;
+    double fl2 = L.rn * L.ut, fr2 = R.rn * R.ut;
+    double d1 = delta1 * unbar;                           // the shear term vanishes along n
+    double d2 = delta1 * utbar + rhobar * (R.ut - L.ut);
+    double fl3 = (L.rE + L.p) * L.un, fr3 = (R.rE + R.p) * R.un;
+    F[0] = ((b1 * L.rn - b2 * R.rn) * invb + (bcoeff * (R.r - L.r) - bM * delta1)) * h;
+    F[1] = ((b1 * fl1 - b2 * fr1) * invb + (bcoeff * (R.rn - L.rn) - bM * d1)) * h;
+    F[2] = ((b1 * fl2 - b2 * fr2) * invb + (bcoeff * (R.rt - L.rt) - bM * d2)) * h;
     F[3] = ((b1 * fl3 - b2 * fr3) * invb +
             (bcoeff * (R.r * R.H - L.r * L.H) - bM * (delta1 * Hbar + rhobar * dH))) * h;
+}
+
+// Flux through an x face (XDIR) or y face in the (x, y) frame.
+template <bool XDIR>
+__device__ __forceinline__ void flux_face(const Prim &L, const Prim &R, double h, double F[4])
+{
+    double G[4];
+    flux_normal(orient(L, XDIR), orient(R, XDIR), h, G);
+    F[0] = G[0];
+    F[1] = XDIR ? G[1] : G[2];
+    F[2] = XDIR ? G[2] : G[1];
+    F[3] = G[3];
+}
+
+// The same with the direction known only at run time (lanes of one wave
+// may differ): one flux evaluation on selected operands instead of a
+// divergent pair.
+__device__ __forceinline__ void flux_face_dir(const Prim &L, const Prim &R, double h, bool xdir, double F[4])
+{
+    double G[4];
+    flux_normal(orient(L, xdir), orient(R, xdir), h, G);
+    F[0] = G[0];
+    F[1] = xdir ? G[1] : G[2];
+    F[2] = xdir ? G[2] : G[1];
+    F[3] = G[3];
 }
 
 // Physical boundary flux: bc_2d (bc_2d_module.F90:14-75) then flux_bc_2d
@@ -379,13 +410,25 @@ template <int N, int PPB>
 struct Tile {
     static constexpr int N2 = N * N;
     static constexpr int BLK = N2 * PPB;
-    static constexpr int T = N + 2;
-    static constexpr int TT = T * T;
-    // LDS record per tile position: r, ru, rv, rE, p, 1/r, sqrt(r).  Tile
-    // positions [0, TT) are the patch with its ghost ring, [TT, TT + 4N) the
-    // second fine neighbour of each coarse side of a hanging face.
+    // LDS record per tile position: r, ru, rv, rE, p, 1/r, sqrt(r), one
+    // array per quantity.  Positions [0, N2) are the patch cells row by row
+    // (pitch N: the lanes of a wave read consecutive addresses, free of bank
+    // conflicts), [N2, N2 + 4N) the ghost cell of each side s = 0..3 (-x, +x,
+    // -y, +y) at N2 + s N + k, [N2 + 4N, N2 + 8N) the second fine neighbour
+    // of each coarse side of a hanging face at the same offsets.
     static constexpr int NP = 7;
-    static constexpr int NPOS = TT + 4 * N;
+    static constexpr int GHOST = N2;
+    static constexpr int SECOND = N2 + 4 * N;
+    static constexpr int NPOS = N2 + 8 * N;
+    // tile position of cell (i, j), -1 <= i, j <= N, not a corner
+    static __device__ __forceinline__ int pos(int i, int j)
+    {
+        if (i < 0) return GHOST + 0 * N + j;
+        if (i >= N) return GHOST + 1 * N + j;
+        if (j < 0) return GHOST + 2 * N + i;
+        if (j >= N) return GHOST + 3 * N + i;
+        return j * N + i;
+    }
     static constexpr int PRIM_PER_PATCH = NP * NPOS;
     static constexpr int FLUX_PER_PATCH = NFLD * 2 * N * (N + 1);
     static constexpr int LDS_DOUBLES =
@@ -430,7 +473,7 @@ template <int N, int PPB, int MINW, int DESIGN>
 __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(StageArgs a)
 {
     using TL = Tile<N, PPB>;
-    constexpr int N2 = TL::N2, T = TL::T;
+    constexpr int N2 = TL::N2;
     __shared__ double lds[TL::LDS_DOUBLES];
     __shared__ double ldsU[PPB * NFLD * N2];     // own state, kept out of registers
     __shared__ double red[(TL::BLK + 63) / 64];
@@ -438,6 +481,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     __shared__ int pnbr[PPB][8];
     __shared__ double pdt[PPB];
     __shared__ double ptime[PPB];
+    __shared__ double ph[PPB], pcfl[PPB];     // cell size and CFL number of each patch
 
     const int tid = threadIdx.x;
     const int nblk = gridDim.x;
@@ -448,15 +492,126 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     const bool active = quad < a.m.nquad;
     const size_t base = (size_t)quad * NFLD * N2 + c;
 
-    // ---- prologue: every independent global load is issued here ----------
-    if (tid < PPB) {
+    // ---- prologue -----------------------------------------------------------
+    // The own-state loads and the halo loads go out first; the waits that
+    // the dependent chains need (patch parameters behind the instance id,
+    // the step's dt behind the CFL slots) come after them, so that no wait
+    // holds back an independent load.  With one patch per wave (N2 a
+    // multiple of 64) the patch's ids and parameters are wave-uniform and
+    // are read through the scalar cache (constant address space): they wait
+    // on their own counter, not behind the vector loads.
+    constexpr bool UNI = (N2 % 64 == 0);
+    constexpr bool ONE_GHOST = (4 * N <= N2);     // at most one halo cell per thread
+    double U[4] = {1.0, 0.0, 0.0, 1.0};
+    double B[4] = {0.0, 0.0, 0.0, 0.0};     // base state U0 of the step
+    const bool needB = (a.mode == MODE_SO && a.U0) || a.mode == MODE_RK4_ACC || a.mode == MODE_RK4_FINAL;
+    if (active) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) U[f] = a.Us[base + f * N2];
+    }
+    int nbw[8];
+    int levw = 0, inw = 0;
+    bool actw = false;
+    if constexpr (UNI) {
+        typedef const int __attribute__((address_space(4))) cint4;
+        const int qw = __builtin_amdgcn_readfirstlane(quad);
+        actw = qw < a.m.nquad;
+        const int qc = actw ? qw : 0;      // a valid index (nquad >= 1 when launched); discarded if inactive
+        const cint4 *nbc = (const cint4 *)a.m.nbr + (size_t)qc * 8;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) nbw[k] = nbc[k];
+        levw = ((const cint4 *)a.m.level)[qc];
+        inw = ((const cint4 *)a.m.inst)[qc];
+        if (!actw) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) nbw[k] = -1;
+            levw = 0;
+            inw = 0;
+        }
+    }
+    // halo: the ghost cell of side s = hh / N, position k = hh % N (and the
+    // second fine neighbour of a coarse side), loaded here and stored as
+    // records after the parameter chains below when there is at most one
+    // per thread
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    bool gsec = false;
+    for (int hh = c; hh < 4 * N; hh += N2) {
+        if (!active) break;
+        const int sd = hh / N, k = hh % N;
+        int na, nb;
+        if constexpr (UNI) {
+            na = sd == 0 ? nbw[0] : sd == 1 ? nbw[2] : sd == 2 ? nbw[4] : nbw[6];
+            nb = sd == 0 ? nbw[1] : sd == 1 ? nbw[3] : sd == 2 ? nbw[5] : nbw[7];
+        } else {
+            na = a.m.nbr[quad * 8 + 2 * sd];
+            nb = a.m.nbr[quad * 8 + 2 * sd + 1];
+        }
+        int ec, oc;   // neighbour's edge cell, own edge cell
+        switch (sd) {
+        case 0: ec = k * N + (N - 1); oc = k * N;           break;
+        case 1: ec = k * N + 0;       oc = k * N + (N - 1); break;
+        case 2: ec = (N - 1) * N + k; oc = k;               break;
+        default: ec = 0 * N + k;      oc = (N - 1) * N + k; break;
+        }
+        // physical boundary: the ghost holds the own edge cell, so that the
+        // generic face evaluation stays finite before the boundary flux
+        // replaces it
+        const double *sa = na < 0 ? a.Us + (size_t)quad * NFLD * N2 + oc
+                                  : a.Us + (size_t)na * NFLD * N2 + ec;
+        g0 = sa[0]; g1 = sa[N2]; g2 = sa[2 * N2]; g3 = sa[3 * N2];
+        gsec = nb >= 0;
+        if (gsec) {
+            const double *sb = a.Us + (size_t)nb * NFLD * N2 + ec;
+            s0 = sb[0]; s1 = sb[N2]; s2 = sb[2 * N2]; s3 = sb[3 * N2];
+        }
+        if constexpr (!ONE_GHOST) {
+            if (gsec) store_rec<N, PPB>(lds, p, TL::SECOND + hh, s0, s1, s2, s3);
+            store_rec<N, PPB>(lds, p, TL::GHOST + hh, g0, g1, g2, g3);
+        }
+    }
+    // patch parameters that depend on the instance
+    double hw = 0.0, cflw = 0.0, dtw = 0.0;
+    if constexpr (UNI) {
+        typedef const InstDev __attribute__((address_space(4))) cinst4;
+        typedef const double __attribute__((address_space(4))) cdbl4;
+        const cinst4 *ipc = (const cinst4 *)a.m.ip;
+        hw = actw ? ldexp(ipc[inw].h_base, -levw) / (double)N : 0.0;
+        cflw = actw ? ipc[inw].cfl : 0.0;
+        double tw = 0.0;
+        if (!(a.role & ROLE_DT)) {
+            // written by an earlier launch, constant during this one
+            dtw = actw && a.mode != MODE_RHS ? ((const cdbl4 *)a.dt_cur)[inw] : 0.0;
+            tw = actw ? ((const cdbl4 *)a.m.t_step)[inw] : 0.0;
+        }
+        if (c == 0) {
+            pquad[p] = actw ? lb * PPB + p : -1;
+            plevel[p] = levw;
+            pinst[p] = inw;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) pnbr[p][k] = nbw[k];
+            ph[p] = hw;
+            pcfl[p] = cflw;
+            if (!(a.role & ROLE_DT)) {
+                pdt[p] = dtw;
+                ptime[p] = tw;
+            }
+        }
+    } else if (tid < PPB) {
         int qq = lb * PPB + tid;
         bool act = qq < a.m.nquad;
+        const int lev = act ? a.m.level[qq] : 0, in = act ? a.m.inst[qq] : 0;
         pquad[tid] = act ? qq : -1;
-        plevel[tid] = act ? a.m.level[qq] : 0;
-        pinst[tid] = act ? a.m.inst[qq] : 0;
+        plevel[tid] = lev;
+        pinst[tid] = in;
 #pragma unroll
         for (int k = 0; k < 8; ++k) pnbr[tid][k] = act ? a.m.nbr[qq * 8 + k] : -1;
+        ph[tid] = act ? ldexp(a.m.ip[in].h_base, -lev) / (double)N : 0.0;
+        pcfl[tid] = act ? a.m.ip[in].cfl : 0.0;
+        if (!(a.role & ROLE_DT)) {
+            pdt[tid] = act && a.mode != MODE_RHS ? a.dt_cur[in] : 0.0;
+            // the step's start time, stable while the time advances
+            ptime[tid] = act ? a.m.t_step[in] : 0.0;
+        }
     }
     if ((a.role & ROLE_DT) && tid < 64) {
         // first stage of a step: wave 0 derives the dt of the block's
@@ -480,46 +635,15 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 ptime[q] = a.t_inst[in];
             }
         }
-    } else if (tid < PPB) {
-        // later stages: the step's start time, stable while the time advances
-        const int qq = lb * PPB + tid;
-        ptime[tid] = qq < a.m.nquad ? a.m.t_step[a.m.inst[qq]] : 0.0;
     }
-    double U[4] = {1.0, 0.0, 0.0, 1.0};
-    double B[4] = {0.0, 0.0, 0.0, 0.0};     // base state U0 of the step
-    const bool needB = (a.mode == MODE_SO && a.U0) || a.mode == MODE_RK4_ACC || a.mode == MODE_RK4_FINAL;
-    if (active) {
-#pragma unroll
-        for (int f = 0; f < 4; ++f) U[f] = a.Us[base + f * N2];
-    }
-    // ghost ring: each thread fetches its neighbour ids itself, so the halo
-    // loads do not wait for the block barrier
-    for (int hh = c; hh < 4 * N; hh += N2) {
-        if (!active) break;
-        int s = hh / N, k = hh % N;
-        int na = a.m.nbr[quad * 8 + 2 * s], nb = a.m.nbr[quad * 8 + 2 * s + 1];
-        int ec, oc, tpos;   // neighbour's edge cell, own edge cell, ghost position
-        switch (s) {
-        case 0: ec = k * N + (N - 1); oc = k * N;           tpos = (k + 1) * T + 0; break;
-        case 1: ec = k * N + 0;       oc = k * N + (N - 1); tpos = (k + 1) * T + (N + 1); break;
-        case 2: ec = (N - 1) * N + k; oc = k;               tpos = 0 * T + (k + 1); break;
-        default: ec = 0 * N + k;      oc = (N - 1) * N + k; tpos = (N + 1) * T + (k + 1); break;
+    if constexpr (ONE_GHOST) {
+        if (active && c < 4 * N) {
+            if (gsec) store_rec<N, PPB>(lds, p, TL::SECOND + c, s0, s1, s2, s3);
+            store_rec<N, PPB>(lds, p, TL::GHOST + c, g0, g1, g2, g3);
         }
-        // physical boundary: the ghost holds the own edge cell, so that the
-        // generic face evaluation stays finite before the boundary flux
-        // replaces it
-        const double *sa = na < 0 ? a.Us + (size_t)quad * NFLD * N2 + oc
-                                  : a.Us + (size_t)na * NFLD * N2 + ec;
-        double q0 = sa[0], q1 = sa[N2], q2 = sa[2 * N2], q3 = sa[3 * N2];
-        if (nb >= 0) {
-            const double *sb = a.Us + (size_t)nb * NFLD * N2 + ec;
-            double r0 = sb[0], r1 = sb[N2], r2 = sb[2 * N2], r3 = sb[3 * N2];
-            store_rec<N, PPB>(lds, p, TL::TT + s * N + k, r0, r1, r2, r3);
-        }
-        store_rec<N, PPB>(lds, p, tpos, q0, q1, q2, q3);
     }
     if (active) {
-        store_rec<N, PPB>(lds, p, (cj + 1) * T + (ci + 1), U[0], U[1], U[2], U[3]);
+        store_rec<N, PPB>(lds, p, c, U[0], U[1], U[2], U[3]);
 #pragma unroll
         for (int f = 0; f < 4; ++f) ldsU[(p * NFLD + f) * N2 + c] = U[f];
     }
@@ -533,16 +657,15 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         double FA[4] = {0, 0, 0, 0}, FB[4] = {0, 0, 0, 0}, FC[4] = {0, 0, 0, 0};
         int dC = -1;
         if (active) {
-            const double h = ldexp(a.m.ip[pinst[p]].h_base, -plevel[p]) / (double)N;
+            const double h = UNI ? hw : ph[p];
             const int na1 = pnbr[p][2], nb1 = pnbr[p][3], na3 = pnbr[p][6], nb3 = pnbr[p][7];
             const bool ex = (ci == N - 1), ey = (cj == N - 1);
             const double hx = (ex && nb1 >= 0) ? 0.5 * h : h;
             const double hy = (ey && nb3 >= 0) ? 0.5 * h : h;
-            const int t0 = (cj + 1) * T + (ci + 1);
             Prim L, RX, RY;
-            load_prim<N, PPB>(lds, p, t0, L);
-            load_prim<N, PPB>(lds, p, t0 + 1, RX);
-            load_prim<N, PPB>(lds, p, t0 + T, RY);
+            load_prim<N, PPB>(lds, p, c, L);
+            load_prim<N, PPB>(lds, p, ex ? TL::GHOST + 1 * N + cj : c + 1, RX);
+            load_prim<N, PPB>(lds, p, ey ? TL::GHOST + 3 * N + ci : c + N, RY);
             flux_face<true>(L, RX, hx, FA);
             flux_face<false>(L, RY, hy, FB);
             if (ex && (na1 < 0 || nb1 >= 0)) {
@@ -553,7 +676,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                                   cj + 0.5, h, ptime[p], FA);
                 } else {
                     Prim F2;
-                    load_prim<N, PPB>(lds, p, TL::TT + 1 * N + cj, F2);
+                    load_prim<N, PPB>(lds, p, TL::SECOND + 1 * N + cj, F2);
                     double G[4];
                     flux_face<true>(L, F2, hx, G);
                     FA[0] += G[0]; FA[1] += G[1]; FA[2] += G[2]; FA[3] += G[3];
@@ -567,7 +690,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                                   (double)N, h, ptime[p], FB);
                 } else {
                     Prim F2;
-                    load_prim<N, PPB>(lds, p, TL::TT + 3 * N + ci, F2);
+                    load_prim<N, PPB>(lds, p, TL::SECOND + 3 * N + ci, F2);
                     double G[4];
                     flux_face<false>(L, F2, hy, G);
                     FB[0] += G[0]; FB[1] += G[1]; FB[2] += G[2]; FB[3] += G[3];
@@ -578,17 +701,17 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         if (tid < PPB * 2 * N) {
             const int tp = tid / (2 * N), e = tid % (2 * N);
             if (pquad[tp] >= 0) {
-                const double h = ldexp(a.m.ip[pinst[tp]].h_base, -plevel[tp]) / (double)N;
+                const double h = ph[tp];
                 const bool xs = e < N;                 // side 0 (x) or side 2 (y)
                 const int k = xs ? e : e - N, side = xs ? 0 : 2;
                 const int na = pnbr[tp][2 * side], nb = pnbr[tp][2 * side + 1];
                 const double hf = nb >= 0 ? 0.5 * h : h;
-                const int tl = xs ? (k + 1) * T : (k + 1);
-                const int tr = xs ? tl + 1 : tl + T;
+                const int tl = TL::GHOST + side * N + k;   // ghost of side 0 or 2
+                const int tr = xs ? k * N : k;             // first cell of row / column k
                 Prim L, R;
                 load_prim<N, PPB>(lds, tp, tl, L);
                 load_prim<N, PPB>(lds, tp, tr, R);
-                if (xs) flux_face<true>(L, R, hf, FC); else flux_face<false>(L, R, hf, FC);
+                flux_face_dir(L, R, hf, xs, FC);
                 if (na < 0) {
                     const double nrm[2] = {xs ? -h : 0.0, xs ? 0.0 : -h};
                     const double q[4] = {R.r, R.ru, R.rv, R.rE};
@@ -597,9 +720,9 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                     FC[0] = -FC[0]; FC[1] = -FC[1]; FC[2] = -FC[2]; FC[3] = -FC[3];
                 } else if (nb >= 0) {
                     Prim F2;
-                    load_prim<N, PPB>(lds, tp, TL::TT + side * N + k, F2);
+                    load_prim<N, PPB>(lds, tp, TL::SECOND + side * N + k, F2);
                     double G[4];
-                    if (xs) flux_face<true>(F2, R, hf, G); else flux_face<false>(F2, R, hf, G);
+                    flux_face_dir(F2, R, hf, xs, G);
                     FC[0] += G[0]; FC[1] += G[1]; FC[2] += G[2]; FC[3] += G[3];
                 }
                 dC = tp * TL::FLUX_PER_PATCH + (xs ? k * (N + 1) : NFLD * N * (N + 1) + k);
@@ -657,17 +780,17 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 }
             }
             if (pquad[tp] < 0) continue;
-            const double h = ldexp(a.m.ip[pinst[tp]].h_base, -plevel[tp]) / (double)N;
+            const double h = ph[tp];
             double F[4];
             // tile positions of the low (L) and high (R) cell of the face
             int tl, tr;
-            if (isx) { tl = (fj + 1) * T + fi; tr = tl + 1; }
-            else     { tl = fj * T + (fi + 1); tr = tl + T; }
+            if (isx) { tl = TL::pos(fi - 1, fj); tr = TL::pos(fi, fj); }
+            else     { tl = TL::pos(fi, fj - 1); tr = TL::pos(fi, fj); }
             if (!edge) {
                 Prim L, R;
                 load_prim<N, PPB>(lds, tp, tl, L);
                 load_prim<N, PPB>(lds, tp, tr, R);
-                if (isx) flux_face<true>(L, R, h, F); else flux_face<false>(L, R, h, F);
+                flux_face_dir(L, R, h, isx, F);
             } else {
                 int na = pnbr[tp][2 * side], nb = pnbr[tp][2 * side + 1];
                 bool lowside = (side == 0 || side == 2);
@@ -689,7 +812,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                     Prim L, R;
                     load_prim<N, PPB>(lds, tp, tl, L);
                     load_prim<N, PPB>(lds, tp, tr, R);
-                    if (isx) flux_face<true>(L, R, h, F); else flux_face<false>(L, R, h, F);
+                    flux_face_dir(L, R, h, isx, F);
                 } else {
                     // coarse side of a hanging face: sub-face k couples to sub-face
                     // k of both fine neighbours, normal length of the fine level
@@ -702,12 +825,12 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                     for (int q = 0; q < 2; ++q) {
                         // one fine neighbour at a time keeps a single flux live
                         Prim fn;
-                        load_prim<N, PPB>(lds, tp, q == 0 ? (lowside ? tl : tr) : TL::TT + side * N + k, fn);
+                        load_prim<N, PPB>(lds, tp, q == 0 ? (lowside ? tl : tr) : TL::SECOND + side * N + k, fn);
                         double G[4];
                         if (lowside) {
-                            if (isx) flux_face<true>(fn, own, hf, G); else flux_face<false>(fn, own, hf, G);
+                            flux_face_dir(fn, own, hf, isx, G);
                         } else {
-                            if (isx) flux_face<true>(own, fn, hf, G); else flux_face<false>(own, fn, hf, G);
+                            flux_face_dir(own, fn, hf, isx, G);
                         }
                         F[0] += G[0]; F[1] += G[1]; F[2] += G[2]; F[3] += G[3];
                         __builtin_amdgcn_sched_barrier(0);
@@ -748,7 +871,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     double cand = INFINITY;
     double sq = 0.0;
     if (active) {
-        const double h = ldexp(a.m.ip[pinst[p]].h_base, -plevel[p]) / (double)N;
+        const double h = UNI ? hw : ph[p];
         const double vol = h * h;
         const double ninvV = -1.0 / vol;
         const double *FX = lds + p * TL::FLUX_PER_PATCH;
@@ -761,9 +884,8 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             k4[f] = Rf * ninvV;
             sq += k4[f] * k4[f];
         }
-        const int inst = pinst[p];
         double dt = 0.0;
-        if (a.mode != MODE_RHS) dt = (a.role & ROLE_DT) ? pdt[p] : a.dt_cur[inst];
+        if (a.mode != MODE_RHS) dt = (UNI && !(a.role & ROLE_DT)) ? dtw : pdt[p];
         switch (a.mode) {
         case MODE_SO:
             if (a.U0) {
@@ -809,7 +931,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         }
 #pragma unroll
         for (int f = 0; f < 4; ++f) a.Uout[base + f * N2] = out[f];
-        if (a.role & ROLE_CFL) cand = cfl_candidate(out, a.m.ip[inst].cfl, vol);
+        if (a.role & ROLE_CFL) cand = cfl_candidate(out, UNI ? cflw : pcfl[p], vol);
     }
     // time bookkeeping: stage 2 advances t by this step's dt and re-arms the
     // CFL slot that the last stage fills
