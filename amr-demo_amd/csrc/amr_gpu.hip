@@ -306,11 +306,34 @@ __device__ __forceinline__ double cfl_candidate(const double q[4], double cfl, d
     return cfl * sqrt(vol) / eig;
 }
 
+// v from the lane `CTRL` (a DPP row-shift control) selects, or v itself
+// where the shift has no source lane
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    int lo = (int)b, hi = (int)(b >> 32);
+    lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_f64(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Minimum over the 64 lanes of a wave (all lanes active), returned to every
+// lane.  Row shifts by 1, 2, 4, 8 leave each 16-lane row's minimum in its
+// last lane (DPP moves: no LDS round trips); the four row minima are read
+// as scalars.  A minimum is exact, so the order does not change the value.
 __device__ __forceinline__ double wave_min(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
+    v = fmin(v, dpp_f64<0x111>(v));    // row_shr:1
+    v = fmin(v, dpp_f64<0x112>(v));    // row_shr:2
+    v = fmin(v, dpp_f64<0x114>(v));    // row_shr:4
+    v = fmin(v, dpp_f64<0x118>(v));    // row_shr:8
+    return fmin(fmin(lane_f64(v, 15), lane_f64(v, 31)), fmin(lane_f64(v, 47), lane_f64(v, 63)));
 }
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -511,7 +534,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     }
     int nbw[8];
     int levw = 0, inw = 0;
-    bool actw = false;
+    bool actw = false, firstw = false;   // firstw: the wave holds its instance's first quadrant
     if constexpr (UNI) {
         typedef const int __attribute__((address_space(4))) cint4;
         const int qw = __builtin_amdgcn_readfirstlane(quad);
@@ -522,11 +545,24 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         for (int k = 0; k < 8; ++k) nbw[k] = nbc[k];
         levw = ((const cint4 *)a.m.level)[qc];
         inw = ((const cint4 *)a.m.inst)[qc];
+        firstw = actw && (qc == 0 || ((const cint4 *)a.m.inst)[qc - 1] != inw);
         if (!actw) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) nbw[k] = -1;
             levw = 0;
             inw = 0;
+        }
+    }
+    // first stage of a step: the CFL sub-slots of the patch's instance, read
+    // with the halo; the step's dt is formed from them after the records
+    unsigned long long slv[(NSLOT + 63) / 64];
+    if constexpr (UNI) {
+        if ((a.role & ROLE_DT) && a.use_cfl) {
+            const unsigned long long *sp = a.slot + (size_t)inw * NSLOT;
+            const int lane = threadIdx.x & 63;
+#pragma unroll
+            for (int k = 0; k < (NSLOT + 63) / 64; ++k)
+                slv[k] = lane + 64 * k < NSLOT ? sp[lane + 64 * k] : DBL_INF_BITS;
         }
     }
     // halo: the ghost cell of side s = hh / N, position k = hh % N (and the
@@ -613,7 +649,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             ptime[tid] = act ? a.m.t_step[in] : 0.0;
         }
     }
-    if ((a.role & ROLE_DT) && tid < 64) {
+    if (!UNI && (a.role & ROLE_DT) && tid < 64) {
         // first stage of a step: wave 0 derives the dt of the block's
         // instances from the CFL sub-slots and publishes it
         int last_inst = -1;
@@ -646,6 +682,38 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         store_rec<N, PPB>(lds, p, c, U[0], U[1], U[2], U[3]);
 #pragma unroll
         for (int f = 0; f < 4; ++f) ldsU[(p * NFLD + f) * N2 + c] = U[f];
+    }
+    if constexpr (UNI) {
+        if (a.role & ROLE_DT) {
+            // the step's dt of the patch's instance (step_dt with the slots
+            // loaded above; every wave of the instance forms the same value)
+            typedef const InstDev __attribute__((address_space(4))) cinst4;
+            typedef const double __attribute__((address_space(4))) cdbl4;
+            const cinst4 *ipc = (const cinst4 *)a.m.ip;
+            double dt = ipc[inw].dt_cap;
+            if (a.use_cfl) {
+                double m = INFINITY;
+#pragma unroll
+                for (int k = 0; k < (NSLOT + 63) / 64; ++k) m = fmin(m, bits2d(slv[k]));
+                m = wave_min(m);
+                if (m < dt) dt = m;
+            }
+            const double ti = ((const cdbl4 *)a.t_inst)[inw];   // advanced by a later stage only
+            const double t_end = ipc[inw].t_end;
+            if (t_end > 0.0) {
+                double rem = t_end - ti;
+                if (rem < 0.0) rem = 0.0;
+                if (rem < dt) dt = rem;
+            }
+            dtw = dt;
+            if (c == 0) {
+                ptime[p] = ti;
+                if (firstw) {                 // one writer per instance (a hot line otherwise)
+                    a.dt_cur[inw] = dt;
+                    a.t_step[inw] = ti;
+                }
+            }
+        }
     }
     __syncthreads();
 
@@ -885,7 +953,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             sq += k4[f] * k4[f];
         }
         double dt = 0.0;
-        if (a.mode != MODE_RHS) dt = (UNI && !(a.role & ROLE_DT)) ? dtw : pdt[p];
+        if (a.mode != MODE_RHS) dt = UNI ? dtw : pdt[p];
         switch (a.mode) {
         case MODE_SO:
             if (a.U0) {
