@@ -47,6 +47,8 @@ EXPORTS = {
     "amrgpu_num_quads": (_i, [_vp]),
     "amrgpu_copy_probe": (_i, [C.c_longlong, _i, _pd]),
     "amrgpu_set_tile": (_i, [_vp, _i, _i, _i]),
+    "amrgpu_set_graphs": (_i, [_vp, _i]),
+    "amrgpu_get_graphs": (_i, [_vp]),
     "amrgpu_get_tile": (_i, [_vp, _pi, _pi, _pi]),
     "amrgpu_tile_variants": (_i, [_i, _pi, _pi, _pi, _i]),
     "amrgpu_autotune": (_i, [_vp, _i, _pi, _pi, _pi, _pi, _pd]),
@@ -615,6 +617,14 @@ class Gpu:
 
     def set_tile(self, ppb, minw=1, design=0):
         check(self.L.amrgpu_set_tile(self.h, ppb, minw, design), "amrgpu_set_tile")
+
+    def set_graphs(self, enable):
+        """step launch mode: True replays a captured HIP graph per step,
+        False launches the stages directly (the default)"""
+        check(self.L.amrgpu_set_graphs(self.h, 1 if enable else 0), "amrgpu_set_graphs")
+
+    def graphs(self):
+        return bool(self.L.amrgpu_get_graphs(self.h))
 
     def tile(self):
         p, m, d = _i(), _i(), _i()

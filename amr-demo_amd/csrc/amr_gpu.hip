@@ -1236,7 +1236,12 @@ struct amrgpu_ctx {
     size_t cap_state = 0;
     int cur = 0;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
-    bool graphs_enabled = true;   // AMRGPU_GRAPHS=0 selects direct launches
+    // Step launch mode: direct stream launches by default (measured on
+    // MI355X, ROCm 7: back-to-back dependent kernels cost ~2.3 us each from a
+    // stream and ~4.7-6.7 us each replayed from a captured graph); graph
+    // replay on request (AMRGPU_GRAPHS=1, amrgpu_set_graphs) or when the
+    // autotuner measures it faster.
+    bool graphs_enabled = false;
     std::vector<InstDev> h_ip;
 };
 
@@ -1528,7 +1533,7 @@ amrgpu_ctx *amrgpu_create(void)
 {
     amrgpu_ctx *c = new amrgpu_ctx();
     const char *ge = getenv("AMRGPU_GRAPHS");
-    if (ge && ge[0] == '0') c->graphs_enabled = false;
+    if (ge) c->graphs_enabled = ge[0] == '1';
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         set_err("hipStreamCreate failed");
         delete c;
@@ -2124,11 +2129,44 @@ int amrgpu_autotune(amrgpu_ctx *c, int nsteps, int *nvar, int *ppb, int *minw, i
         HIPCHK(hipStreamSynchronize(c->stream));
         c->cur = cur0;
     }
+    if (nvar) *nvar = k;
+    int rc = amrgpu_set_tile(c, kVariants[best].ppb, kVariants[best].minw, kVariants[best].design);
+    if (rc) return rc;
+    // launch mode for the chosen tile: direct launches against graph replay
+    double per_mode[2] = {0.0, 0.0};
+    for (int m = 0; m < 2; ++m) {
+        c->graphs_enabled = (m == 1);
+        if ((rc = prime_run(c))) return rc;
+        for (int s = 0; s < 2; ++s) if ((rc = one_step(c, true, false))) return rc;
+        HIPCHK(hipEventRecord(e0, c->stream));
+        for (int s = 0; s < nsteps; ++s) if ((rc = one_step(c, true, false))) return rc;
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        per_mode[m] = ms / nsteps;
+        HIPCHK(hipMemcpyAsync(c->U[cur0], c->d_stage, sizeof(double) * nstate, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_t, t0.data(), sizeof(double) * c->ninst, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->cur = cur0;
+    }
+    c->graphs_enabled = per_mode[1] < per_mode[0];
+    if (!c->graphs_enabled) free_graphs(c);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (nvar) *nvar = k;
-    return amrgpu_set_tile(c, kVariants[best].ppb, kVariants[best].minw, kVariants[best].design);
+    return AMRGPU_OK;
 }
+
+int amrgpu_set_graphs(amrgpu_ctx *c, int enable)
+{
+    if (!c) { set_err("amrgpu_set_graphs: no context"); return AMRGPU_ERR_ARG; }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->graphs_enabled = enable != 0;
+    if (!c->graphs_enabled) free_graphs(c);
+    return AMRGPU_OK;
+}
+
+int amrgpu_get_graphs(const amrgpu_ctx *c) { return c && c->graphs_enabled ? 1 : 0; }
 
 int amrgpu_copy_probe(long long nbytes, int reps, double *gbps)
 {

@@ -17,7 +17,9 @@ interval-by-interval path through hpc_amr_evolve_solver).
 
 A step = one SSP-RK3 step (3 fused residual + stage-update launches, CFL
 folded into the last) over that mesh.  value = sub-grid cells x 3 stages x
-steps / wall time of the timed region (graph replay, HIP events).
+steps / wall time of the timed region (HIP events; the stages launched
+directly from the stream or replayed from a captured graph, whichever the
+autotuner measured faster).
 
 roofline: the fused stage kernel; algorithmic bytes = 85.33 B per cell-stage
 (stage 1 reads U^n, writes U^1; stages 2-3 read U^s and U^n, write U^s+1;
@@ -352,13 +354,14 @@ def main():
         log("autotune (ppb, minw, design, ms/step): " +
             ", ".join(f"({a},{b},{c},{d:.4f})" for a, b, c, d in tune))
     tile = g.tile()
+    launch = "graph replay" if g.graphs() else "direct stream launches"
     if args.warmup:
         g.run(args.warmup)
     ms, stage_ms = g.bench(args.steps)
     ms_per_step = ms / args.steps
     value = cells * 3 * args.steps / (ms * 1e-3) / 1e6
     # average launch duration over the timed region (HIP events on the
-    # context's stream around the graph replays): region time / launches
+    # context's stream around the steps): region time / launches
     avg_stage = ms / (3 * args.steps)
     bytes_launch = cells * B_ALG_STAGE
     achieved = bytes_launch / (avg_stage * 1e-3) / 1e9
@@ -390,7 +393,7 @@ def main():
             "levels": [int(lv.min()), int(lv.max())], "parallelism": "1 GPU, whole forest",
             "tile": {"patches_per_workgroup": tile[0], "min_waves_per_simd": tile[1],
                      "design": ["face task list", "paired faces"][tile[2]],
-                     "autotuned": tune is not None},
+                     "autotuned": tune is not None, "launch": launch},
         },
         "roofline": {
             "bound": "hbm",
@@ -402,7 +405,7 @@ def main():
             "kernel": f"stage_kernel<{args.n},{tile[0]},{tile[1]},{tile[2]}>",
             "bytes_per_launch": int(bytes_launch),
             "avg_launch_us": round(avg_stage * 1e3, 3),
-            "avg_launch_from": "timed region / launches (HIP events, graph replay)",
+            "avg_launch_from": f"timed region / launches (HIP events, {launch})",
             "stage_launch_us_bracketed": [round(s * 1e3, 3) for s in stage_ms],
             "copy_probe_GBs": round(probe, 1) if probe else None,
         },

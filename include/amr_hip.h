@@ -164,16 +164,25 @@ int amrgpu_set_geometry(amrgpu_ctx *c, const double *geom);
 int amrgpu_rhs(amrgpu_ctx *c, double *rhs);
 /* CFL minimum of the current state per instance (no dt cap). */
 int amrgpu_cfl(amrgpu_ctx *c, double *dt_inst);
-/* Advance nsteps steps.  use_graph: replay one captured step.  On return
- * resnorm (if non-NULL) holds sqrt(sum rhs^2) of the last stage. */
+/* Advance nsteps steps.  use_graph: replay one captured step when graph
+ * replay is enabled (amrgpu_set_graphs), else launch the stages directly.
+ * On return resnorm (if non-NULL) holds sqrt(sum rhs^2) of the last stage. */
 int amrgpu_run(amrgpu_ctx *c, int nsteps, int use_graph, double *resnorm);
+/* Step launch mode: 0 direct stream launches (the default: on MI355X a
+ * dependent kernel costs less launched from the stream than replayed from a
+ * captured graph), 1 replay of one captured HIP graph per step.  The
+ * environment variable AMRGPU_GRAPHS=0/1 sets the initial mode;
+ * amrgpu_autotune times both for the chosen tile and keeps the faster. */
+int amrgpu_set_graphs(amrgpu_ctx *c, int enable);
+int amrgpu_get_graphs(const amrgpu_ctx *c);
 /* Per-instance physical time and last dt. */
 int amrgpu_get_time(amrgpu_ctx *c, double *t_inst, double *dt_inst);
 int amrgpu_set_time(amrgpu_ctx *c, const double *t_inst);
 
-/* Benchmark hooks: run nsteps steps, timing the whole region with HIP events
- * (graph replay) -> *ms_total; then nsteps more steps launched directly with
- * events around every stage kernel -> avg_stage_ms[s] for s < nstages. */
+/* Benchmark hooks: run nsteps steps in the current launch mode, timing the
+ * whole region with HIP events -> *ms_total; then nsteps more steps launched
+ * directly with events around every stage kernel -> avg_stage_ms[s] for
+ * s < nstages. */
 int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_ms,
                  int *nstages);
 int amrgpu_num_cells(const amrgpu_ctx *c);   /* sub-grid cells */
@@ -181,8 +190,9 @@ int amrgpu_num_cells(const amrgpu_ctx *c);   /* sub-grid cells */
  * per SIMD reserved by register allocation, and the face-evaluation design
  * (0: face task list, 1: paired +x/+y faces per cell).  amrgpu_autotune
  * times every compiled variant for the mesh's patch size over nsteps steps
- * (state and time are restored afterwards), keeps the fastest and reports
- * each variant's ms per step. */
+ * with direct launches (state and time are restored afterwards), keeps the
+ * fastest and reports each variant's ms per step, then sets the launch mode
+ * (amrgpu_set_graphs) that is faster for that tile. */
 int amrgpu_set_tile(amrgpu_ctx *c, int ppb, int minw, int design);
 /* The compiled variants for patch size n (returns their number). */
 int amrgpu_tile_variants(int n, int *ppb, int *minw, int *design, int max);

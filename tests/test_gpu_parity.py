@@ -136,11 +136,14 @@ def test_graph_and_direct_launch_identical(gpu):
     f = riemann_forest(8, seed=3, lmax=4)
     ci, fi, geom, soln = f.external()
     setup(gpu, 8, 3, 1.0, (0, 1, 2, 0), 0.3, 1.0, ci, fi, geom, soln)
-    gpu.run(7, graph=True)
+    assert not gpu.graphs()          # direct launches unless asked otherwise
+    gpu.set_graphs(True)
+    gpu.run(7, graph=True)           # replays of the captured step graphs
     a = gpu.soln()
     gpu.upload(soln)
     gpu.set_time([0.0])
-    gpu.run(7, graph=False)
+    gpu.set_graphs(False)
+    gpu.run(7, graph=True)           # graph requested, replay disabled: direct
     b = gpu.soln()
     assert np.array_equal(a, b)
 
