@@ -188,24 +188,33 @@ __device__ __forceinline__ void flux_normal(const OPrim &L, const OPrim &R, doub
     double Mn = unbar * inva;
     double eigp = unbar + abar, eigm = unbar - abar;
     double eigp_max = R.un + abar, eigm_min = L.un - abar;
-    double b1 = (eigp > 0.0 || eigp_max > 0.0) ? fmax(eigp, eigp_max) : 0.0;
-    double b2 = (eigm < 0.0 || eigm_min < 0.0) ? fmin(eigm, eigm_min) : 0.0;
-    double invb = frcp(b1 - b2);
+    // b1 = max(eig+, eig+_max) if either is positive, else 0 (b2 alike):
+    // the same values as the reference's branches, NaN operands included
+    double b1 = fmax(fmax(eigp, eigp_max), 0.0);
+    double b2 = fmin(fmin(eigm, eigm_min), 0.0);
+    // 1/(b1 - b2) and 1/(1 + |Mn|) from one refined reciprocal of their
+    // product (b1 - b2 > 0 whenever the sound speed is positive)
+    const double d1 = b1 - b2, d2 = 1.0 + fabs(Mn);
+    const double rr = frcp(d1 * d2);
+    double invb = rr * d2;
+    double Mc = rr * d1;
     double bcoeff = b1 * b2 * invb;
     double delta1 = (R.r - L.r) - (R.p - L.p) * (inva * inva);
-    double Mc = frcp(1.0 + fabs(Mn));
-    double bM = bcoeff * Mc;
     double dH = R.H - L.H;
+    // the face length folded into the coefficients
+    const double invbh = invb * h;
+    const double c1 = b1 * invbh, c2 = b2 * invbh;
+    const double bch = bcoeff * h, bMh = bcoeff * Mc * h;
     double fl1 = L.rn * L.un + L.p, fr1 = R.rn * R.un + R.p;
     double fl2 = L.rn * L.ut, fr2 = R.rn * R.ut;
-    double d1 = delta1 * unbar;                           // the shear term vanishes along n
-    double d2 = delta1 * utbar + rhobar * (R.ut - L.ut);
+    double dn = delta1 * unbar;                           // the shear term vanishes along n
+    double dt = delta1 * utbar + rhobar * (R.ut - L.ut);
     double fl3 = (L.rE + L.p) * L.un, fr3 = (R.rE + R.p) * R.un;
-    F[0] = ((b1 * L.rn - b2 * R.rn) * invb + (bcoeff * (R.r - L.r) - bM * delta1)) * h;
-    F[1] = ((b1 * fl1 - b2 * fr1) * invb + (bcoeff * (R.rn - L.rn) - bM * d1)) * h;
-    F[2] = ((b1 * fl2 - b2 * fr2) * invb + (bcoeff * (R.rt - L.rt) - bM * d2)) * h;
-    F[3] = ((b1 * fl3 - b2 * fr3) * invb +
-            (bcoeff * (R.r * R.H - L.r * L.H) - bM * (delta1 * Hbar + rhobar * dH))) * h;
+    F[0] = fma(-bMh, delta1, fma(bch, R.r - L.r, fma(-c2, R.rn, L.rn * c1)));
+    F[1] = fma(-bMh, dn, fma(bch, R.rn - L.rn, fma(-c2, fr1, fl1 * c1)));
+    F[2] = fma(-bMh, dt, fma(bch, R.rt - L.rt, fma(-c2, fr2, fl2 * c1)));
+    F[3] = fma(-bMh, delta1 * Hbar + rhobar * dH,
+               fma(bch, R.r * R.H - L.r * L.H, fma(-c2, fr3, fl3 * c1)));
 }
 
 // Flux through an x face (XDIR) or y face in the (x, y) frame.
