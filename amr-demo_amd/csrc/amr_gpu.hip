@@ -1007,7 +1007,8 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             for (int f = 0; f < 4; ++f) out[f] = U[f];
         }
 #pragma unroll
-        for (int f = 0; f < 4; ++f) a.Uout[base + f * N2] = out[f];
+        for (int f = 0; f < 4; ++f)   // streamed out: no other wave of this launch reads it
+            __builtin_nontemporal_store(out[f], &a.Uout[base + f * N2]);
         if (a.role & ROLE_CFL) cand = cfl_candidate(out, UNI ? cflw : pcfl[p], vol);
     }
     // time bookkeeping: stage 2 advances t by this step's dt and re-arms the
