@@ -1999,7 +1999,7 @@ int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_
     if (!c || !c->nquad || nsteps <= 0) { set_err("amrgpu_bench: no mesh"); return AMRGPU_ERR_ARG; }
     int rc = prime_run(c);
     if (rc) return rc;
-    // capture the graphs of every starting buffer before timing
+    // in graph mode, capture the graphs of every starting buffer before timing
     if (c->graphs_enabled) {
         int cur = c->cur;
         for (int s = 0; s < 2; ++s) {

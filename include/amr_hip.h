@@ -11,8 +11,9 @@
  *    forest, so the MPI_Comm argument is an opaque pointer that is ignored.
  *
  * 2. A device-resident context (amrgpu_*), with no reference counterpart,
- *    that keeps the state in HBM across steps, runs whole evolve intervals as
- *    captured HIP graphs and exposes the timing hooks the benchmark needs.
+ *    that keeps the state in HBM across steps, runs whole evolve intervals
+ *    (stages launched directly, or replayed from captured HIP graphs) and
+ *    exposes the timing hooks the benchmark needs.
  *    It also carries the batched mode: several independent forests
  *    ("instances") advanced by the same launches.
  *
