@@ -223,6 +223,7 @@ int amr_regrid_tags(amr_forest_t *f, const int *tags, int allow_coarsening)
     /* tag_reset + regrid_feature; a "keep" tag (2) refines nothing but, being
      * non-zero, stops tag_coarsen from merging the family */
     int *flags = (int *)malloc(sizeof(int) * (size_t)(nq > 0 ? nq : 1));
+    if (!flags) return -1;
     for (int k = 0; k < nq; ++k) {
         tag[k] = tags[k] == 1 ? 1 : (tags[k] ? 2 : 0);
         flags[k] = tags[k] == 1;
@@ -233,7 +234,7 @@ int amr_regrid_tags(amr_forest_t *f, const int *tags, int allow_coarsening)
     amr_forest_refine(f, flags);
     free(flags);
     if (allow_coarsening) amr_forest_coarsen(f);
-    amr_forest_balance_marked(f);
+    if (amr_forest_balance_marked(f) < 0) return -1;
     return amr_forest_num_quads(f);
 }
 
@@ -429,8 +430,8 @@ static void *lag_regrid_thread(void *arg)
     lag_job_t *j = (lag_job_t *)arg;
     double t0 = now_s();
     amr_forest_track_begin(j->f, 0);
-    amr_regrid_tags(j->f, j->tags, j->allow);
-    j->ok = amr_forest_track_end(j->f);
+    const int r = amr_regrid_tags(j->f, j->tags, j->allow);
+    j->ok = amr_forest_track_end(j->f) && r >= 0;
     ext_of(j->f, &j->ext);
     j->wall = now_s() - t0;
     return NULL;
@@ -446,8 +447,8 @@ int amr_device_regrid(amr_forest_t *f, amrgpu_ctx *g, const int *tags, int allow
     double *geom, *soln;
     double t0 = now_s();
     amr_forest_track_begin(f, 0);
-    amr_regrid_tags(f, tags, allow_coarsening);
-    if (!amr_forest_track_end(f)) return AMRGPU_ERR_MESH;
+    const int r = amr_regrid_tags(f, tags, allow_coarsening);
+    if (!amr_forest_track_end(f) || r < 0) return AMRGPU_ERR_MESH;
     if (amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom, &soln))
         return AMRGPU_ERR_MESH;
     double t1 = now_s();
@@ -689,8 +690,8 @@ static void *regrid_worker(void *arg)
         j->ok[k] = 1;
         if (!j->done[k]) {
             amr_forest_track_begin(j->f[k], 0);
-            amr_regrid_tags(j->f[k], j->tags + j->obase[k], j->allow);
-            j->ok[k] = amr_forest_track_end(j->f[k]);
+            const int r = amr_regrid_tags(j->f[k], j->tags + j->obase[k], j->allow);
+            j->ok[k] = amr_forest_track_end(j->f[k]) && r >= 0;
         }
         ext_of(j->f[k], &j->ext[k]);
     }

@@ -590,6 +590,7 @@ typedef struct {
     nodeent_t *e;
     uint64_t mask;
     int used;
+    int oom;           /* an allocation failed: the set is no longer complete */
 } nodeset_t;
 
 static nodeent_t *ns_find(const nodeset_t *S, int meta, uint64_t key)
@@ -601,12 +602,15 @@ static nodeent_t *ns_find(const nodeset_t *S, int meta, uint64_t key)
     }
 }
 
-static void ns_alloc(nodeset_t *S, uint64_t cap)
+static int ns_alloc(nodeset_t *S, uint64_t cap)
 {
     S->e = (nodeent_t *)malloc(sizeof(nodeent_t) * (size_t)cap);
-    for (uint64_t i = 0; i < cap; ++i) S->e[i].meta = -1;
     S->mask = cap - 1;
     S->used = 0;
+    S->oom = S->e == NULL;
+    if (S->oom) return -1;
+    for (uint64_t i = 0; i < cap; ++i) S->e[i].meta = -1;
+    return 0;
 }
 
 static void ns_put(nodeset_t *S, int meta, uint64_t key, int leaf)
@@ -622,9 +626,10 @@ static void ns_put(nodeset_t *S, int meta, uint64_t key, int leaf)
 /* insert a node known to be absent, keeping the load at most 1/2 */
 static void ns_insert(nodeset_t *S, int meta, uint64_t key, int leaf)
 {
+    if (S->oom) return;
     if (2 * (uint64_t)(S->used + 1) > S->mask + 1) {
         nodeset_t T;
-        ns_alloc(&T, 2 * (S->mask + 1));
+        if (ns_alloc(&T, 2 * (S->mask + 1))) { S->oom = 1; return; }
         for (uint64_t i = 0; i <= S->mask; ++i)
             if (S->e[i].meta != -1) ns_put(&T, S->e[i].meta, S->e[i].key, S->e[i].leaf);
         free(S->e);
@@ -643,15 +648,20 @@ static inline nodeent_t *ns_at(const nodeset_t *S, int tree, int l, int x, int y
 typedef struct {
     amr_quad_t *q;
     unsigned char *kind;   /* 1: check for too-coarse neighbours, 2: also too-fine */
-    int n, cap;
+    int n, cap, oom;
 } qqueue_t;
 
 static void qq_push(qqueue_t *Q, amr_quad_t q, unsigned char kind)
 {
+    if (Q->oom) return;
     if (Q->n == Q->cap) {
-        Q->cap = Q->cap ? 2 * Q->cap : 256;
-        Q->q = (amr_quad_t *)realloc(Q->q, sizeof(amr_quad_t) * (size_t)Q->cap);
-        Q->kind = (unsigned char *)realloc(Q->kind, (size_t)Q->cap);
+        const int cap = Q->cap ? 2 * Q->cap : 256;
+        amr_quad_t *nq = (amr_quad_t *)realloc(Q->q, sizeof(amr_quad_t) * (size_t)cap);
+        if (nq) Q->q = nq;
+        unsigned char *nk = nq ? (unsigned char *)realloc(Q->kind, (size_t)cap) : NULL;
+        if (nk) Q->kind = nk;
+        if (!nq || !nk) { Q->oom = 1; return; }
+        Q->cap = cap;
     }
     Q->q[Q->n] = q;
     Q->kind[Q->n] = kind;
@@ -719,16 +729,7 @@ static void emit_leaves(const amr_forest_t *f, const nodeset_t *S, emit_t *E, am
         }
         return;
     }
-    if (E->n == E->cap) {
-        E->cap *= 2;
-        E->q = (amr_quad_t *)realloc(E->q, sizeof(amr_quad_t) * (size_t)E->cap);
-        E->key = (uint64_t *)realloc(E->key, sizeof(uint64_t) * (size_t)E->cap);
-        E->tag = (int *)realloc(E->tag, sizeof(int) * (size_t)E->cap);
-        E->type = (int *)realloc(E->type, sizeof(int) * (size_t)E->cap);
-        E->mark = (unsigned char *)realloc(E->mark, (size_t)E->cap);
-        if (E->src) E->src = (int *)realloc(E->src, sizeof(int) * (size_t)E->cap);
-        if (E->move) E->soln = (double *)realloc(E->soln, sizeof(double) * (size_t)E->ps * (size_t)E->cap);
-    }
+    if (E->n >= E->cap) return;       /* cannot happen: the capacity is the exact leaf count */
     const int j = E->n++;
     E->q[j] = q;
     E->key[j] = morton(q.x, q.y);
@@ -745,8 +746,8 @@ int amr_forest_balance_marked(amr_forest_t *f)
     nodeset_t S;
     uint64_t cap = 16;
     while (cap < 4 * (uint64_t)nq) cap <<= 1;
-    ns_alloc(&S, cap);
-    qqueue_t Q = {NULL, NULL, 0, 0};
+    if (ns_alloc(&S, cap)) return -1;
+    qqueue_t Q = {NULL, NULL, 0, 0, 0};
     for (int k = 0; k < nq; ++k) {
         const amr_quad_t *q = &f->q[k];
         ns_insert(&S, q->tree * 32 + q->level, f->key[k], 1);
@@ -763,7 +764,7 @@ int amr_forest_balance_marked(amr_forest_t *f)
         if (f->mark[k]) qq_push(&Q, *q, f->mark[k] == 2 ? 2 : 1);
     }
     int refined = 0;
-    for (int it = 0; it < Q.n; ++it) {
+    for (int it = 0; it < Q.n && !S.oom && !Q.oom; ++it) {
         const amr_quad_t q = Q.q[it];
         const unsigned char kind = Q.kind[it];
         const nodeent_t *self = ns_find(&S, q.tree * 32 + q.level, morton(q.x, q.y));
@@ -776,7 +777,7 @@ int amr_forest_balance_marked(amr_forest_t *f)
         const int qtx = q.tree % f->ntx, qty = q.tree / f->ntx;
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
-                if (!dx && !dy) continue;
+                if ((!dx && !dy) || S.oom || Q.oom) continue;
                 const int sibling = q.level > 0 && (!dx || dx == sx_dir) && (!dy || dy == sy_dir);
                 if (sibling && kind != 2) continue;
                 int t, nx, ny;
@@ -816,10 +817,11 @@ int amr_forest_balance_marked(amr_forest_t *f)
             qq_push(&Q, q, 1);
         }
     }
-    if (refined) {
+    if (S.oom || Q.oom) refined = -1;      /* the forest is left as refine/coarsen made it */
+    if (refined > 0) {
         emit_t E;
         memset(&E, 0, sizeof(E));
-        E.cap = nq + 3 * refined + 16;
+        E.cap = nq + 3 * refined;           /* each balance refinement adds three leaves */
         E.q = (amr_quad_t *)malloc(sizeof(amr_quad_t) * (size_t)E.cap);
         E.key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)E.cap);
         E.tag = (int *)malloc(sizeof(int) * (size_t)E.cap);
@@ -832,6 +834,13 @@ int amr_forest_balance_marked(amr_forest_t *f)
             (void)amr_forest_soln(f);
             E.soln = (double *)malloc(sizeof(double) * (size_t)E.ps * (size_t)E.cap);
         }
+        if (!E.q || !E.key || !E.tag || !E.type || !E.mark || (f->track && !E.src) || (E.move && !E.soln)) {
+            free(E.q); free(E.key); free(E.tag); free(E.type); free(E.mark); free(E.src); free(E.soln);
+            free(S.e);
+            free(Q.q);
+            free(Q.kind);
+            return -1;
+        }
         for (int k = 0; k < nq; ++k) emit_leaves(f, &S, &E, f->q[k], k);
         free(f->q); free(f->key); free(f->tag); free(f->type); free(f->mark);
         f->q = E.q; f->key = E.key; f->tag = E.tag; f->type = E.type; f->mark = E.mark;
@@ -840,7 +849,7 @@ int amr_forest_balance_marked(amr_forest_t *f)
         f->nquad = E.n;
         f->cap = E.cap;
         f->hvalid = 0;
-    } else {
+    } else if (refined == 0) {
         memset(f->mark, 0, (size_t)(nq > 0 ? nq : 1));
     }
     free(S.e);

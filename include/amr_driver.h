@@ -61,7 +61,9 @@ void amr_indicator_gradient(amr_forest_t *f, double *ind);
 int amr_regrid(amr_forest_t *f, double tol, int allow_coarsening);
 /* The same regrid driven by given per-quadrant tags: 1 refine, 0 may be
  * coarsened with its family, any other value keeps the quadrant (no refine,
- * its family is not coarsened). */
+ * its family is not coarsened).  Returns the new number of quadrants, or -1
+ * if an allocation failed (the forest is then refined / coarsened but not
+ * balanced). */
 int amr_regrid_tags(amr_forest_t *f, const int *tags, int allow_coarsening);
 /* Initial refinement (regrid_feature with initial = 1): repeatedly tag and
  * refine, re-evaluating the initial condition on new quadrants

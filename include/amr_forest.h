@@ -74,7 +74,8 @@ int amr_forest_coarsen(amr_forest_t *f);
 int amr_forest_balance(amr_forest_t *f);
 /* The same closure after a regrid of a balanced forest, examining only the
  * quadrants created by refine / coarsen since amr_forest_clear_marks (their
- * neighbours are the only possible violations).  Returns quadrants refined. */
+ * neighbours are the only possible violations).  Returns quadrants refined,
+ * or -1 with the forest unchanged if an allocation failed. */
 int amr_forest_balance_marked(amr_forest_t *f);
 void amr_forest_clear_marks(amr_forest_t *f);
 /* 1 if the forest is 2:1 balanced across faces and corners. */
