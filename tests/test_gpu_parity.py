@@ -99,6 +99,28 @@ def test_every_tile_variant(gpu, oracle, n):
         assert parity_err(gpu.soln(), q_ref) <= TRAJ_TOL
 
 
+def test_autotune_restores_state_and_keeps_parity(gpu, oracle):
+    """the autotuner's trial steps (every tile, then both launch modes) leave
+    state and time as they were; the tile and mode it keeps meet the bar"""
+    f = riemann_forest(8, seed=11, lmax=5)
+    ci, fi, geom, soln = f.external()
+    bc = (0, 0, 1, 1)
+    setup(gpu, 8, 3, 1.0, bc, 0.3, 1.0, ci, fi, geom, soln)
+    log(f"autotune on {f.nquad} quads")
+    res = gpu.autotune(3)
+    variants = amrhip.tile_variants(8)
+    assert len(res) == len(variants) and all(ms > 0 for *_, ms in res)
+    assert gpu.tile() in [tuple(v) for v in variants]
+    assert gpu.graphs() in (True, False)
+    t, _ = gpu.time()
+    assert t[0] == 0.0
+    assert np.array_equal(gpu.soln(), soln)
+    gpu.run(3)
+    p = oracle.params(8, bc=bc, cfl=0.3, dt=1.0, h_base=1.0, scheme=3)
+    q_ref, _, _ = oracle.run(p, ci, fi, geom, soln, 3)
+    assert parity_err(gpu.soln(), q_ref) <= TRAJ_TOL
+
+
 def test_rhs_config1_vortex(gpu, oracle):
     """the isentropic vortex on the uniform level-4 forest with 8x8 patches"""
     log("vortex level 4 n=8")
