@@ -775,8 +775,12 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             }
         }
         __builtin_amdgcn_sched_barrier(0);
-        if (tid < PPB * 2 * N) {
-            const int tp = tid / (2 * N), e = tid % (2 * N);
+        // the wave that takes the patch-edge round rotates with the block
+        // index, spreading its extra round over the SIMDs
+        constexpr int NWV = (TL::BLK + 63) / 64;
+        const int et = tid - (PPB * 2 * N <= 64 ? (lb % NWV) * 64 : 0);
+        if (et >= 0 && et < PPB * 2 * N) {
+            const int tp = et / (2 * N), e = et % (2 * N);
             if (pquad[tp] >= 0) {
                 const double h = ph[tp];
                 const bool xs = e < N;                 // side 0 (x) or side 2 (y)
