@@ -16,8 +16,8 @@ import time
 
 import numpy as np
 
-from . import (Forest, Mlp, MlpTagger, Tagset, dmr_ic, dmr_inst, get_params, riemann4_ic, set_params, vortex_ic,
-               TAG_REFINE)
+from . import (Forest, Gpu, Mlp, MlpTagger, TAG_REFINE, Tagset, dmr_ic, dmr_inst, get_params, riemann4_ic,
+               set_params, vortex_ic)
 
 ROOT_LEN = 1 << 30
 
@@ -168,10 +168,11 @@ def l1_error(rho, ref):
 
 
 # ---- double-Mach reflection (config 5): [0,4] x [0,1], four unit trees ----
-def dmr_params(t_end, regrid=10):
+def dmr_params(t_end, regrid=10, verbose=0):
+    """verbose = 1: the device loop prints a progress line every 10 s"""
     set_params(nelem_subgrid=8, time_scheme=3, time_steps=10 ** 9, dt=1.0, cfl=0.25 * 0.4 * 3,
                coarse_grid_h=1.0, regrid_interval=regrid, checkpoint_interval=2 ** 31 - 1,
-               visualization_interval=2 ** 31 - 1, bc=(0, 0, 0, 0, 1, 1), t_end=t_end, verbose=0)
+               visualization_interval=2 ** 31 - 1, bc=(0, 0, 0, 0, 1, 1), t_end=t_end, verbose=verbose)
     p = get_params()
     return dmr_inst(1.0, p.cfl, p.dt, t_end)
 
@@ -185,15 +186,32 @@ def dmr_forest(lmin, lmax, tol=0.1):
 
 
 def dmr_uniform_reference(level, t_end, log=None):
+    """the plain solution on the uniform forest of `level`, advanced on one
+    device context in chunks of 16 steps (the instance freezes at t_end, as
+    in the evolve loop), logging progress every 30 s"""
     inst = dmr_params(t_end, regrid=2 ** 31 - 1)
     f = Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, 8, level, level)
     f.apply_ic(dmr_ic())
-    t0 = time.perf_counter()
-    st = f.evolve_device(0.1, 0, inst=inst)
+    ci, fi, geom, soln = f.external()
+    g = Gpu()
+    g.setup(8, 3, [inst], ci, fi, geom, soln)
+    t0 = last = time.perf_counter()
+    steps = 0
+    while True:
+        g.run(16)
+        steps += 16
+        t = float(g.time()[0][0])
+        if t >= t_end:
+            break
+        if log and time.perf_counter() - last > 30.0:
+            log(f"  uniform level-{level} reference: {steps} steps, t = {t:.4f}")
+            last = time.perf_counter()
+    f.set_soln(np.asarray(g.soln()).reshape(-1))
+    g.close()
     if log:
-        log(f"uniform level-{level} double-Mach reference: {f.nquad * 64} cells, {st.steps} steps, "
+        log(f"uniform level-{level} double-Mach reference: {f.nquad * 64} cells, {steps} steps, "
             f"{time.perf_counter() - t0:.1f} s")
-    return density_on_grid(f, level), st
+    return density_on_grid(f, level), steps
 
 
 def dmr_collect(lmin, lmax, t_end, tol=0.1, window=40, label_stride=5, tagset=None, log=None):
@@ -207,8 +225,8 @@ def dmr_collect(lmin, lmax, t_end, tol=0.1, window=40, label_stride=5, tagset=No
     return ts
 
 
-def dmr_deploy(lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None, lag=0):
-    inst = dmr_params(t_end, regrid=interval)
+def dmr_deploy(lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None, lag=0, verbose=0):
+    inst = dmr_params(t_end, regrid=interval, verbose=verbose)
     f = dmr_forest(lmin, lmax, tol)
     t0 = time.perf_counter()
     st = f.evolve_device(tol, interval, tagger=tagger, inst=inst, regrid_lag=lag)

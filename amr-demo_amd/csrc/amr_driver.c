@@ -477,7 +477,7 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
     double *ind = NULL;
     int *tags = NULL;
     int cap = 0, istep = 0;
-    double t = 0.0;
+    double t = 0.0, t_print = now_s();
     s.wall_regrid += now_s() - t0;
     while (!rc) {
         long long nsteps = (long long)p.time_steps - istep;
@@ -499,6 +499,11 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
         s.cell_stages += (double)nsteps * ncell * n * n * stages[p.time_scheme];
         if (p.t_end > 0.0 || g_outdir[0]) amrgpu_get_time(g, &t, NULL);
         s.wall_gpu += now_s() - t1;
+        if (p.verbose > 0 && now_s() - t_print > 10.0) {       /* long runs: a progress line every 10 s */
+            fprintf(stderr, "amr_evolve_device: step %d, t = %.6e, %d quadrants, %d regrids\n", istep, t, ncell,
+                    s.regrids);
+            t_print = now_s();
+        }
         const int done = istep >= p.time_steps || (p.t_end > 0.0 && t >= p.t_end);
         if (g_outdir[0] && !done) {
             const int vis = p.visualization_interval > 0 && istep % p.visualization_interval == 0;

@@ -194,7 +194,8 @@ def dmr_experiment(amrhip, args, model):
         runs.append((f"learned_every_{iv}", iv, amrhip.MlpTagger(model, args.tol, True)))
     for name, interval, tagger in runs:
         r, _ = L.dmr_deploy(2, lmax, t_end, interval, tagger, args.tol, ref, lmax,
-                            args.learned_lag if tagger is not None else 0)
+                            args.learned_lag if tagger is not None else 0,
+                            verbose=1 if lmax >= 9 else 0)      # progress lines on long runs
         out[name] = r
         log(f"double-Mach {name}: {r['steps']} steps, {r['regrids']} regrids, peak {r['peak_cells']} cells, "
             f"{r['wall_s']:.2f} s, {r['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s, L1 {r['l1_density']:.4e}")
