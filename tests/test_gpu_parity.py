@@ -297,6 +297,38 @@ def test_device_evolve_equals_host_regrid_loop(allow_coarsening):
     assert np.array_equal(fa.soln(), fb.soln())
 
 
+def test_headline_workload_full_size(gpu, oracle):
+    """the benchmark's workload at full size: the four-quadrant Riemann
+    problem, levels 3-7, 8x8 patches, regrid every 10 steps, to t = 0.25
+    (~2200 steps, ~220 regrids, ~207k cells at the end).  The state-resident
+    loop equals the interval-by-interval host loop bit for bit over the whole
+    run; on the mesh and state reached, the residual meets the 1e-12 bar
+    against the oracle and 5 SSP-RK3 steps the trajectory bar"""
+    def make():
+        return riemann_forest(8, seed=0, lmin=3, lmax=7)
+    amrhip.set_params(nelem_subgrid=8, time_scheme=3, time_steps=10 ** 9, dt=1.0, cfl=0.25 * 0.4 * 3,
+                      coarse_grid_h=1.0, regrid_interval=10, checkpoint_interval=2 ** 31 - 1,
+                      visualization_interval=2 ** 31 - 1, bc=(0, 0, 0, 0, 1, 1), t_end=0.25)
+    fb = make()
+    sb = fb.evolve_device(0.1, 10)
+    log(f"device loop: {sb.steps} steps, {sb.regrids} regrids, {fb.nquad} quads at t={sb.t}")
+    fa = make()
+    sa = fa.evolve(0.1)
+    log(f"host loop: {sa.steps} steps, {sa.regrids} regrids")
+    assert sb.t == pytest.approx(0.25, abs=0) and sb.regrids > 200 and fb.nquad * 64 > 150000
+    assert sa.steps == sb.steps and sa.regrids == sb.regrids
+    assert fa.quads() == fb.quads()
+    assert np.array_equal(fa.soln(), fb.soln())
+    ci, fi, geom, soln = fb.external()
+    p = oracle.params(8, bc=(0, 0, 0, 0), cfl=0.3, dt=1.0, h_base=1.0, scheme=3)
+    setup(gpu, 8, 3, 1.0, (0, 0, 0, 0), 0.3, 1.0, ci, fi, geom, soln)
+    assert parity_err(gpu.rhs(), oracle.rhs(p, ci, fi, geom, soln)) <= RHS_TOL
+    gpu.run(5)
+    q_ref, _, dt_ref = oracle.run(p, ci, fi, geom, soln, 5)
+    assert parity_err(gpu.soln(), q_ref) <= TRAJ_TOL
+    assert gpu.time()[1][0] == pytest.approx(dt_ref, rel=1e-12)
+
+
 def vortex_l1(f, soln, t):
     """L1 density error per unit area against the exactly advected vortex
     (amr_ic_vortex_density at the sub-cell centres, nearest periodic image)"""
