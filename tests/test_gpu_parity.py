@@ -4,6 +4,7 @@ bit).  Bars: residual per cell and field within 1e-12 relative to
 max(|r_ref|, ||r_ref_field||_inf); CFL time step bit-exact; RK trajectories
 within 1e-11 of the oracle's k-form (the GPU uses the algebraically identical
 Shu-Osher / low-storage forms); conservation to round-off."""
+import math
 import os
 
 import numpy as np
@@ -182,6 +183,28 @@ def test_conservation_periodic(gpu):
     gpu.run(50)
     m1 = totals(gpu.soln())
     assert np.all(np.abs(m1 - m0) <= 1e-12 * np.abs(m0))
+
+
+def test_conservation_level9_full_size(gpu):
+    """the benchmark's HBM-scale mesh (uniform level 9, 16.8M cells, two
+    quadrants refined so that hanging faces are present): mass, momentum and
+    energy conserved to round-off over 20 SSP-RK3 steps on the periodic box"""
+    n = 8
+    f = vortex_forest(n, 9)
+    ci, fi, geom, soln = f.external()
+    log(f"level 9: {f.nquad} quads, {f.nquad * n * n} cells")
+    setup(gpu, n, 3, 10.0, (0, 0, 0, 0), 0.3, 1.0, ci, fi, geom, soln)
+    vol = ((10.0 / 2.0 ** f.levels()) / n) ** 2
+    def totals(s):
+        # correctly rounded sums: a plain sum over 16.8M terms carries a
+        # rounding error of order 1e-11 relative, far above the drift tested
+        q = s.reshape(f.nquad, n * n, 4) * vol[:, None, None]
+        return np.array([math.fsum(q[:, :, k].ravel()) for k in range(4)])
+    m0 = totals(soln)
+    gpu.run(20)
+    m1 = totals(gpu.soln())
+    log(f"drift {m1 - m0}")
+    assert np.all(np.abs(m1 - m0) <= 1e-12 * np.maximum(np.abs(m0), np.abs(m0).max()))
 
 
 def test_batched_instances_match_single(gpu):
