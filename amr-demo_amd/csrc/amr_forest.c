@@ -647,7 +647,9 @@ static inline nodeent_t *ns_at(const nodeset_t *S, int tree, int l, int x, int y
 
 typedef struct {
     amr_quad_t *q;
-    unsigned char *kind;   /* 1: check for too-coarse neighbours, 2: also too-fine */
+    unsigned char *kind;   /* 1: leaf, check for too-coarse neighbours; 2: also too-fine;
+                              3: a refined quadrant, check for neighbours too coarse
+                              for its children */
     int n, cap, oom;
 } qqueue_t;
 
@@ -701,8 +703,8 @@ static void refine_leaf(nodeset_t *S, qqueue_t *Q, nodeent_t *e, amr_quad_t q, i
         ch.x = q.x + (c & 1) * h;
         ch.y = q.y + (c >> 1) * h;
         ns_insert(S, ch.tree * 32 + ch.level, morton(ch.x, ch.y), 1);
-        qq_push(Q, ch, 1);
     }
+    qq_push(Q, q, 3);    /* the four children share the parent's neighbourhood */
     (*count)++;
 }
 
@@ -761,14 +763,25 @@ int amr_forest_balance_marked(amr_forest_t *f)
             if (ns_find(&S, meta, key)) break;
             ns_insert(&S, meta, key, 0);
         }
-        if (f->mark[k]) qq_push(&Q, *q, f->mark[k] == 2 ? 2 : 1);
+        if (f->mark[k] == 2) {
+            qq_push(&Q, *q, 2);
+        } else if (f->mark[k] && !((q->x | q->y) & qlen(q->level))) {
+            /* the first child of a quadrant refined by this regrid: one
+             * check of the parent's neighbourhood stands for all four */
+            const int pl = qlen(q->level - 1);
+            amr_quad_t par = {q->tree, q->level - 1, q->x & ~(pl - 1), q->y & ~(pl - 1)};
+            qq_push(&Q, par, 3);
+        }
     }
     int refined = 0;
     for (int it = 0; it < Q.n && !S.oom && !Q.oom; ++it) {
         const amr_quad_t q = Q.q[it];
         const unsigned char kind = Q.kind[it];
         const nodeent_t *self = ns_find(&S, q.tree * 32 + q.level, morton(q.x, q.y));
-        if (!self->leaf) continue;                                             /* refined since */
+        if (kind != 3 && !self->leaf) continue;                                /* refined since */
+        /* lc: level of the leaves whose neighbours are checked (q's children
+         * for kind 3); a neighbour leaf is too coarse below lc - 1 */
+        const int lc = kind == 3 ? q.level + 1 : q.level;
         int recheck = 0, too_coarse_self = 0;
         const int len = qlen(q.level);
         /* q's parent is not a leaf, so every leaf inside it is at q's level
@@ -782,10 +795,10 @@ int amr_forest_balance_marked(amr_forest_t *f)
                 if (sibling && kind != 2) continue;
                 int t, nx, ny;
                 if (!near_region(f, &q, qtx, qty, dx, dy, &t, &nx, &ny)) continue;
-                if (!sibling && q.level >= 2 && !ns_at(&S, t, q.level - 1, nx, ny)) {
+                if (!sibling && lc >= 2 && !ns_at(&S, t, lc - 1, nx, ny)) {
                     /* the neighbour leaf containing (nx, ny) is coarser than
-                     * level - 1: find it and refine it */
-                    for (int l = q.level - 2; l >= 0; --l) {
+                     * lc - 1: find it and refine it */
+                    for (int l = lc - 2; l >= 0; --l) {
                         nodeent_t *e = ns_at(&S, t, l, nx, ny);
                         if (e && e->leaf) {
                             const int ln = qlen(l);
@@ -814,7 +827,7 @@ int amr_forest_balance_marked(amr_forest_t *f)
             nodeent_t *e = ns_find(&S, q.tree * 32 + q.level, morton(q.x, q.y));
             refine_leaf(&S, &Q, e, q, &refined);
         } else if (recheck) {
-            qq_push(&Q, q, 1);
+            qq_push(&Q, q, kind == 3 ? 3 : 1);
         }
     }
     if (S.oom || Q.oom) refined = -1;      /* the forest is left as refine/coarsen made it */
