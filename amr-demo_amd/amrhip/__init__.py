@@ -93,7 +93,11 @@ EXPORTS = {
     "amr_forest_track_end": (_i, [_vp]),
     "amrgpu_indicator": (_i, [_vp, _pd]),
     "amrgpu_remesh": (_i, [_vp, _i, _pi, _pi, _i, _pi, _pi]),
+    "amrgpu_remesh_nbr": (_i, [_vp, _i, _pi, _pi, _pi, _pi]),
     "amrgpu_set_geometry": (_i, [_vp, _pd]),
+    "amrgpu_set_origins": (_i, [_vp, _pd]),
+    "amrgpu_nbr_from_faces": (_i, [_i, _i, _pi, _pi]),
+    "amr_forest_build_nbr": (_i, [_vp, _pi, _pi, _pd]),
     "amrgpu_device_state": (_i, [_vp, _vp, _vp, _pi, _pi, _vp]),
     # include/amr_tagger.h
     "amr_mlp_create": (_vp, [C.c_ulonglong]),
@@ -328,6 +332,19 @@ class Forest:
         geom = np.ctypeslib.as_array(g, shape=(12 * nc,)).copy()
         soln = np.ctypeslib.as_array(s, shape=(nd,)).copy()
         return cell_info.astype(np.int32), face_info.astype(np.int32), geom, soln
+
+    def mesh_table(self):
+        """(nbr[8 nquad], level[nquad], org[2 nquad]): the device mesh table
+        built straight from the forest (amr_forest_build_nbr)"""
+        nq = self.nquad
+        nbr = np.empty(8 * nq, np.int32)
+        level = np.empty(nq, np.int32)
+        org = np.empty(2 * nq, np.float64)
+        rc = self.L.amr_forest_build_nbr(self.h, nbr.ctypes.data_as(_pi), level.ctypes.data_as(_pi),
+                                         org.ctypes.data_as(_pd))
+        if rc != 0:
+            raise ValueError("forest is not 2:1 balanced / inconsistent")
+        return nbr, level, org
 
     def set_soln(self, flat):
         v = self.soln().reshape(-1)
@@ -672,6 +689,15 @@ def get_params():
     p = PhysicsParams()
     lib().physics_gpu_get_params(C.byref(p))
     return p
+
+
+def nbr_from_faces(ncell, face_info):
+    """the device mesh table the library derives from reference face_info"""
+    fi = np.ascontiguousarray(face_info, dtype=np.int32)
+    nbr = np.empty(8 * ncell, np.int32)
+    check(lib().amrgpu_nbr_from_faces(ncell, len(fi) // 11, fi.ctypes.data_as(_pi), nbr.ctypes.data_as(_pi)),
+          "amrgpu_nbr_from_faces")
+    return nbr
 
 
 def tile_variants(n):

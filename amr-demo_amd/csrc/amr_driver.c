@@ -417,11 +417,37 @@ static int ext_of(amr_forest_t *f, ext_t *e)
 
 /* Host half of a lagged regrid: topology and boundary arrays of one forest on
  * a thread of its own while the GPU keeps stepping the old mesh. */
+/* the device mesh table of a forest (amr_forest_build_nbr), for remeshes
+ * that skip the reference-format arrays */
+typedef struct {
+    int nq, rc;
+    int *nbr, *level;
+    double *org;
+} tab_t;
+
+static int tab_of(amr_forest_t *f, tab_t *t)
+{
+    const int nq = amr_forest_num_quads(f);
+    t->nq = nq;
+    t->nbr = (int *)malloc(sizeof(int) * 8 * (size_t)(nq > 0 ? nq : 1));
+    t->level = (int *)malloc(sizeof(int) * (size_t)(nq > 0 ? nq : 1));
+    t->org = (double *)malloc(sizeof(double) * 2 * (size_t)(nq > 0 ? nq : 1));
+    t->rc = (!t->nbr || !t->level || !t->org) ? -1 : amr_forest_build_nbr(f, t->nbr, t->level, t->org);
+    return t->rc;
+}
+
+static void tab_free(tab_t *t)
+{
+    free(t->nbr); free(t->level); free(t->org);
+    t->nbr = t->level = NULL;
+    t->org = NULL;
+}
+
 typedef struct {
     amr_forest_t *f;
     const int *tags;
     int allow, ok;
-    ext_t ext;
+    tab_t tab;
     double wall;
 } lag_job_t;
 
@@ -432,7 +458,7 @@ static void *lag_regrid_thread(void *arg)
     amr_forest_track_begin(j->f, 0);
     const int r = amr_regrid_tags(j->f, j->tags, j->allow);
     j->ok = amr_forest_track_end(j->f) && r >= 0;
-    ext_of(j->f, &j->ext);
+    tab_of(j->f, &j->tab);
     j->wall = now_s() - t0;
     return NULL;
 }
@@ -442,18 +468,18 @@ double amr_now(void) { return now_s(); }
 int amr_device_regrid(amr_forest_t *f, amrgpu_ctx *g, const int *tags, int allow_coarsening, double *t_forest,
                       double *t_remesh)
 {
-    int ncell, nghost, nface, dof;
-    int *cell_info, *face_info;
-    double *geom, *soln;
     double t0 = now_s();
     amr_forest_track_begin(f, 0);
     const int r = amr_regrid_tags(f, tags, allow_coarsening);
     if (!amr_forest_track_end(f) || r < 0) return AMRGPU_ERR_MESH;
-    if (amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom, &soln))
-        return AMRGPU_ERR_MESH;
+    /* the device takes its mesh table straight from the forest: the
+     * reference-format arrays are not needed between device steps */
+    tab_t tb;
+    if (tab_of(f, &tb)) { tab_free(&tb); return AMRGPU_ERR_MESH; }
     double t1 = now_s();
-    int rc = amrgpu_remesh(g, ncell, cell_info, NULL, nface, face_info, amr_forest_track_src(f));
-    if (!rc) rc = amrgpu_set_geometry(g, geom);
+    int rc = amrgpu_remesh_nbr(g, tb.nq, tb.level, NULL, tb.nbr, amr_forest_track_src(f));
+    if (!rc) rc = amrgpu_set_origins(g, tb.org);
+    tab_free(&tb);
     if (t_forest) *t_forest += t1 - t0;
     if (t_remesh) *t_remesh += now_s() - t1;
     return rc;
@@ -558,11 +584,12 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
             s.wall_gpu += t5 - t4;
             if (threaded) pthread_join(th, NULL);
             s.wall_forest += job.wall;
-            if (!rc && (!job.ok || job.ext.rc)) rc = AMRGPU_ERR_MESH;
+            if (!rc && (!job.ok || job.tab.rc)) rc = AMRGPU_ERR_MESH;
             const double t6 = now_s();
-            if (!rc) rc = amrgpu_remesh(g, job.ext.ncell, job.ext.cell_info, NULL, job.ext.nface,
-                                        job.ext.face_info, amr_forest_track_src(f));
-            if (!rc) rc = amrgpu_set_geometry(g, job.ext.geom);
+            if (!rc) rc = amrgpu_remesh_nbr(g, job.tab.nq, job.tab.level, NULL, job.tab.nbr,
+                                            amr_forest_track_src(f));
+            if (!rc) rc = amrgpu_set_origins(g, job.tab.org);
+            tab_free(&job.tab);
             s.wall_remesh += now_s() - t6;
             /* the regrid wall counts what the host made the GPU wait for:
              * tagging, the join after the overlapped steps, the remesh */

@@ -393,6 +393,30 @@ int amr_forest_build_external(amr_forest_t *f, int *ncell, int *nghost, int *nfa
     return err ? -1 : 0;
 }
 
+int amr_forest_build_nbr(amr_forest_t *f, int *nbr, int *level, double *org)
+{
+    const int nq = f->nquad;
+    if (!f->hvalid) hash_build(f);               /* shared read-only below */
+    int err = 0;
+#pragma omp parallel for schedule(static) reduction(| : err) if (nq >= PAR_MIN && t_parallel)
+    for (int k = 0; k < nq; ++k) {
+        double g[12];
+        amr_forest_quad_geom(f, k, g);
+        level[k] = f->q[k].level;
+        org[2 * (size_t)k] = g[0];
+        org[2 * (size_t)k + 1] = g[1];
+        for (int s = 0; s < 4; ++s) {
+            int nb[2] = {-1, -1};
+            const int c = classify_side(f, k, s, nb);
+            int *e = &nbr[8 * (size_t)k + 2 * s];
+            if (c < 0) { err = 1; e[0] = e[1] = -1; continue; }
+            e[0] = c == 0 ? -1 : nb[0];                 /* boundary; same size or coarser; finer */
+            e[1] = c == 3 ? nb[1] : -1;
+        }
+    }
+    return err ? -1 : 0;
+}
+
 /* Replace the quadrant array by applying refine flags (one level). */
 int amr_forest_refine(amr_forest_t *f, const int *flags)
 {

@@ -1644,15 +1644,21 @@ static int ensure_capacity(amrgpu_ctx *c, int n, int ncell, int ninst, bool need
     return AMRGPU_OK;
 }
 
+// Levels of the reference's cell_info (already validated).
+static std::vector<int> levels_of(int ncell, const int *cell_info)
+{
+    std::vector<int> level(ncell);
+    for (int q = 0; q < ncell; ++q) level[q] = cell_info[3 * q + 1];
+    return level;
+}
+
 // Upload the face table, levels and instances of a mesh already validated.
-static int upload_mesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst,
+static int upload_mesh(amrgpu_ctx *c, int ncell, const int *level, const int *cell_inst,
                        const std::vector<int> &nbr)
 {
-    std::vector<int> level(ncell), qi(ncell, 0);
-    for (int q = 0; q < ncell; ++q) {
-        level[q] = cell_info[3 * q + 1];
-        if (cell_inst) qi[q] = cell_inst[q];
-    }
+    std::vector<int> qi(ncell, 0);
+    if (cell_inst)
+        for (int q = 0; q < ncell; ++q) qi[q] = cell_inst[q];
     c->nquad = ncell;
     if (c->variant < 0 || kVariants[c->variant].n != c->n) c->variant = default_variant(c->n);
     c->nblocks = (ncell + kVariants[c->variant].ppb - 1) / kVariants[c->variant].ppb;
@@ -1661,7 +1667,7 @@ static int upload_mesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int
         return AMRGPU_ERR_ARG;
     }
     HIPCHK(hipMemcpyAsync(c->d_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_level, level.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_level, level, sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_inst, qi.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return AMRGPU_OK;
@@ -1728,7 +1734,7 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
     HIPCHK(hipMemcpyAsync(c->d_t, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_tstep, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_dt_cur, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
-    rc = upload_mesh(c, ncell, cell_info, cell_inst, nbr);
+    rc = upload_mesh(c, ncell, levels_of(ncell, cell_info).data(), cell_inst, nbr);
     if (rc) return rc;
     c->org_ok = false;
     if (geom) {
@@ -1763,27 +1769,25 @@ int amrgpu_indicator(amrgpu_ctx *c, double *ind)
     return AMRGPU_OK;
 }
 
-int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst, int nface,
-                  const int *face_info, const int *src)
+// The source map of a remesh refers to the old mesh only.
+static int check_src(int ncell, const int *src, int nold, const char *who)
 {
-    if (c) c->org_ok = false;          // the new quadrants' coordinates come with amrgpu_set_geometry
-    if (!c || !c->nquad || !cell_info || !face_info || !src || ncell <= 0 || nface < 0) {
-        set_err("amrgpu_remesh: bad arguments");
-        return AMRGPU_ERR_ARG;
-    }
-    const int n2 = c->n * c->n, nold = c->nquad;
     for (int q = 0; q < ncell; ++q) {
         const int s = src[q];
         if ((s >= 0 && s >= nold) || (s < 0 && (-s - 1) + 3 >= nold)) {
-            set_err("amrgpu_remesh: quadrant %d source %d outside the old mesh (%d)", q, s, nold);
+            set_err("%s: quadrant %d source %d outside the old mesh (%d)", who, q, s, nold);
             return AMRGPU_ERR_ARG;
         }
     }
-    int rc = validate_cells(n2, c->ninst, ncell, cell_info, cell_inst, "amrgpu_remesh");
-    if (rc) return rc;
-    std::vector<int> nbr;
-    rc = build_nbr(ncell, nface, face_info, nbr);
-    if (rc) return rc;
+    return AMRGPU_OK;
+}
+
+// Remap the state onto a validated new mesh and upload its table.
+static int remesh_to(amrgpu_ctx *c, int ncell, const int *level, const int *cell_inst,
+                     const std::vector<int> &nbr, const int *src)
+{
+    const int n2 = c->n * c->n, nold = c->nquad;
+    int rc = AMRGPU_OK;
     HIPCHK(hipStreamSynchronize(c->stream));
     free_graphs(c);
     const long long ncells = (long long)ncell * n2;
@@ -1824,7 +1828,77 @@ int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cel
         (void)hipFree(map);
         c->cur = 0;
     }
-    return upload_mesh(c, ncell, cell_info, cell_inst, nbr);
+    return upload_mesh(c, ncell, level, cell_inst, nbr);
+}
+
+int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst, int nface,
+                  const int *face_info, const int *src)
+{
+    if (c) c->org_ok = false;          // the new quadrants' coordinates come with amrgpu_set_geometry
+    if (!c || !c->nquad || !cell_info || !face_info || !src || ncell <= 0 || nface < 0) {
+        set_err("amrgpu_remesh: bad arguments");
+        return AMRGPU_ERR_ARG;
+    }
+    int rc = check_src(ncell, src, c->nquad, "amrgpu_remesh");
+    if (!rc) rc = validate_cells(c->n * c->n, c->ninst, ncell, cell_info, cell_inst, "amrgpu_remesh");
+    if (rc) return rc;
+    std::vector<int> nbr;
+    rc = build_nbr(ncell, nface, face_info, nbr);
+    if (rc) return rc;
+    return remesh_to(c, ncell, levels_of(ncell, cell_info).data(), cell_inst, nbr, src);
+}
+
+int amrgpu_remesh_nbr(amrgpu_ctx *c, int ncell, const int *level, const int *cell_inst, const int *nbr,
+                      const int *src)
+{
+    if (c) c->org_ok = false;          // the new quadrants' coordinates come with amrgpu_set_origins
+    if (!c || !c->nquad || !level || !nbr || !src || ncell <= 0) {
+        set_err("amrgpu_remesh_nbr: bad arguments");
+        return AMRGPU_ERR_ARG;
+    }
+    int rc = check_src(ncell, src, c->nquad, "amrgpu_remesh_nbr");
+    if (rc) return rc;
+    // every id the kernels will index with must lie in the new mesh
+    for (int q = 0; q < ncell; ++q) {
+        if (level[q] < 0 || level[q] > 29) {
+            set_err("amrgpu_remesh_nbr: quadrant %d level %d", q, level[q]);
+            return AMRGPU_ERR_MESH;
+        }
+        if (cell_inst && (cell_inst[q] < 0 || cell_inst[q] >= c->ninst)) {
+            set_err("amrgpu_remesh_nbr: quadrant %d instance %d", q, cell_inst[q]);
+            return AMRGPU_ERR_ARG;
+        }
+        for (int sd = 0; sd < 4; ++sd) {
+            const int na = nbr[8 * (size_t)q + 2 * sd], nb = nbr[8 * (size_t)q + 2 * sd + 1];
+            if (na < -1 || na >= ncell || nb < -1 || nb >= ncell || (nb >= 0 && na < 0)) {
+                set_err("amrgpu_remesh_nbr: quadrant %d side %d neighbours %d %d", q, sd, na, nb);
+                return AMRGPU_ERR_MESH;
+            }
+        }
+    }
+    return remesh_to(c, ncell, level, cell_inst, std::vector<int>(nbr, nbr + 8 * (size_t)ncell), src);
+}
+
+int amrgpu_set_origins(amrgpu_ctx *c, const double *org)
+{
+    if (!c || !c->nquad || !org) { set_err("amrgpu_set_origins: no mesh"); return AMRGPU_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(c->d_org, org, sizeof(double) * 2 * (size_t)c->nquad, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->org_ok = true;
+    return AMRGPU_OK;
+}
+
+int amrgpu_nbr_from_faces(int ncell, int nface, const int *face_info, int *nbr)
+{
+    if (ncell <= 0 || nface < 0 || !face_info || !nbr) {
+        set_err("amrgpu_nbr_from_faces: bad arguments");
+        return AMRGPU_ERR_ARG;
+    }
+    std::vector<int> t;
+    int rc = build_nbr(ncell, nface, face_info, t);
+    if (rc) return rc;
+    memcpy(nbr, t.data(), sizeof(int) * t.size());
+    return AMRGPU_OK;
 }
 
 int amrgpu_upload_soln(amrgpu_ctx *c, const double *soln)

@@ -59,6 +59,15 @@ void amr_forest_quad_geom(const amr_forest_t *f, int k, double xyz[12]);
  * call or destroy).  Returns 0 on success. */
 int amr_forest_build_external(amr_forest_t *f, int *ncell, int *nghost, int *nface, int *dof,
                               int **cell_info, int **face_info, double **geom, double **soln);
+/* The device mesh table of the current forest, without the reference's
+ * boundary arrays (what amrgpu_remesh_nbr takes): per quadrant k and side s
+ * (-x, +x, -y, +y) nbr[8k + 2s] = a, nbr[8k + 2s + 1] = b with a < 0 a
+ * physical boundary, b < 0 one neighbour a (same size, or the coarser one),
+ * else the two finer neighbours a, b in increasing coordinate along the side;
+ * level[k]; org[2k], org[2k + 1] the lower-left corner.  The same table as
+ * amrgpu_nbr_from_faces makes of amr_forest_build_external's face_info.
+ * Arrays of 8, 1 and 2 entries per quadrant.  Returns 0, -1 if inconsistent. */
+int amr_forest_build_nbr(amr_forest_t *f, int *nbr, int *level, double *org);
 
 /* Refine every quadrant with flags[k] != 0 (and level < max_level) by one
  * level; children copy the parent patch (solver_refine_operator_2d) and

@@ -158,9 +158,22 @@ int amrgpu_indicator(amrgpu_ctx *c, double *ind);
  * settings and time are kept. */
 int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst, int nface,
                   const int *face_info, const int *src);
+/* The same remesh given the device mesh table directly (levels, instances
+ * and nbr as amr_forest_build_nbr makes them) instead of the reference's
+ * cell_info / face_info; every neighbour id is checked to lie in the new
+ * mesh.  No counterpart in the reference: it spares a driver that owns the
+ * forest the reference-format arrays on every regrid. */
+int amrgpu_remesh_nbr(amrgpu_ctx *c, int ncell, const int *level, const int *cell_inst, const int *nbr,
+                      const int *src);
 /* Quadrant coordinates of the current mesh (geom as amrgpu_setup) for the
  * position-dependent wall types; a remesh drops them until this is called. */
 int amrgpu_set_geometry(amrgpu_ctx *c, const double *geom);
+/* The same from the quadrants' lower-left corners (org[2k], org[2k + 1]). */
+int amrgpu_set_origins(amrgpu_ctx *c, const double *org);
+/* The device mesh table that amrgpu_setup / amrgpu_remesh derive from the
+ * reference's face_info (nbr[8 ncell], format of amr_forest_build_nbr);
+ * host only, no device needed.  Rejects inconsistent face tables. */
+int amrgpu_nbr_from_faces(int ncell, int nface, const int *face_info, int *nbr);
 /* Residual rhs = -R/V of the current state (reference layout, host out). */
 int amrgpu_rhs(amrgpu_ctx *c, double *rhs);
 /* CFL minimum of the current state per instance (no dt cap). */

@@ -91,6 +91,26 @@ def test_balance_and_face_table(seed):
     assert np.all(seen == 1), "every quadrant side is described exactly once"
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_direct_mesh_table_equals_face_table(seed):
+    """amr_forest_build_nbr (the device table straight from the forest) is the
+    table the library derives from the reference-format face_info, and its
+    levels and origins are cell_info's levels and geom's lower-left corners"""
+    f = random_forest(seed)
+    ci, fi, geom, _ = f.external()
+    nbr, level, org = f.mesh_table()
+    assert np.array_equal(nbr, amrhip.nbr_from_faces(f.nquad, fi))
+    assert np.array_equal(level, ci.reshape(-1, 3)[:, 1])
+    assert np.array_equal(org.reshape(-1, 2), geom.reshape(-1, 4, 3)[:, 0, :2])
+    # a planted inconsistency in face_info is rejected, not turned into a table
+    bad = fi.copy().reshape(-1, 11)
+    k = int(np.nonzero(bad[:, 0] == 2)[0][0])
+    bad[k, 3] = bad[k, 2]
+    bad[k, 1] ^= 1
+    with pytest.raises(amrhip.GpuError):
+        amrhip.nbr_from_faces(f.nquad, bad.reshape(-1))
+
+
 def test_refine_copies_and_coarsen_averages(oracle):
     import pyoracle
     f = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 1.0, 4, 1, 3)
