@@ -7,6 +7,7 @@
 #include "amr_hip.h"
 #include "amr_internal.h"
 
+#include <limits.h>
 #include <pthread.h>
 #include <stdio.h>
 #include <sys/stat.h>
@@ -415,8 +416,6 @@ static int ext_of(amr_forest_t *f, ext_t *e)
     return e->rc;
 }
 
-/* Host half of a lagged regrid: topology and boundary arrays of one forest on
- * a thread of its own while the GPU keeps stepping the old mesh. */
 /* the device mesh table of a forest (amr_forest_build_nbr), for remeshes
  * that skip the reference-format arrays */
 typedef struct {
@@ -443,6 +442,8 @@ static void tab_free(tab_t *t)
     t->org = NULL;
 }
 
+/* Host half of a lagged regrid: topology and mesh table of one forest on a
+ * thread of its own while the GPU keeps stepping the old mesh. */
 typedef struct {
     amr_forest_t *f;
     const int *tags;
@@ -622,11 +623,15 @@ typedef struct {
     int *ci, *fi, *cinst, *src;
     double *geom, *soln;
     size_t cap_c, cap_f;
+    int *nbr, *lev, *tinst, *tsrc;     /* concatenated mesh tables (regrids) */
+    double *org;
+    size_t cap_t;
 } batch_bufs_t;
 
 static void batch_free(batch_bufs_t *b)
 {
     free(b->ci); free(b->fi); free(b->cinst); free(b->src); free(b->geom); free(b->soln);
+    free(b->nbr); free(b->lev); free(b->tinst); free(b->tsrc); free(b->org);
     memset(b, 0, sizeof(*b));
 }
 
@@ -697,14 +702,59 @@ static int batch_build(const ext_t *ext, int ninst, int n, int *base, batch_bufs
     return AMRGPU_OK;
 }
 
+/* Concatenated mesh tables of all forests from each forest's own table
+ * tab[k] (quadrants of instance k at base[k] .. base[k+1]-1, neighbour ids
+ * shifted by the same offsets), with the instance of every quadrant. */
+static int batch_build_tab(const tab_t *tab, int ninst, int *base, batch_bufs_t *b, int *ncell_out)
+{
+    size_t nc = 0;
+    for (int k = 0; k < ninst; ++k) {
+        if (tab[k].rc) return AMRGPU_ERR_MESH;
+        nc += (size_t)tab[k].nq;
+    }
+    if (nc > (size_t)INT_MAX / 8) return AMRGPU_ERR_ARG;
+    if (nc > b->cap_t || !b->nbr) {
+        const size_t c = nc + nc / 2 + 1;
+        int *nbr = (int *)realloc(b->nbr, sizeof(int) * 8 * c);
+        if (nbr) b->nbr = nbr;
+        int *lev = (int *)realloc(b->lev, sizeof(int) * c);
+        if (lev) b->lev = lev;
+        double *org = (double *)realloc(b->org, sizeof(double) * 2 * c);
+        if (org) b->org = org;
+        int *tinst = (int *)realloc(b->tinst, sizeof(int) * c);
+        if (tinst) b->tinst = tinst;
+        int *tsrc = (int *)realloc(b->tsrc, sizeof(int) * c);
+        if (tsrc) b->tsrc = tsrc;
+        if (!nbr || !lev || !org || !tinst || !tsrc) return AMRGPU_ERR_ARG;
+        b->cap_t = c;
+    }
+    nc = 0;
+    for (int k = 0; k < ninst; ++k) {
+        const int nq = tab[k].nq, off = (int)nc;
+        base[k] = off;
+        for (size_t e = 0; e < 8 * (size_t)nq; ++e) {
+            const int v = tab[k].nbr[e];
+            b->nbr[8 * nc + e] = v >= 0 ? v + off : v;
+        }
+        memcpy(b->lev + nc, tab[k].level, sizeof(int) * (size_t)nq);
+        memcpy(b->org + 2 * nc, tab[k].org, sizeof(double) * 2 * (size_t)nq);
+        for (int q = 0; q < nq; ++q) b->tinst[nc + (size_t)q] = k;
+        nc += (size_t)nq;
+    }
+    base[ninst] = (int)nc;
+    *ncell_out = (int)nc;
+    return AMRGPU_OK;
+}
+
 /* Per-instance regrid work of one batch regrid, shared by the host threads:
  * instance k (taken in turn from a counter) is regridded by its tags when it
- * has not reached its stop, and its boundary arrays are rebuilt. */
+ * has not reached its stop, and its mesh table is rebuilt when it changed
+ * (or was never built). */
 typedef struct {
     amr_forest_t **f;
     const int *done, *tags, *obase;
     int allow, ninst;
-    ext_t *ext;
+    tab_t *tab;
     int *ok;
     int next;
     pthread_mutex_t lock;
@@ -725,7 +775,10 @@ static void *regrid_worker(void *arg)
             const int r = amr_regrid_tags(j->f[k], j->tags + j->obase[k], j->allow);
             j->ok[k] = amr_forest_track_end(j->f[k]) && r >= 0;
         }
-        ext_of(j->f[k], &j->ext[k]);
+        if (!j->done[k] || !j->tab[k].nbr || j->tab[k].rc) {
+            tab_free(&j->tab[k]);
+            tab_of(j->f[k], &j->tab[k]);
+        }
     }
     return NULL;
 }
@@ -782,6 +835,7 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
     int *done = (int *)calloc((size_t)ninst, sizeof(int));
     double *t = (double *)calloc((size_t)ninst, sizeof(double));
     ext_t *ext = (ext_t *)calloc((size_t)ninst, sizeof(ext_t));
+    tab_t *tab = (tab_t *)calloc((size_t)ninst, sizeof(tab_t));
     int *ok = (int *)calloc((size_t)ninst, sizeof(int));
     int *obase = (int *)malloc(sizeof(int) * (size_t)(ninst + 1));
     batch_bufs_t b;
@@ -791,7 +845,7 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
     int ncell = 0, nface = 0, rc = 0, istep = 0, cap = 0;
     amrgpu_ctx *g = NULL;
     double t0 = now_s();
-    if (!st || !base || !done || !t || !ext || !ok || !obase) rc = AMRGPU_ERR_ARG;
+    if (!st || !base || !done || !t || !ext || !tab || !ok || !obase) rc = AMRGPU_ERR_ARG;
     for (int k = 0; k < ninst && !rc; ++k) rc = ext_of(f[k], &ext[k]);
     if (!rc) rc = batch_build(ext, ninst, n, base, &b, &ncell, &nface, 1);
     if (!rc) {
@@ -800,6 +854,12 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
     }
     if (!rc)
         rc = amrgpu_setup(g, n, p.time_scheme, ninst, inst, ncell, b.ci, b.cinst, nface, b.fi, b.geom, b.soln);
+    {   /* the reference-format arrays serve the setup only; regrids use the mesh tables */
+        free(b.ci); free(b.fi); free(b.cinst); free(b.src); free(b.geom); free(b.soln);
+        b.ci = b.fi = b.cinst = b.src = NULL;
+        b.geom = b.soln = NULL;
+        b.cap_c = b.cap_f = 0;
+    }
     if (!rc && o && o->tile_ppb > 0) rc = amrgpu_set_tile(g, o->tile_ppb, o->tile_minw, o->tile_design);
     for (int k = 0; k < ninst && !rc; ++k) st[k].peak_quads = base[k + 1] - base[k];
     tot.peak_quads = ncell;
@@ -857,7 +917,7 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
         memset(&job, 0, sizeof(job));
         job.f = f; job.done = done; job.tags = tags; job.obase = obase;
         job.allow = o ? o->allow_coarsening : 1;
-        job.ninst = ninst; job.ext = ext; job.ok = ok;
+        job.ninst = ninst; job.tab = tab; job.ok = ok;
         const int lag = o ? o->regrid_lag : 0;
         double t_gpu_lag = 0.0;
         if (lag > 0 && lag < interval) {
@@ -893,22 +953,22 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
             if (!ok[k]) rc = AMRGPU_ERR_MESH;
             if (!done[k]) st[k].regrids++;
         }
-        if (!rc) rc = batch_build(ext, ninst, n, base, &b, &ncell, &nface, 0);
+        if (!rc) rc = batch_build_tab(tab, ninst, base, &b, &ncell);
         for (int k = 0; k < ninst && !rc; ++k) {
             const int nk = base[k + 1] - base[k];
             if (done[k]) {
-                for (int q = 0; q < nk; ++q) b.src[base[k] + q] = obase[k] + q;
+                for (int q = 0; q < nk; ++q) b.tsrc[base[k] + q] = obase[k] + q;
             } else {
                 const int *s = amr_forest_track_src(f[k]);
                 for (int q = 0; q < nk; ++q)
-                    b.src[base[k] + q] = s[q] >= 0 ? obase[k] + s[q] : -(obase[k] + (-s[q] - 1)) - 1;
+                    b.tsrc[base[k] + q] = s[q] >= 0 ? obase[k] + s[q] : -(obase[k] + (-s[q] - 1)) - 1;
             }
             if (nk > st[k].peak_quads) st[k].peak_quads = nk;
         }
         double t4 = now_s();
         tot.wall_forest += t4 - t3 - t_gpu_lag;
-        if (!rc) rc = amrgpu_remesh(g, ncell, b.ci, b.cinst, nface, b.fi, b.src);
-        if (!rc) rc = amrgpu_set_geometry(g, b.geom);
+        if (!rc) rc = amrgpu_remesh_nbr(g, ncell, b.lev, b.tinst, b.nbr, b.tsrc);
+        if (!rc) rc = amrgpu_set_origins(g, b.org);
         tot.wall_remesh += now_s() - t4;
         tot.regrids++;
         if (ncell > tot.peak_quads) tot.peak_quads = ncell;
@@ -938,6 +998,8 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
     if (per_inst && st) memcpy(per_inst, st, sizeof(amr_evolve_stats_t) * (size_t)ninst);
     if (total) *total = tot;
     batch_free(&b);
-    free(st); free(base); free(done); free(t); free(ind); free(tags); free(ext); free(ok); free(obase);
+    if (tab)
+        for (int k = 0; k < ninst; ++k) tab_free(&tab[k]);
+    free(st); free(base); free(done); free(t); free(ind); free(tags); free(ext); free(tab); free(ok); free(obase);
     return rc;
 }
