@@ -148,33 +148,25 @@ static inline double jump(double a, double b)
     return fabs(a - b) / (0.5 * (a + b));
 }
 
-/* neighbour edge densities of side s of quadrant k, index-aligned */
-static void edge_rho(amr_forest_t *f, const int *fi_side, const double *soln, int n, int s,
-                     int nb, double *out)
+/* sub-cell (i, j) at position t along side s (-x, +x, -y, +y) of a patch */
+static inline void side_cell(int n, int s, int t, int *i, int *j)
 {
-    (void)fi_side;
-    const double *Q = &soln[(size_t)4 * n * n * nb];
-    for (int t = 0; t < n; ++t) {
-        int i, j;
-        switch (s) {
-        case 0: i = n - 1; j = t; break;
-        case 1: i = 0; j = t; break;
-        case 2: i = t; j = n - 1; break;
-        default: i = t; j = 0; break;
-        }
-        out[t] = Q[4 * (i + n * j)];
+    switch (s) {
+    case 0: *i = 0; *j = t; break;
+    case 1: *i = n - 1; *j = t; break;
+    case 2: *i = t; *j = 0; break;
+    default: *i = t; *j = n - 1; break;
     }
-    (void)f;
 }
 
-void amr_indicator_gradient(amr_forest_t *f, double *ind)
+int amr_indicator_gradient(amr_forest_t *f, double *ind)
 {
     int ncell, nghost, nface, dof;
     int *cell_info, *face_info;
     double *geom, *soln;
-    amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom, &soln);
-    int n = amr_forest_nelem_subgrid(f);
-    double *e1 = (double *)malloc(sizeof(double) * n);
+    if (amr_forest_build_external(f, &ncell, &nghost, &nface, &dof, &cell_info, &face_info, &geom, &soln))
+        return -1;
+    const int n = amr_forest_nelem_subgrid(f);
     /* interior pairs */
     for (int k = 0; k < ncell; ++k) {
         const double *Q = &soln[(size_t)4 * n * n * k];
@@ -187,34 +179,31 @@ void amr_indicator_gradient(amr_forest_t *f, double *ind)
             }
         ind[k] = m;
     }
-    /* across faces: both sides see the same pairs */
+    /* across faces: both sides see the same pairs (the neighbour's edge
+     * cells lie on its opposite side s ^ 1, index-aligned) */
     for (int fc = 0; fc < nface; ++fc) {
         const int *fi = &face_info[11 * fc];
         if (fi[0] == 1) continue;
-        int s = fi[1], a = fi[2];
+        const int s = fi[1], a = fi[2];
         const double *Qa = &soln[(size_t)4 * n * n * a];
-        int nbs[2] = {fi[3], fi[0] == 3 ? fi[4] : -1};
+        const int nbs[2] = {fi[3], fi[0] == 3 ? fi[4] : -1};
         for (int h = 0; h < 2; ++h) {
-            int b = nbs[h];
+            const int b = nbs[h];
             if (b < 0) continue;
-            edge_rho(f, fi, soln, n, s, b, e1);
+            const double *Qb = &soln[(size_t)4 * n * n * b];
             double m = 0.0;
             for (int t = 0; t < n; ++t) {
-                int i, j;
-                switch (s) {
-                case 0: i = 0; j = t; break;
-                case 1: i = n - 1; j = t; break;
-                case 2: i = t; j = 0; break;
-                default: i = t; j = n - 1; break;
-                }
-                double v = jump(Qa[4 * (i + n * j)], e1[t]);
+                int i, j, ib, jb;
+                side_cell(n, s, t, &i, &j);
+                side_cell(n, s ^ 1, t, &ib, &jb);
+                double v = jump(Qa[4 * (i + n * j)], Qb[4 * (ib + n * jb)]);
                 if (v > m) m = v;
             }
             if (m > ind[a]) ind[a] = m;
             if (m > ind[b]) ind[b] = m;
         }
     }
-    free(e1);
+    return 0;
 }
 
 int amr_regrid_tags(amr_forest_t *f, const int *tags, int allow_coarsening)
@@ -244,9 +233,11 @@ int amr_regrid(amr_forest_t *f, double tol, int allow_coarsening)
     int nq = amr_forest_num_quads(f);
     double *ind = (double *)malloc(sizeof(double) * (size_t)(nq > 0 ? nq : 1));
     int *tags = (int *)malloc(sizeof(int) * (size_t)(nq > 0 ? nq : 1));
-    amr_indicator_gradient(f, ind);
-    for (int k = 0; k < nq; ++k) tags[k] = ind[k] > tol;
-    int r = amr_regrid_tags(f, tags, allow_coarsening);
+    int r = -1;
+    if (ind && tags && !amr_indicator_gradient(f, ind)) {
+        for (int k = 0; k < nq; ++k) tags[k] = ind[k] > tol;
+        r = amr_regrid_tags(f, tags, allow_coarsening);
+    }
     free(tags);
     free(ind);
     return r;
@@ -254,12 +245,15 @@ int amr_regrid(amr_forest_t *f, double tol, int allow_coarsening)
 
 int amr_regrid_initial(amr_forest_t *f, const amr_ic_t *ic, double tol)
 {
-    int n = amr_forest_nelem_subgrid(f);
     for (int pass = 0; pass < AMR_MAXLEVEL; ++pass) {
         int nq = amr_forest_num_quads(f);
-        double *ind = (double *)malloc(sizeof(double) * (size_t)nq);
-        int *flags = (int *)malloc(sizeof(int) * (size_t)nq);
-        amr_indicator_gradient(f, ind);
+        double *ind = (double *)malloc(sizeof(double) * (size_t)(nq > 0 ? nq : 1));
+        int *flags = (int *)malloc(sizeof(int) * (size_t)(nq > 0 ? nq : 1));
+        if (!ind || !flags || amr_indicator_gradient(f, ind)) {
+            free(ind);
+            free(flags);
+            return -1;
+        }
         for (int k = 0; k < nq; ++k) flags[k] = ind[k] > tol;
         int r = amr_forest_refine(f, flags);
         free(ind);
@@ -273,7 +267,6 @@ int amr_regrid_initial(amr_forest_t *f, const amr_ic_t *ic, double tol)
     int nq = amr_forest_num_quads(f);
     int *tag = amr_forest_tags(f);
     for (int k = 0; k < nq; ++k) tag[k] = 0;
-    (void)n;
     return nq;
 }
 
@@ -351,7 +344,7 @@ int amr_evolve(amr_forest_t *f, double tol, int allow_coarsening, amr_evolve_sta
         if (rc) break;
         if (regrid && evolve) {
             double t3 = now_s();
-            amr_regrid(f, tol, allow_coarsening);
+            if (amr_regrid(f, tol, allow_coarsening) < 0) { rc = AMRGPU_ERR_MESH; break; }
             s.wall_regrid += now_s() - t3;
             s.regrids++;
             if (amr_forest_num_quads(f) > s.peak_quads) s.peak_quads = amr_forest_num_quads(f);
@@ -552,6 +545,7 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
             free(ind); free(tags);
             ind = (double *)malloc(sizeof(double) * (size_t)cap);
             tags = (int *)malloc(sizeof(int) * (size_t)cap);
+            if (!ind || !tags) { rc = AMRGPU_ERR_ARG; break; }
         }
         if (o && o->tagger) {
             rc = o->tagger(o->tagger_ctx, g, f, tags);

@@ -71,6 +71,21 @@ static inline uint64_t morton(int x, int y) { return spread((uint32_t)x) | (spre
 
 static int patch_size(const amr_forest_t *f) { return 4 * f->n * f->n; }
 
+/* Allocation failure ends the process with a message, as p4est's allocator
+ * does: a half-rebuilt forest has no state to return to.  (The balance
+ * after a regrid is the exception: it reports failure and leaves the forest
+ * as it was.) */
+static void *xrealloc(void *p, size_t bytes)
+{
+    void *r = realloc(p, bytes ? bytes : 1);
+    if (!r) {
+        fprintf(stderr, "amr_forest: out of memory (%zu bytes)\n", bytes);
+        abort();
+    }
+    return r;
+}
+static void *xmalloc(size_t bytes) { return xrealloc(NULL, bytes); }
+
 static inline uint64_t hmix(uint64_t key, int meta)
 {
     uint64_t z = key ^ ((uint64_t)(uint32_t)meta * 0x9E3779B97F4A7C15ull);
@@ -85,7 +100,7 @@ static void hash_build(amr_forest_t *f)
     while (cap < 2 * f->nquad) cap <<= 1;
     if (cap != f->hcap) {
         free(f->hash);
-        f->hash = (struct hentry *)malloc(sizeof(struct hentry) * (size_t)cap);
+        f->hash = (struct hentry *)xmalloc(sizeof(struct hentry) * (size_t)cap);
         f->hcap = cap;
     }
     for (int i = 0; i < cap; ++i) f->hash[i].meta = -1;
@@ -120,12 +135,12 @@ static void reserve(amr_forest_t *f, int cap)
 {
     if (cap <= f->cap) return;
     int ps = patch_size(f);
-    f->q = (amr_quad_t *)realloc(f->q, sizeof(amr_quad_t) * (size_t)cap);
-    f->key = (uint64_t *)realloc(f->key, sizeof(uint64_t) * (size_t)cap);
-    f->soln = (double *)realloc(f->soln, sizeof(double) * (size_t)ps * (size_t)cap);
-    f->tag = (int *)realloc(f->tag, sizeof(int) * (size_t)cap);
-    f->type = (int *)realloc(f->type, sizeof(int) * (size_t)cap);
-    f->mark = (unsigned char *)realloc(f->mark, (size_t)cap);
+    f->q = (amr_quad_t *)xrealloc(f->q, sizeof(amr_quad_t) * (size_t)cap);
+    f->key = (uint64_t *)xrealloc(f->key, sizeof(uint64_t) * (size_t)cap);
+    f->soln = (double *)xrealloc(f->soln, sizeof(double) * (size_t)ps * (size_t)cap);
+    f->tag = (int *)xrealloc(f->tag, sizeof(int) * (size_t)cap);
+    f->type = (int *)xrealloc(f->type, sizeof(int) * (size_t)cap);
+    f->mark = (unsigned char *)xrealloc(f->mark, (size_t)cap);
     f->cap = cap;
     f->soln_cap = cap;
 }
@@ -138,6 +153,7 @@ amr_forest_t *amr_forest_new(int ntx, int nty, int periodic_x, int periodic_y,
         max_level > AMR_MAXLEVEL - 1)
         return NULL;
     amr_forest_t *f = (amr_forest_t *)calloc(1, sizeof(amr_forest_t));
+    if (!f) return NULL;
     f->ntx = ntx; f->nty = nty;
     f->periodic[0] = periodic_x; f->periodic[1] = periodic_y;
     f->xlo = xlo; f->ylo = ylo; f->scale = scale;
@@ -184,7 +200,7 @@ double *amr_forest_soln(amr_forest_t *f)
     /* regrids that leave the data on the device do not move patches; the
      * array only grows here, contents of new entries undefined */
     if (f->soln_cap < f->nquad) {
-        f->soln = (double *)realloc(f->soln, sizeof(double) * (size_t)patch_size(f) * (size_t)f->nquad);
+        f->soln = (double *)xrealloc(f->soln, sizeof(double) * (size_t)patch_size(f) * (size_t)f->nquad);
         f->soln_cap = f->nquad;
     }
     return f->soln;
@@ -328,15 +344,15 @@ int amr_forest_build_external(amr_forest_t *f, int *ncell, int *nghost, int *nfa
     if (nq > f->ext_cap) {
         const int c = nq + nq / 4 + 16;
         free(f->cell_info); free(f->face_info); free(f->geom);
-        f->cell_info = (int *)malloc(sizeof(int) * 3 * (size_t)c);
-        f->face_info = (int *)malloc(sizeof(int) * 11 * 4 * (size_t)c);
-        f->geom = (double *)malloc(sizeof(double) * 12 * (size_t)c);
+        f->cell_info = (int *)xmalloc(sizeof(int) * 3 * (size_t)c);
+        f->face_info = (int *)xmalloc(sizeof(int) * 11 * 4 * (size_t)c);
+        f->geom = (double *)xmalloc(sizeof(double) * 12 * (size_t)c);
         f->ext_cap = c;
     }
     if (nq + 1 > f->ext_rec_cap) {
         const int c = nq + nq / 4 + 16;
         free(f->ext_rec);
-        f->ext_rec = (int *)malloc(sizeof(int) * 45 * (size_t)c);    /* 4 records of 11 + count */
+        f->ext_rec = (int *)xmalloc(sizeof(int) * 45 * (size_t)c);    /* 4 records of 11 + count */
         f->ext_rec_cap = c;
     }
     if (!f->hvalid) hash_build(f);               /* shared read-only below */
@@ -425,14 +441,14 @@ int amr_forest_refine(amr_forest_t *f, const int *flags)
     for (int k = 0; k < nq; ++k) if (flags[k] && f->q[k].level < f->max_level) ++cnt;
     if (!cnt) return 0;
     int nn = nq + 3 * cnt;
-    amr_quad_t *q = (amr_quad_t *)malloc(sizeof(amr_quad_t) * (size_t)nn);
-    uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)nn);
+    amr_quad_t *q = (amr_quad_t *)xmalloc(sizeof(amr_quad_t) * (size_t)nn);
+    uint64_t *key = (uint64_t *)xmalloc(sizeof(uint64_t) * (size_t)nn);
     const int move = !f->track || f->track_data;
-    double *soln = move ? (double *)malloc(sizeof(double) * (size_t)ps * (size_t)nn) : NULL;
-    int *tag = (int *)malloc(sizeof(int) * (size_t)nn);
-    int *type = (int *)malloc(sizeof(int) * (size_t)nn);
-    unsigned char *mark = (unsigned char *)malloc((size_t)nn);
-    int *src = f->track ? (int *)malloc(sizeof(int) * (size_t)nn) : NULL;
+    double *soln = move ? (double *)xmalloc(sizeof(double) * (size_t)ps * (size_t)nn) : NULL;
+    int *tag = (int *)xmalloc(sizeof(int) * (size_t)nn);
+    int *type = (int *)xmalloc(sizeof(int) * (size_t)nn);
+    unsigned char *mark = (unsigned char *)xmalloc((size_t)nn);
+    int *src = f->track ? (int *)xmalloc(sizeof(int) * (size_t)nn) : NULL;
     int j = 0;
     for (int k = 0; k < nq; ++k) {
         if (flags[k] && f->q[k].level < f->max_level) {
@@ -567,7 +583,7 @@ int amr_forest_balance(amr_forest_t *f)
 {
     int total = 0;
     for (;;) {
-        int *flags = (int *)malloc(sizeof(int) * (size_t)(f->nquad > 0 ? f->nquad : 1));
+        int *flags = (int *)xmalloc(sizeof(int) * (size_t)(f->nquad > 0 ? f->nquad : 1));
         int any = balance_flags(f, flags);
         if (any) {
             /* balance may refine past max_level only if the forest already
@@ -585,7 +601,7 @@ int amr_forest_balance(amr_forest_t *f)
 
 int amr_forest_is_balanced(amr_forest_t *f)
 {
-    int *flags = (int *)malloc(sizeof(int) * (size_t)(f->nquad > 0 ? f->nquad : 1));
+    int *flags = (int *)xmalloc(sizeof(int) * (size_t)(f->nquad > 0 ? f->nquad : 1));
     int any = balance_flags(f, flags);
     free(flags);
     return !any;
@@ -903,7 +919,7 @@ void amr_forest_clear_marks(amr_forest_t *f)
 void amr_forest_track_begin(amr_forest_t *f, int keep_data)
 {
     free(f->src);
-    f->src = (int *)malloc(sizeof(int) * (size_t)(f->nquad > 0 ? f->nquad : 1));
+    f->src = (int *)xmalloc(sizeof(int) * (size_t)(f->nquad > 0 ? f->nquad : 1));
     for (int k = 0; k < f->nquad; ++k) f->src[k] = k;
     f->track = 1;
     f->track_data = keep_data;

@@ -111,6 +111,44 @@ def test_direct_mesh_table_equals_face_table(seed):
         amrhip.nbr_from_faces(f.nquad, bad.reshape(-1))
 
 
+def side_cells(n, s):
+    """(i, j) of the n sub-cells along side s (-x, +x, -y, +y) of a patch"""
+    t = np.arange(n)
+    return {0: (0 * t, t), 1: (0 * t + n - 1, t), 2: (t, 0 * t), 3: (t, 0 * t + n - 1)}[s]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_host_indicator_per_quadrant_sides(seed):
+    """amr_indicator_gradient walks the reference-format face list; the same
+    indicator from every quadrant's own sides through the direct mesh table
+    (index-aligned pairs across conforming and hanging faces, the jump
+    |a - b| / ((a + b) / 2) being symmetric) must agree bit for bit"""
+    n = 4
+    f = random_forest(seed, n=n, periodic=(1, 1) if seed % 2 else (1, 0))
+    rng = np.random.default_rng(300 + seed)
+    s = f.soln()
+    s[:] = rng.uniform(0.5, 1.5, s.shape)
+    got = f.indicator()
+    rho = f.soln().reshape(f.nquad, n, n, 4)[..., 0]      # [quad, j, i]
+    nbr, _, _ = f.mesh_table()
+    nbr = nbr.reshape(-1, 4, 2)
+
+    def jump(a, b):
+        return np.abs(a - b) / (0.5 * (a + b))
+    exp = np.zeros(f.nquad)
+    for k in range(f.nquad):
+        r = rho[k]
+        m = max(jump(r[:, :-1], r[:, 1:]).max(), jump(r[:-1, :], r[1:, :]).max())
+        for sd in range(4):
+            i, j = side_cells(n, sd)
+            ib, jb = side_cells(n, sd ^ 1)
+            for d in nbr[k, sd]:
+                if d >= 0:
+                    m = max(m, jump(r[j, i], rho[d][jb, ib]).max())
+        exp[k] = m
+    assert np.array_equal(got, exp)
+
+
 def test_refine_copies_and_coarsen_averages(oracle):
     import pyoracle
     f = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 1.0, 4, 1, 3)
