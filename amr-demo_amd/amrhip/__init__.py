@@ -81,7 +81,7 @@ EXPORTS = {
     "amr_ic_patch": (None, [_vp, _i, _pd, _pd]),
     "amr_ic_apply": (None, [_vp, _vp]),
     "amr_ic_vortex_density": (_d, [_vp, _d, _d, _d, _d, _d, _d, _d]),
-    "amr_indicator_gradient": (C.c_int, [_vp, _pd]),
+    "amr_indicator_gradient": (None, [_vp, _pd]),
     "amr_regrid": (_i, [_vp, _d, _i]),
     "amr_regrid_initial": (_i, [_vp, _vp, _d]),
     "amr_regrid_tags": (_i, [_vp, _pi, _i]),
@@ -368,8 +368,7 @@ class Forest:
 
     def indicator(self):
         ind = np.zeros(self.nquad)
-        if self.L.amr_indicator_gradient(self.h, dpt(ind)):
-            raise RuntimeError("amr_indicator_gradient: the forest's boundary arrays could not be built")
+        self.L.amr_indicator_gradient(self.h, dpt(ind))
         return ind
 
     def regrid(self, tol, allow_coarsening=True):

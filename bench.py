@@ -12,8 +12,10 @@ Setup (untimed): the adaptive problem is run on the GPU path through the C
 evolve loop with the state resident in HBM (device indicator and remesh, host
 forest topology) to t = --t-end; the mesh and state reached there are the
 benchmark input, resident in HBM.  The end-to-end rate of that adaptive run is
-reported beside the headline (and, with --host-loop, that of the
-interval-by-interval path through hpc_amr_evolve_solver).
+reported beside the headline: the run is made twice, and adaptive_run gives
+the second (steady state) with the first, which pays the runtime's one-time
+set-up, beside it (with --host-loop also the interval-by-interval path
+through hpc_amr_evolve_solver).
 
 A step = one SSP-RK3 step (3 fused residual + stage-update launches, CFL
 folded into the last) over that mesh.  value = sub-grid cells x 3 stages x
@@ -317,8 +319,14 @@ def main():
     if args.workload == "riemann4":
         log(f"setup: adaptive Riemann seed={args.seed} levels {args.lmin}-{args.lmax} n={args.n} "
             f"to t={args.t_end}")
+        # the first run in the process pays the runtime's one-time set-up
+        # (code-object load, first allocations); the second is the steady state
+        f_cold, st_cold, wall_cold = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, args.n, args.tol,
+                                                   args.t_end, 10, device=True)
         f, st, wall = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, args.n, args.tol,
                                          args.t_end, 10, device=True)
+        if (st.steps, st.regrids, f.nquad) != (st_cold.steps, st_cold.regrids, f_cold.nquad):
+            raise RuntimeError("two identical adaptive runs differ")
         host_run = None
         if args.host_loop:
             _, sth, wallh = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, args.n,
@@ -331,6 +339,7 @@ def main():
         f = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 10.0, args.n, args.level, args.level)
         f.apply_ic(amrhip.vortex_ic(5.0, 5.0, 5.0))
         st, wall, host_run = None, 0.0, None
+        st_cold, wall_cold = None, 0.0
         h_base, bc = 10.0, (0, 0, 0, 0)
         workload = (f"isentropic vortex strength 5, periodic [0,10]^2, uniform level {args.level}, "
                     f"{args.n}x{args.n} FP64 patches, SSP-RK3 CFL 0.4")
@@ -420,6 +429,12 @@ def main():
             "regrid_split_s": {"tags": round(st.wall_tags, 3), "host_forest": round(st.wall_forest, 3),
                                "device_remesh": round(st.wall_remesh, 3)},
             "end_to_end_Mcell_updates_per_s": round(st.cell_stages / wall / 1e6, 3),
+            "first_run_in_process": {
+                "wall_s": round(wall_cold, 3), "regrid_s": round(st_cold.wall_regrid, 3),
+                "end_to_end_Mcell_updates_per_s": round(st_cold.cell_stages / wall_cold / 1e6, 3),
+                "note": "includes the runtime's one-time set-up; the figures above are the second, "
+                        "identical run",
+            },
         }
         if host_run is not None:
             sth, wallh = host_run

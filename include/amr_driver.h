@@ -51,15 +51,16 @@ double amr_ic_vortex_density(const amr_ic_t *ic, double x, double y, double t,
                              double x0, double Lx, double y0, double Ly);
 
 /* Project-defined density-gradient indicator per quadrant (see above);
- * ind[k] receives the indicator value.  Returns 0, or -1 when the forest's
- * boundary arrays cannot be built (an inconsistent forest). */
-int amr_indicator_gradient(amr_forest_t *f, double *ind);
+ * ind[k] receives the indicator value.  On a forest that is not 2:1
+ * balanced (between the passes of amr_regrid_initial) the sides facing a
+ * neighbour two or more levels away contribute no pairs. */
+void amr_indicator_gradient(amr_forest_t *f, double *ind);
 
 /* Regrid in the order of regrid_solution (src/amr/src/amr_regrid.c:15-81):
  * reset tags, tag by indicator > tol and refine tagged quadrants one level
  * (below max_level), coarsen untagged families (unless initial), 2:1 balance.
  * Returns the new number of quadrants, or -1 on failure (as
- * amr_regrid_tags, or the indicator could not be evaluated). */
+ * amr_regrid_tags). */
 int amr_regrid(amr_forest_t *f, double tol, int allow_coarsening);
 /* The same regrid driven by given per-quadrant tags: 1 refine, 0 may be
  * coarsened with its family, any other value keeps the quadrant (no refine,
@@ -70,7 +71,7 @@ int amr_regrid_tags(amr_forest_t *f, const int *tags, int allow_coarsening);
 /* Initial refinement (regrid_feature with initial = 1): repeatedly tag and
  * refine, re-evaluating the initial condition on new quadrants
  * (initialize_quadrant_data), then balance.  Returns the number of
- * quadrants, or -1 if a pass could not evaluate the indicator. */
+ * quadrants, or -1 if an allocation failed. */
 int amr_regrid_initial(amr_forest_t *f, const amr_ic_t *ic, double tol);
 
 /* Output of the evolve loops (as src/amr/src/amr_evolve.c:45-66): when a

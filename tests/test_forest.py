@@ -1,5 +1,7 @@
 """Host forest (p4est stand-in) invariants: p4est is absent, so topology,
 balance and the boundary arrays are pinned by construction checks."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -117,21 +119,45 @@ def side_cells(n, s):
     return {0: (0 * t, t), 1: (0 * t + n - 1, t), 2: (t, 0 * t), 3: (t, 0 * t + n - 1)}[s]
 
 
-@pytest.mark.parametrize("seed", range(4))
+def raw_mesh_table(f):
+    """amr_forest_build_nbr's table and status, also on an unbalanced forest
+    (sides facing a neighbour two or more levels away are left at -1)"""
+    nbr = np.empty(8 * f.nquad, np.int32)
+    level = np.empty(f.nquad, np.int32)
+    org = np.empty(2 * f.nquad, np.float64)
+    pi = C.POINTER(C.c_int)
+    rc = amrhip.lib().amr_forest_build_nbr(f.h, nbr.ctypes.data_as(pi), level.ctypes.data_as(pi),
+                                           org.ctypes.data_as(C.POINTER(C.c_double)))
+    return rc, nbr.reshape(-1, 4, 2)
+
+
+@pytest.mark.parametrize("seed", range(5))
 def test_host_indicator_per_quadrant_sides(seed):
     """amr_indicator_gradient walks the reference-format face list; the same
     indicator from every quadrant's own sides through the direct mesh table
     (index-aligned pairs across conforming and hanging faces, the jump
-    |a - b| / ((a + b) / 2) being symmetric) must agree bit for bit"""
+    |a - b| / ((a + b) / 2) being symmetric) must agree bit for bit.  Seed 4
+    refines one quadrant twice without balancing, as the passes of the
+    initial refinement do: sides facing a neighbour two levels away have no
+    pairs, in both formulations"""
     n = 4
-    f = random_forest(seed, n=n, periodic=(1, 1) if seed % 2 else (1, 0))
+    if seed < 4:
+        f = random_forest(seed, n=n, periodic=(1, 1) if seed % 2 else (1, 0))
+    else:
+        # uniform level 1, the lower-left quadrant refined, then its
+        # upper-right child again: level 3 next to level 1
+        f = amrhip.Forest(1, 1, (1, 0), 0.0, 0.0, 1.0, n, 1, 5)
+        f.refine(np.array([1, 0, 0, 0], np.int32))
+        f.refine(np.eye(f.nquad, dtype=np.int32)[3])
+        assert not f.is_balanced()
     rng = np.random.default_rng(300 + seed)
     s = f.soln()
     s[:] = rng.uniform(0.5, 1.5, s.shape)
     got = f.indicator()
     rho = f.soln().reshape(f.nquad, n, n, 4)[..., 0]      # [quad, j, i]
-    nbr, _, _ = f.mesh_table()
-    nbr = nbr.reshape(-1, 4, 2)
+    rc, nbr = raw_mesh_table(f)
+    assert rc == (-1 if seed == 4 else 0)
+    lv = f.levels()
 
     def jump(a, b):
         return np.abs(a - b) / (0.5 * (a + b))
@@ -143,7 +169,7 @@ def test_host_indicator_per_quadrant_sides(seed):
             i, j = side_cells(n, sd)
             ib, jb = side_cells(n, sd ^ 1)
             for d in nbr[k, sd]:
-                if d >= 0:
+                if d >= 0 and abs(int(lv[d]) - int(lv[k])) <= 1:
                     m = max(m, jump(r[j, i], rho[d][jb, ib]).max())
         exp[k] = m
     assert np.array_equal(got, exp)
@@ -182,6 +208,20 @@ def test_find_leaf_and_levels():
         assert L.amr_forest_find_leaf(f.h, tree, x, y) == k
         size = ROOT_LEN >> level
         assert L.amr_forest_find_leaf(f.h, tree, x + size - 1, y + size - 1) == k
+
+
+def test_regrid_initial_through_unbalanced_passes():
+    """the initial refinement balances only after its last pass, so its
+    indicator sees forests with two-level jumps; it must still end balanced
+    (periodic vortex, levels 2-4: the mesh of the device-loop lag test)"""
+    ic = amrhip.vortex_ic(5.0, 5.0, 5.0)
+    f = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 10.0, 8, 2, 4)
+    f.apply_ic(ic)
+    nq = f.regrid_initial(ic, 0.02)
+    assert nq == f.nquad > 16
+    assert f.is_balanced() and f.levels().max() == 4
+    rc, _ = raw_mesh_table(f)
+    assert rc == 0
 
 
 def test_regrid_refines_discontinuity():
