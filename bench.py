@@ -296,7 +296,8 @@ def main():
                     help="also time the interval-by-interval host-regrid loop")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-tagger comparison")
     ap.add_argument("--learned-seeds", type=int, default=8, help="training runs (seeds 1..k)")
-    ap.add_argument("--learned-lmax", type=int, default=8, help="finest level of the deployed run")
+    ap.add_argument("--learned-lmax", type=int, default=9,
+                    help="finest level of the deployed run (9: about 1M cells at peak)")
     ap.add_argument("--learned-interval", type=int, default=40)
     ap.add_argument("--learned-epochs", type=int, default=30)
     ap.add_argument("--learned-lag", type=int, default=10,
