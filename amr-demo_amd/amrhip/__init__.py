@@ -588,6 +588,22 @@ class Gpu:
         check(rc, "amrgpu_remesh")
         self.ndof = ci.size // 3 * 4 * self.n2
 
+    def remesh_nbr(self, level, nbr, src, cell_inst=None):
+        """amrgpu_remesh_nbr: the new mesh as the device table of
+        Forest.mesh_table (levels, nbr[8 nquad]) instead of cell_info / face_info"""
+        lv = np.ascontiguousarray(level, dtype=np.int32)
+        nb = np.ascontiguousarray(nbr, dtype=np.int32)
+        sr = np.ascontiguousarray(src, dtype=np.int32)
+        cinst = None if cell_inst is None else np.ascontiguousarray(cell_inst, dtype=np.int32)
+        rc = self.L.amrgpu_remesh_nbr(self.h, lv.size, ipt(lv), ipt(cinst) if cinst is not None else None,
+                                      ipt(nb), ipt(sr))
+        check(rc, "amrgpu_remesh_nbr")
+        self.ndof = lv.size * 4 * self.n2
+
+    def set_origins(self, org):
+        check(self.L.amrgpu_set_origins(self.h, dpt(np.ascontiguousarray(org, dtype=np.float64))),
+              "amrgpu_set_origins")
+
     def rhs(self):
         out = np.empty(self.ndof)
         check(self.L.amrgpu_rhs(self.h, dpt(out)), "amrgpu_rhs")

@@ -1834,7 +1834,6 @@ static int remesh_to(amrgpu_ctx *c, int ncell, const int *level, const int *cell
 int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst, int nface,
                   const int *face_info, const int *src)
 {
-    if (c) c->org_ok = false;          // the new quadrants' coordinates come with amrgpu_set_geometry
     if (!c || !c->nquad || !cell_info || !face_info || !src || ncell <= 0 || nface < 0) {
         set_err("amrgpu_remesh: bad arguments");
         return AMRGPU_ERR_ARG;
@@ -1844,14 +1843,14 @@ int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cel
     if (rc) return rc;
     std::vector<int> nbr;
     rc = build_nbr(ncell, nface, face_info, nbr);
-    if (rc) return rc;
+    if (rc) return rc;                 // a refused call leaves the context as it was
+    c->org_ok = false;                 // the new quadrants' coordinates come with amrgpu_set_geometry
     return remesh_to(c, ncell, levels_of(ncell, cell_info).data(), cell_inst, nbr, src);
 }
 
 int amrgpu_remesh_nbr(amrgpu_ctx *c, int ncell, const int *level, const int *cell_inst, const int *nbr,
                       const int *src)
 {
-    if (c) c->org_ok = false;          // the new quadrants' coordinates come with amrgpu_set_origins
     if (!c || !c->nquad || !level || !nbr || !src || ncell <= 0) {
         set_err("amrgpu_remesh_nbr: bad arguments");
         return AMRGPU_ERR_ARG;
@@ -1876,6 +1875,7 @@ int amrgpu_remesh_nbr(amrgpu_ctx *c, int ncell, const int *level, const int *cel
             }
         }
     }
+    c->org_ok = false;                 // the new quadrants' coordinates come with amrgpu_set_origins
     return remesh_to(c, ncell, level, cell_inst, std::vector<int>(nbr, nbr + 8 * (size_t)ncell), src);
 }
 

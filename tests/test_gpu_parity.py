@@ -302,6 +302,50 @@ def test_device_remesh_matches_host_operators(gpu):
     assert np.array_equal(gpu.soln(), soln2)
 
 
+def test_direct_table_remesh_equals_face_route(gpu):
+    """amrgpu_remesh_nbr with the forest's own mesh table (the device loops'
+    route) gives the state and residual of amrgpu_remesh with the reference
+    face list; tables with an id outside the new mesh, a level or an
+    instance out of range are refused and leave the context as it was
+    (position walls included: their coordinates stay valid)"""
+    rng = np.random.default_rng(5)
+    f = dmr_forest(5)
+    ci, fi, geom, soln = (np.array(a, copy=True) for a in f.external())
+    insts = [amrhip.dmr_inst(1.0, 0.3, 1.0)]
+    gpu.setup(8, 3, insts, ci, fi, geom, soln)
+    tags = (rng.uniform(size=f.nquad) < 0.3).astype(np.int32)
+    f.track_begin(keep_data=True)
+    f.regrid_tags(tags)
+    ok, src = f.track_end()
+    assert ok and np.any(src < 0)
+    ci2, fi2, geom2, soln2 = (np.array(a, copy=True) for a in f.external())
+    nbr, level, org = f.mesh_table()
+    r_before = gpu.rhs()
+    bad_nbr = nbr.copy()
+    bad_nbr[8 * 3 + 2] = f.nquad                     # an id one past the new mesh
+    bad_level = level.copy()
+    bad_level[0] = 30
+    for args, kw in (((level, bad_nbr, src), {}), ((bad_level, nbr, src), {}),
+                     ((level, nbr, src), {"cell_inst": np.full(f.nquad, 1, np.int32)})):
+        with pytest.raises(amrhip.GpuError):
+            gpu.remesh_nbr(*args, **kw)
+    assert np.array_equal(gpu.soln(), soln) and np.array_equal(gpu.rhs(), r_before)
+    gpu.remesh_nbr(level, nbr, src)
+    with pytest.raises(amrhip.GpuError):             # position walls need the new origins
+        gpu.rhs()
+    gpu.set_origins(org)
+    assert np.array_equal(gpu.soln(), soln2)
+    r_tab = gpu.rhs()
+    g2 = amrhip.Gpu()
+    try:
+        g2.setup(8, 3, insts, ci, fi, geom, soln)
+        g2.remesh(ci2, fi2, src)
+        amrhip.check(amrhip.lib().amrgpu_set_geometry(g2.h, amrhip.dpt(geom2)), "geom")
+        assert np.array_equal(g2.rhs(), r_tab)
+    finally:
+        g2.close()
+
+
 @pytest.mark.parametrize("allow_coarsening", [True, False])
 def test_device_evolve_equals_host_regrid_loop(allow_coarsening):
     """state resident in HBM across regrids gives bit for bit the result of
