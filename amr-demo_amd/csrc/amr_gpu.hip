@@ -48,6 +48,11 @@ struct InstDev {
     double t_end;
     double bstate[2][4];              // boundary states of wall types 3-5
     double shock_x0, shock_cot, shock_vx;   // x_s(y,t) = x0 + y cot(a) + vx t
+    // with h0 = h_base / n, the sub-cell size at level 0: -1 / (h0 h0) and
+    // cfl sqrt(h0 h0).  Sizes of other levels are h0 2^-level, so -1 / (h h)
+    // and cfl sqrt(h h) are these scaled by 4^level and 2^-level exactly
+    // (one ldexp per patch instead of a divide or a square root)
+    double ninv_vol0, cfl_h0;
 };
 
 // Per-quadrant face table entry for side s (0 xlo, 1 xhi, 2 ylo, 3 yhi):
@@ -301,8 +306,9 @@ __device__ __forceinline__ void flux_boundary(const InstDev &ip, int side, const
 }
 
 // Per-cell CFL candidate, solver_time_step_2d_module.F90:107-116, evaluated
-// without FMA contraction so that it reproduces the reference bit for bit.
-__device__ __forceinline__ double cfl_candidate(const double q[4], double cfl, double vol)
+// without FMA contraction so that it reproduces the reference bit for bit;
+// cfl_h = cfl * sqrt(vol), the patch's numerator.
+__device__ __forceinline__ double cfl_candidate_h(const double q[4], double cfl_h)
 {
 #pragma clang fp contract(off)
     double oneOrho = 1.0 / q[0];
@@ -312,7 +318,13 @@ __device__ __forceinline__ double cfl_candidate(const double q[4], double cfl, d
     double pOverRho = 0.4 * (q[3] * oneOrho - 0.5 * V2);
     double c = sqrt(1.4 * pOverRho);
     double eig = sqrt(V2) + c;
-    return cfl * sqrt(vol) / eig;
+    return cfl_h / eig;
+}
+
+__device__ __forceinline__ double cfl_candidate(const double q[4], double cfl, double vol)
+{
+#pragma clang fp contract(off)
+    return cfl_candidate_h(q, cfl * sqrt(vol));
 }
 
 // v from the lane `CTRL` (a DPP row-shift control) selects, or v itself
@@ -513,7 +525,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     __shared__ int pnbr[PPB][8];
     __shared__ double pdt[PPB];
     __shared__ double ptime[PPB];
-    __shared__ double ph[PPB], pcfl[PPB];     // cell size and CFL number of each patch
+    __shared__ double ph[PPB];     // cell size of each patch
 
     const int tid = threadIdx.x;
     const int nblk = gridDim.x;
@@ -615,13 +627,12 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         }
     }
     // patch parameters that depend on the instance
-    double hw = 0.0, cflw = 0.0, dtw = 0.0;
+    double hw = 0.0, dtw = 0.0;
     if constexpr (UNI) {
         typedef const InstDev __attribute__((address_space(4))) cinst4;
         typedef const double __attribute__((address_space(4))) cdbl4;
         const cinst4 *ipc = (const cinst4 *)a.m.ip;
         hw = actw ? ldexp(ipc[inw].h_base, -levw) / (double)N : 0.0;
-        cflw = actw ? ipc[inw].cfl : 0.0;
         double tw = 0.0;
         if (!(a.role & ROLE_DT)) {
             // written by an earlier launch, constant during this one
@@ -635,7 +646,6 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
 #pragma unroll
             for (int k = 0; k < 8; ++k) pnbr[p][k] = nbw[k];
             ph[p] = hw;
-            pcfl[p] = cflw;
             if (!(a.role & ROLE_DT)) {
                 pdt[p] = dtw;
                 ptime[p] = tw;
@@ -651,7 +661,6 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
 #pragma unroll
         for (int k = 0; k < 8; ++k) pnbr[tid][k] = act ? a.m.nbr[qq * 8 + k] : -1;
         ph[tid] = act ? ldexp(a.m.ip[in].h_base, -lev) / (double)N : 0.0;
-        pcfl[tid] = act ? a.m.ip[in].cfl : 0.0;
         if (!(a.role & ROLE_DT)) {
             pdt[tid] = act && a.mode != MODE_RHS ? a.dt_cur[in] : 0.0;
             // the step's start time, stable while the time advances
@@ -951,10 +960,20 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     double out[4] = {0.0, 0.0, 0.0, 0.0};
     double cand = INFINITY;
     double sq = 0.0;
+    double cflh = 0.0;      // cfl sqrt(h h) of the patch
     if (active) {
-        const double h = UNI ? hw : ph[p];
-        const double vol = h * h;
-        const double ninvV = -1.0 / vol;
+        // -1 / (h h) and cfl sqrt(h h) from the instance's level-0 values
+        double ninvV;
+        if constexpr (UNI) {
+            typedef const InstDev __attribute__((address_space(4))) cinst4;
+            const cinst4 *ipc = (const cinst4 *)a.m.ip;
+            ninvV = ldexp(ipc[inw].ninv_vol0, 2 * levw);
+            cflh = ldexp(ipc[inw].cfl_h0, -levw);
+        } else {
+            const InstDev &ip = a.m.ip[pinst[p]];
+            ninvV = ldexp(ip.ninv_vol0, 2 * plevel[p]);
+            cflh = ldexp(ip.cfl_h0, -plevel[p]);
+        }
         const double *FX = lds + p * TL::FLUX_PER_PATCH;
         const double *FY = FX + NFLD * N * (N + 1);
         const int stride = N * (N + 1);
@@ -1013,7 +1032,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
 #pragma unroll
         for (int f = 0; f < 4; ++f)   // streamed out: no other wave of this launch reads it
             __builtin_nontemporal_store(out[f], &a.Uout[base + f * N2]);
-        if (a.role & ROLE_CFL) cand = cfl_candidate(out, UNI ? cflw : pcfl[p], vol);
+        if (a.role & ROLE_CFL) cand = cfl_candidate_h(out, cflh);
     }
     // time bookkeeping: stage 2 advances t by this step's dt and re-arms the
     // CFL slot that the last stage fills
@@ -1728,6 +1747,9 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
         d.shock_x0 = inst[i].shock_x0;
         d.shock_cot = a != 0.0 ? cos(a) / sin(a) : 0.0;
         d.shock_vx = a != 0.0 ? inst[i].shock_speed / sin(a) : 0.0;
+        const double h0 = inst[i].h_base / nelem_subgrid, v0 = h0 * h0;
+        d.ninv_vol0 = -1.0 / v0;
+        d.cfl_h0 = inst[i].cfl * sqrt(v0);
     }
     HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice, c->stream));
     std::vector<double> zeros(ninst, 0.0);
