@@ -13,18 +13,36 @@
  * geom[12*c] (4 corners x xyz, z-order) as built by
  * src/amr/src/amr_external.c:337-508.
  *
- * Pinning (what checks each function; see tests/test_oracle_*.py):
- *   orc_flux_roe_2d ........ golden vectors from the reference's own Fortran
- *                            (oracle/_ref), consistency F(U,U)=F_phys(U)n,
+ * Pinning (what checks each function; tests/test_oracle_*.py,
+ * tests/test_walls.py; golden vectors come from the reference's own
+ * hot-path sources compiled into oracle/_ref by tests/golden/make_golden.py):
+ *   orc_flux_roe_2d ........ golden vectors, consistency F(U,U)=F_phys(U)n,
  *                            supersonic upwinding, x<->y mirror symmetry
- *   orc_flux_bc_2d, orc_bc_2d  golden vectors; slip wall pure-pressure flux
- *   orc_residual_*_2d ...... golden vectors through physics_rhs_structured of
- *                            the reference on uniform, hanging and boundary
- *                            forests; free-stream preservation; conservation
- *   orc_time_step_2d ....... golden vectors; closed form on uniform states
- *   orc_rk_step ............ closed-form stability polynomials on L(Q)=lambda Q
- *                            (the reference rk_* needs mpi.h and is not built)
+ *   orc_flux_bc_2d, orc_bc_2d, orc_flux_native_2d
+ *                            golden vectors (bc_2d + flux_bc_2d, whose physical
+ *                            flux is flux_native_2d); slip wall pure-pressure
+ *   orc_residual_*_2d, orc_bilinear_node_2d, orc_rhs_structured(_range)
+ *                            golden vectors of physics_rhs_structured on
+ *                            uniform, hanging and boundary forests (n = 2..16);
+ *                            free-stream preservation; discrete conservation
+ *   orc_residual_bc_ext .... wall types 3-5, project additions with no
+ *                            reference counterpart: closed forms (a uniform
+ *                            prescribed state is steady; over gas at rest only
+ *                            the cells behind the shock line / before x0 change)
+ *   orc_time_step_2d, orc_time_step
+ *                            golden physics_time_step minima (bit for bit);
+ *                            closed form on uniform states
+ *   orc_rk_step, orc_rk_generic, orc_run_steps(_threaded)
+ *                            closed-form stability polynomials on L(Q)=lambda Q,
+ *                            convergence orders on a rotation system, LSERK45
+ *                            local order 5 (the reference rk_* needs mpi.h and
+ *                            is not built)
  *   orc_initial_condition_2d golden vectors from initial_condition_2d
+ *   orc_refine_2d, orc_coarsen_2d, orc_tag_feature
+ *                            golden vectors of solver_refine_operator_2d,
+ *                            solver_coarsen_operator_2d, solver_tag_feature
+ *   orc_bf16 ... orc_mlp_grad (tagger_oracle.c) have no reference counterpart:
+ *                            pinned by the mathematics in tests/test_tagger_oracle.py
  */
 #ifndef AMR_ORACLE_H
 #define AMR_ORACLE_H

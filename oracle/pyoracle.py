@@ -301,6 +301,16 @@ class Reference:
                                   dpt(np.ascontiguousarray(q, float)), C.byref(tag))
         return tag.value
 
+    def refine(self, n, parent):
+        kids = [np.zeros(4 * n * n) for _ in range(4)]
+        un, nf, nn, ct = C.c_int(0), C.c_int(4), C.c_int(n), C.c_int(0)
+        h = C.c_double(1.0)
+        xyz = np.zeros(12)
+        self.L.solver_refine_operator_2d(C.byref(un), C.byref(nf), C.byref(nn), C.byref(ct), C.byref(h),
+                                         dpt(xyz), dpt(np.ascontiguousarray(parent, float)),
+                                         *[dpt(k) for k in kids])
+        return kids
+
     def coarsen(self, n, c1, c2, c3, c4):
         out = np.zeros(4 * n * n)
         un, nf, nn, ct = C.c_int(0), C.c_int(4), C.c_int(n), C.c_int(0)

@@ -15,7 +15,8 @@ Fixtures:
                  n = 2, 4, 8)
   ic.npz         initial_condition_2d (free stream and vortex, reference
                  parameters)
-  ops.npz        solver_tag_feature, solver_coarsen_operator_2d
+  ops.npz        solver_tag_feature, solver_coarsen_operator_2d,
+                 solver_refine_operator_2d
 """
 import os
 import sys
@@ -150,11 +151,12 @@ def gen_ops(ref, rng):
         tols = [float(rng.uniform(0.1, 2.0)) for _ in methods]
         cases.append((q, methods, tols))
         tags.append(ref.tag(methods, tols, n, q))
+    parent = rand_states(rng, n * n).reshape(-1)     # drawn last: the earlier fixtures stay as they were
     np.savez_compressed(os.path.join(OUT, "ops.npz"), children=np.array(qs), coarse=coarse,
                         tag_q=np.array([c[0] for c in cases]),
                         tag_methods=np.array([c[1] + [0] * (2 - len(c[1])) for c in cases]),
                         tag_tols=np.array([c[2] + [0.0] * (2 - len(c[2])) for c in cases]),
-                        tags=np.array(tags))
+                        tags=np.array(tags), refine_parent=parent, refine_children=np.array(ref.refine(n, parent)))
 
 
 def main():

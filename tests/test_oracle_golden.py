@@ -61,4 +61,10 @@ def test_coarsen_and_tag_ops(oracle):
         q = np.ascontiguousarray(d["tag_q"][k])
         nt = int(np.count_nonzero(m))
         assert oracle.L.orc_tag_feature(nt, ipt(m), dpt(t), n, dpt(q)) == int(d["tags"][k])
+    # solver_refine_operator_2d: every child a copy of the parent
+    parent = np.ascontiguousarray(d["refine_parent"])
+    kids = [np.full(4 * n * n, np.nan) for _ in range(4)]
+    oracle.L.orc_refine_2d(n, dpt(parent), *[dpt(k) for k in kids])
+    for k, r in zip(kids, d["refine_children"]):
+        assert np.array_equal(k, r) and np.array_equal(k, parent)
     _ = C
