@@ -571,7 +571,6 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
             int nl = lag;
             if (istep + nl > p.time_steps) nl = p.time_steps - istep;
             double t4 = now_s();
-            double resnorm;
             if (nl > 0) rc = amrgpu_run(g, nl, 1, &resnorm);
             if (!rc && nl > 0) {
                 istep += nl;
@@ -928,7 +927,6 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
             int nl = lag;
             if (istep + nl > p.time_steps) nl = p.time_steps - istep;
             const double t4 = now_s();
-            double resnorm;
             if (nl > 0) rc = amrgpu_run(g, nl, 1, &resnorm);
             if (!rc && nl > 0) {
                 for (int k = 0; k < ninst; ++k) {
@@ -972,12 +970,12 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
         if (ncell > tot.peak_quads) tot.peak_quads = ncell;
         tot.wall_regrid += now_s() - t2 - t_gpu_lag;
         if (t_gpu_lag > 0.0) {       /* stops reached during the overlapped steps */
-            int all = 1;
+            int all_done = 1;
             for (int k = 0; k < ninst; ++k) {
                 if (inst[k].t_end > 0.0 && t[k] >= inst[k].t_end) done[k] = 1;
-                if (!done[k]) all = 0;
+                if (!done[k]) all_done = 0;
             }
-            if (istep >= p.time_steps || all) break;
+            if (istep >= p.time_steps || all_done) break;
         }
     }
     if (!rc) {

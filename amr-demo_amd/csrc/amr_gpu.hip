@@ -141,21 +141,6 @@ __device__ __forceinline__ void fsqrt_rsq(double x, double &s, double &rs)
     rs = 2.0 * h;
 }
 
-__device__ __forceinline__ void make_prim(double r, double ru, double rv, double rE, Prim &P)
-{
-    // flux_2d_module.F90:93-99, with sqrt(rho) and 1/rho from one rsq
-    double sr, rsr;
-    fsqrt_rsq(r, sr, rsr);
-    double oneOr = rsr * rsr;
-    P.r = r; P.ru = ru; P.rv = rv; P.rE = rE;
-    P.u = ru * oneOr;
-    P.v = rv * oneOr;
-    double E = rE * oneOr;
-    P.p = (1.4 - 1.0) * (rE - 0.5 * r * (P.u * P.u + P.v * P.v));
-    P.H = E + P.p * oneOr;
-    P.sr = sr;
-}
-
 // A face's primitive record oriented along the face normal: n is the normal
 // direction (x for an x face), t the tangential one.
 struct OPrim {
@@ -488,6 +473,8 @@ __device__ __forceinline__ void store_rec(double *lds, int p, int t, double r, d
                                           double rE)
 {
     using TL = Tile<N, PPB>;
+    // the primitive variables of flux_2d_module.F90:93-99, with sqrt(rho)
+    // and 1/rho from one rsq; u, v and H are formed again by load_prim
     double sr, rsr;
     fsqrt_rsq(r, sr, rsr);
     const double ir = rsr * rsr;
@@ -1172,7 +1159,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK)) void indicator_kernel(MeshDev 
         atomicMax(&pmax[p], d2bits(v));    // non-negative doubles order as integers
     }
     __syncthreads();
-    if (tid < PPB && blockIdx.x * PPB + tid < m.nquad) ind[blockIdx.x * PPB + tid] = bits2d(pmax[tid]);
+    if (tid < PPB && (int)blockIdx.x * PPB + tid < m.nquad) ind[blockIdx.x * PPB + tid] = bits2d(pmax[tid]);
 }
 
 // New state after a regrid, one thread per new sub-cell: src >= 0 copies
