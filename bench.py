@@ -310,8 +310,9 @@ def main():
     ap.add_argument("--batch-t-end", type=float, default=0.25)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on exactly one GPU (the whole forest lives on one "
-              "MI355X); multi-GPU sharding is described in DESIGN.md and not launched.",
+        print("bench.py: the benchmark runs on exactly one GPU (the whole forest lives on one "
+              "MI355X); sharding over several GPUs is designed but not implemented, so --gpus N "
+              "with N > 1 is not run.",
               file=sys.stderr)
         sys.exit(2)
 
