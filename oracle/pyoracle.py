@@ -2,7 +2,8 @@
 hot-path build (oracle/_ref/libamrref.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product never loads it.
+cpu_baseline leg of bench.py, as the reference that GPU results are compared
+with (and, for the baseline, timed).  The product never loads it.
 """
 from __future__ import annotations
 
