@@ -659,11 +659,11 @@ static int batch_grow(batch_bufs_t *b, size_t nc, size_t nf, int n2)
     return 0;
 }
 
-/* Concatenated boundary arrays of all forests (quadrants of instance k at
- * base[k] .. base[k+1]-1; face cell ids shifted by the same offsets) from
- * each forest's own arrays ext[k]. */
+/* Concatenated boundary arrays and states of all forests for the batch's
+ * setup (quadrants of instance k at base[k] .. base[k+1]-1; face cell ids
+ * shifted by the same offsets) from each forest's own arrays ext[k]. */
 static int batch_build(const ext_t *ext, int ninst, int n, int *base, batch_bufs_t *b, int *ncell_out,
-                       int *nface_out, int with_data)
+                       int *nface_out)
 {
     const int n2 = n * n;
     size_t nc = 0, nf = 0;
@@ -682,7 +682,7 @@ static int batch_build(const ext_t *ext, int ninst, int n, int *base, batch_bufs
             b->cinst[g] = k;
         }
         memcpy(b->geom + 12 * nc, geom, sizeof(double) * 12 * (size_t)ncell);
-        if (with_data) memcpy(b->soln + 4 * (size_t)n2 * nc, soln, sizeof(double) * 4 * (size_t)n2 * ncell);
+        memcpy(b->soln + 4 * (size_t)n2 * nc, soln, sizeof(double) * 4 * (size_t)n2 * ncell);
         for (int e = 0; e < nface; ++e) {
             const int *src = face_info + 11 * e;
             int *dst = b->fi + 11 * (nf + (size_t)e);
@@ -844,7 +844,7 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
     double t0 = now_s();
     if (!st || !base || !done || !t || !ext || !tab || !ok || !obase) rc = AMRGPU_ERR_ARG;
     for (int k = 0; k < ninst && !rc; ++k) rc = ext_of(f[k], &ext[k]);
-    if (!rc) rc = batch_build(ext, ninst, n, base, &b, &ncell, &nface, 1);
+    if (!rc) rc = batch_build(ext, ninst, n, base, &b, &ncell, &nface);
     if (!rc) {
         g = amrgpu_create();
         if (!g) rc = AMRGPU_ERR_HIP;
