@@ -135,10 +135,14 @@ def test_rhs_config1_vortex(gpu, oracle):
     assert gpu.cfl()[0] == oracle.time_step(p, ci, geom, soln, np.inf)
 
 
+@pytest.mark.parametrize("n", [4, 8, 16])
 @pytest.mark.parametrize("scheme", [1, 2, 3, 4, 5])
-def test_rk_trajectory(gpu, oracle, scheme):
-    log(f"trajectory scheme {scheme}")
-    n = 8
+def test_rk_trajectory(gpu, oracle, scheme, n):
+    """every RK scheme's stage plan (Shu-Osher two-register stages, the RK4
+    accumulator, LSERK45's 2N register) on a forest with hanging faces, for
+    the patch sizes whose kernels differ (several patches per wave, one, or
+    several waves per patch)"""
+    log(f"trajectory scheme {scheme} n {n}")
     f = vortex_forest(n, 2)
     ci, fi, geom, soln = f.external()
     cfl = 0.25 * 0.4 * scheme
