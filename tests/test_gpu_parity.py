@@ -461,9 +461,9 @@ def test_config1_vortex_200_steps_l1(gpu, oracle):
     assert e5 < 0.75 * e_gpu
 
 
-def dmr_forest(seed=4):
+def dmr_forest(seed=4, n=8):
     rng = np.random.default_rng(seed)
-    f = amrhip.Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, 8, 1, 3)
+    f = amrhip.Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, n, 1, 3)
     f.refine((rng.uniform(size=f.nquad) < 0.5).astype(np.int32))
     f.balance()
     f.refine((rng.uniform(size=f.nquad) < 0.3).astype(np.int32))
@@ -472,22 +472,24 @@ def dmr_forest(seed=4):
     return f
 
 
-def dmr_oracle_params(oracle, t):
+def dmr_oracle_params(oracle, t, n=8):
     post, pre, w = amrhip.dmr_states()
-    return oracle.params(8, bc=(3, 0, 4, 5), cfl=0.3, bstate=(post, pre), shock=(1.0 / 6.0, 60.0, w), time=t)
+    return oracle.params(n, bc=(3, 0, 4, 5), cfl=0.3, bstate=(post, pre), shock=(1.0 / 6.0, 60.0, w), time=t)
 
 
+@pytest.mark.parametrize("n", [4, 8, 16])
 @pytest.mark.parametrize("t", [0.0, 0.07])
-def test_rhs_dmr_walls_every_tile(gpu, oracle, t):
-    """wall types 3-5 (position- and time-dependent states) on a hanging-face
-    mesh of [0,4] x [0,1], every tile variant, against the oracle"""
-    f = dmr_forest()
+def test_rhs_dmr_walls_every_tile(gpu, oracle, t, n):
+    """wall types 3-5 (position- and time-dependent states, evaluated at the
+    sub-cell face centres) on a hanging-face mesh of [0,4] x [0,1], every
+    tile variant of each patch size, against the oracle"""
+    f = dmr_forest(n=n)
     ci, fi, geom, soln = f.external()
-    gpu.setup(8, 3, [amrhip.dmr_inst(1.0, 0.3, 1.0)], ci, fi, geom, soln)
+    gpu.setup(n, 3, [amrhip.dmr_inst(1.0, 0.3, 1.0)], ci, fi, geom, soln)
     gpu.set_time([t])
-    ref = oracle.rhs(dmr_oracle_params(oracle, t), ci, fi, geom, soln)
-    for ppb, minw, design in amrhip.tile_variants(8):
-        log(f"dmr walls t={t} tile {(ppb, minw, design)}")
+    ref = oracle.rhs(dmr_oracle_params(oracle, t, n), ci, fi, geom, soln)
+    for ppb, minw, design in amrhip.tile_variants(n):
+        log(f"dmr walls n={n} t={t} tile {(ppb, minw, design)}")
         gpu.set_tile(ppb, minw, design)
         assert parity_err(gpu.rhs(), ref) <= RHS_TOL
 
