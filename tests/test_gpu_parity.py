@@ -211,23 +211,26 @@ def test_conservation_level9_full_size(gpu):
     assert np.all(np.abs(m1 - m0) <= 1e-12 * np.maximum(np.abs(m0), np.abs(m0).max()))
 
 
-def test_batched_instances_match_single(gpu):
-    """two independent forests in one context give each forest's own result"""
+@pytest.mark.parametrize("n", [4, 8, 16])
+def test_batched_instances_match_single(gpu, n):
+    """two independent forests in one context, with their own walls, cell
+    size and CFL number, give each forest's own result bit for bit (with
+    4x4 patches a workgroup holds patches of both instances)"""
     ga = []
     parts = []
-    for seed, bc in ((1, (0, 1, 2, 0)), (2, (1, 1, 1, 1))):
-        f = riemann_forest(8, seed=seed, lmax=4)
-        parts.append((f.external(), bc))
-    for (ci, fi, geom, soln), bc in parts:
+    for seed, bc, hb, cfl in ((1, (0, 1, 2, 0), 1.0, 0.3), (2, (1, 1, 1, 1), 2.0, 0.25)):
+        f = riemann_forest(n, seed=seed, lmax=4 if n < 16 else 3)
+        parts.append((tuple(np.array(a, copy=True) for a in f.external()), bc, hb, cfl))
+    for (ci, fi, geom, soln), bc, hb, cfl in parts:
         g = amrhip.Gpu()
-        setup(g, 8, 3, 1.0, bc, 0.3, 1.0, ci, fi, geom, soln, t_end=0.01)
+        setup(g, n, 3, hb, bc, cfl, 1.0, ci, fi, geom, soln, t_end=0.01)
         g.run(5)
         ga.append((g.soln(), g.time()[0][0]))
         g.close()
     # concatenate
     cis, fis, gs, ss, inst = [], [], [], [], []
     off_c, off_d = 0, 0
-    for k, ((ci, fi, geom, soln), bc) in enumerate(parts):
+    for k, ((ci, fi, geom, soln), *_) in enumerate(parts):
         c = ci.reshape(-1, 3).copy()
         c[:, 2] += off_d
         fr = fi.reshape(-1, 11).copy()
@@ -238,7 +241,7 @@ def test_batched_instances_match_single(gpu):
         off_c += len(c)
         off_d += soln.size
     g = amrhip.Gpu()
-    g.setup(8, 3, [amrhip.make_inst(1.0, bc, 0.3, 1.0, 0.01) for (_, bc) in parts],
+    g.setup(n, 3, [amrhip.make_inst(hb, bc, cfl, 1.0, 0.01) for (_, bc, hb, cfl) in parts],
             np.concatenate(cis), np.concatenate(fis), np.concatenate(gs), np.concatenate(ss),
             cell_inst=np.concatenate(inst))
     g.run(5)
