@@ -175,8 +175,10 @@ def test_graph_and_direct_launch_identical(gpu):
     assert np.array_equal(a, b)
 
 
-def test_conservation_periodic(gpu):
-    n = 8
+@pytest.mark.parametrize("n", [2, 4, 8, 16])
+def test_conservation_periodic(gpu, n):
+    """mass, momentum and energy over 50 SSP-RK3 steps on the periodic box
+    with hanging faces, for every patch size"""
     f = vortex_forest(n, 3)
     ci, fi, geom, soln = f.external()
     setup(gpu, n, 3, 10.0, (0, 0, 0, 0), 0.3, 1.0, ci, fi, geom, soln)
