@@ -122,6 +122,35 @@ def test_autotune_restores_state_and_keeps_parity(gpu, oracle):
     assert parity_err(gpu.soln(), q_ref) <= TRAJ_TOL
 
 
+@pytest.mark.parametrize("n", [2, 4, 8, 16])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_single_quadrant(gpu, oracle, n, periodic):
+    """the smallest mesh: one quadrant, whose neighbour across every periodic
+    side is itself (or four walls), in a workgroup whose other patch slots
+    are empty; residual, bit-exact CFL and 5 SSP-RK3 steps against the
+    oracle.  Random states (density and pressure in [0.5, 1.5], velocities
+    in [-0.3, 0.3]): a symmetric initial condition would leave whole
+    residual fields at round-off, where a per-field relative bar is void"""
+    per = (1, 1) if periodic else (0, 0)
+    bc = (0, 0, 0, 0) if periodic else (1, 2, 0, 1)
+    f = amrhip.Forest(1, 1, per, 0.0, 0.0, 10.0, n, 0, 0)
+    rng = np.random.default_rng(40 + n)
+    q = f.soln().reshape(-1, 4)
+    rho, u, v, pr = (rng.uniform(0.5, 1.5, len(q)), rng.uniform(-0.3, 0.3, len(q)),
+                     rng.uniform(-0.3, 0.3, len(q)), rng.uniform(0.5, 1.5, len(q)))
+    q[:] = np.stack([rho, rho * u, rho * v, pr / 0.4 + 0.5 * rho * (u * u + v * v)], axis=1)
+    ci, fi, geom, soln = f.external()
+    assert f.nquad == 1
+    setup(gpu, n, 3, 10.0, bc, 0.3, 1.0, ci, fi, geom, soln)
+    ref = oracle.rhs(oracle.params(n, bc=bc, h_base=10.0), ci, fi, geom, soln)
+    assert parity_err(gpu.rhs(), ref) <= RHS_TOL
+    assert gpu.cfl()[0] == oracle.time_step(oracle.params(n, bc=bc, cfl=0.3, h_base=10.0), ci, geom, soln, 1.0)
+    gpu.run(5)
+    p = oracle.params(n, bc=bc, cfl=0.3, dt=1.0, h_base=10.0, scheme=3)
+    q_ref, _, _ = oracle.run(p, ci, fi, geom, soln, 5)
+    assert parity_err(gpu.soln(), q_ref) <= TRAJ_TOL
+
+
 def test_rhs_config1_vortex(gpu, oracle):
     """the isentropic vortex on the uniform level-4 forest with 8x8 patches"""
     log("vortex level 4 n=8")
