@@ -6,6 +6,7 @@
  */
 #define _POSIX_C_SOURCE 200809L
 #include "amr_hip.h"
+#include "amr_internal.h"
 
 #include <limits.h>
 #include <math.h>
@@ -181,6 +182,12 @@ int physics_go_gpu(int *dim, int *unstructured, int *nfields, int *nelem_subgrid
     if (*istep >= g_p.time_steps) {
         *evolve_solution = 0;
         return AMRGPU_OK;
+    }
+    /* one GPU owns the whole forest, so an interval always has quadrants
+     * (the reference's empty MPI ranks have no counterpart here) */
+    if (*ncell <= 0 || *nface < 0) {
+        amr_set_error("physics_go_gpu: empty mesh (ncell %d, nface %d)", *ncell, *nface);
+        return AMRGPU_ERR_ARG;
     }
     if ((rc = ensure_ctx())) return rc;
 

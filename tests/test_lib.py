@@ -71,6 +71,25 @@ def test_physics_go_gpu_early_exit_without_gpu():
     assert rc == 0 and evolve.value == 0     # physics_go.c:66-69
 
 
+def test_physics_go_gpu_rejects_empty_mesh_without_gpu():
+    """one GPU owns the whole forest: an interval with no quadrants is a
+    caller error, reported before any device is touched, state untouched"""
+    amrhip.set_params(time_steps=5, time_scheme=3)
+    L = amrhip.lib()
+    z = [C.c_int(0) for _ in range(16)]       # ncell = nghost = nface = dof = 0, ...
+    dim, uns, nf, n = C.c_int(2), C.c_int(0), C.c_int(4), C.c_int(8)
+    istep, evolve = C.c_int(0), C.c_int(1)
+    flags = [C.c_int(0) for _ in range(3)]
+    d = [C.c_double(0.0) for _ in range(3)]
+    rc = L.physics_go_gpu(C.byref(dim), C.byref(uns), C.byref(nf), C.byref(n),
+                          *[C.byref(x) for x in z[:11]], None, None, C.byref(istep),
+                          *[C.byref(x) for x in flags], C.byref(evolve), None,
+                          *[C.byref(x) for x in d])
+    assert rc == -1                           # AMRGPU_ERR_ARG (include/amr_hip.h)
+    assert b"empty mesh" in amrhip.lib().amrgpu_last_error()
+    assert istep.value == 0 and evolve.value == 1
+
+
 def test_threaded_cpu_baseline_matches_serial(oracle):
     rng = np.random.default_rng(21)
     f = amrhip.Forest(1, 1, (0, 1), 0.0, 0.0, 1.0, 4, 2, 4)
