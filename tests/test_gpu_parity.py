@@ -285,13 +285,17 @@ def test_batched_instances_match_single(gpu, n):
     assert t[0] == ga[0][1] and t[1] == ga[1][1]
 
 
-def test_physics_go_gpu_interval(oracle):
-    """the drop-in entry point: interval semantics and state vs the oracle"""
+@pytest.mark.parametrize("n", [4, 8, 16])
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4, 5])
+def test_physics_go_gpu_interval(oracle, scheme, n):
+    """the drop-in entry point: interval semantics and state vs the oracle,
+    for every time scheme and patch size (RK2 takes no CFL step: the input
+    dt, kept small, is its step)"""
     import ctypes as C
-    n = 8
     f = vortex_forest(n, 2)
     ci, fi, geom, soln = f.external()
-    amrhip.set_params(nelem_subgrid=n, time_scheme=3, time_steps=12, dt=1.0, cfl=0.3,
+    dt_in = 0.01 if scheme == 2 else 1.0
+    amrhip.set_params(nelem_subgrid=n, time_scheme=scheme, time_steps=12, dt=dt_in, cfl=0.3,
                       coarse_grid_h=10.0, regrid_interval=5, checkpoint_interval=2 ** 31 - 1,
                       visualization_interval=2 ** 31 - 1, bc=(0, 0, 0, 0, 1, 1), t_end=0.0)
     L = amrhip.lib()
@@ -312,7 +316,7 @@ def test_physics_go_gpu_interval(oracle):
                           C.byref(chk), C.byref(ev), None, C.byref(tt), C.byref(ct), C.byref(rt))
     assert rc == 0
     assert istep.value == 5 and rg.value == 1 and ev.value == 1
-    p = oracle.params(n, cfl=0.3, dt=1.0, h_base=10.0, scheme=3)
+    p = oracle.params(n, cfl=0.3, dt=dt_in, h_base=10.0, scheme=scheme)
     q_ref, _, _ = oracle.run(p, ci, fi, geom, soln, 5)
     assert np.max(np.abs(q - q_ref) / np.maximum(np.abs(q_ref), 1.0)) <= TRAJ_TOL
 
