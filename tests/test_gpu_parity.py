@@ -188,20 +188,29 @@ def test_rk_trajectory(gpu, oracle, scheme, n):
     assert dt[0] == pytest.approx(dt_ref, rel=1e-12)
 
 
-def test_graph_and_direct_launch_identical(gpu):
-    f = riemann_forest(8, seed=3, lmax=4)
+@pytest.mark.parametrize("n", [4, 8, 16])
+@pytest.mark.parametrize("scheme", [3, 4, 5])
+def test_graph_and_direct_launch_identical(gpu, n, scheme):
+    """replays of the captured step graphs equal direct launches bit for bit,
+    checked after every replay: each run of 2 steps replays one captured
+    step, then launches the last (norm) step directly, with the run's own
+    eager work (CFL priming, norm download, synchronisation) between
+    replays"""
+    f = riemann_forest(n, seed=3, lmax=4 if n < 16 else 3)
     ci, fi, geom, soln = f.external()
-    setup(gpu, 8, 3, 1.0, (0, 1, 2, 0), 0.3, 1.0, ci, fi, geom, soln)
+    setup(gpu, n, scheme, 1.0, (0, 1, 2, 0), 0.3, 1.0, ci, fi, geom, soln)
     assert not gpu.graphs()          # direct launches unless asked otherwise
-    gpu.set_graphs(True)
-    gpu.run(7, graph=True)           # replays of the captured step graphs
-    a = gpu.soln()
+    direct = []
+    for _ in range(5):
+        gpu.run(2, graph=True)       # graph requested, replay disabled: direct
+        direct.append(gpu.soln())
     gpu.upload(soln)
     gpu.set_time([0.0])
+    gpu.set_graphs(True)
+    for k in range(5):
+        gpu.run(2, graph=True)       # one replay of a captured step per run
+        assert np.array_equal(gpu.soln(), direct[k]), k
     gpu.set_graphs(False)
-    gpu.run(7, graph=True)           # graph requested, replay disabled: direct
-    b = gpu.soln()
-    assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("n", [2, 4, 8, 16])
