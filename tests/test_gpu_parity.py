@@ -151,6 +151,27 @@ def test_single_quadrant(gpu, oracle, n, periodic):
     assert parity_err(gpu.soln(), q_ref) <= TRAJ_TOL
 
 
+@pytest.mark.parametrize("n", [4, 8, 16])
+def test_nan_residual_is_reported(gpu, n):
+    """a non-physical state (negative density in one cell: sqrt(rho) of the
+    Roe averages is NaN here as in the reference) turns the residual into
+    NaN: the run reports AMRGPU_ERR_NAN (3), where the reference aborts
+    (physics_runge_kutta_interface.c:352-355); the context stays usable for
+    a valid state.  (A negative energy alone need not: the Roe-averaged
+    enthalpy, and with it the sound speed, can stay real.)"""
+    f = vortex_forest(n, 2)
+    ci, fi, geom, soln = f.external()
+    bad = np.array(soln, copy=True).reshape(-1, 4)
+    bad[5, 0] = -1.0
+    setup(gpu, n, 3, 10.0, (0, 0, 0, 0), 0.3, 1.0, ci, fi, geom, bad.reshape(-1))
+    with pytest.raises(FloatingPointError):
+        gpu.run(2)
+    assert b"nan" in amrhip.lib().amrgpu_last_error()
+    gpu.upload(soln)
+    gpu.set_time([0.0])
+    assert math.isfinite(gpu.run(2))
+
+
 def test_rhs_config1_vortex(gpu, oracle):
     """the isentropic vortex on the uniform level-4 forest with 8x8 patches"""
     log("vortex level 4 n=8")
