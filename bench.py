@@ -260,7 +260,11 @@ def batched_experiment(amrhip, args, model):
                                 "device_remesh": round(tot.wall_remesh, 3)},
              "cell_stages": useful,
              "end_to_end_Mcell_upd_s": useful / wall / 1e6,
-             "launched_Mcell_upd_s": tot.cell_stages / wall / 1e6}
+             "launched_Mcell_upd_s": tot.cell_stages / wall / 1e6,
+             # the stage launches alone: every launched cell-stage over the
+             # run's GPU time, and that rate's algorithmic bytes against HBM peak
+             "gpu_Mcell_upd_s": tot.cell_stages / tot.wall_gpu / 1e6,
+             "gpu_hbm_frac": tot.cell_stages / tot.wall_gpu * B_ALG_STAGE / (HBM_PEAK_GBS * 1e9)}
         if tagger is None:
             same = [bool(a.quads() == b.quads() and np.array_equal(a.soln(), b.soln())) for a, b in zip(alone, fb)]
             r["instances_bitwise_equal_to_alone"] = int(sum(same))
