@@ -1263,6 +1263,8 @@ struct amrgpu_ctx {
     // autotuner measures it faster.
     bool graphs_enabled = false;
     std::vector<InstDev> h_ip;
+    int *h_pin = nullptr;      // pinned staging of the mesh uploads (ints)
+    size_t h_pin_cap = 0;
 };
 
 namespace {
@@ -1577,6 +1579,7 @@ void amrgpu_destroy(amrgpu_ctx *c)
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
     free_mesh(c);
+    if (c->h_pin) (void)hipHostFree(c->h_pin);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1659,12 +1662,33 @@ static std::vector<int> levels_of(int ncell, const int *cell_info)
 }
 
 // Upload the face table, levels and instances of a mesh already validated.
-static int upload_mesh(amrgpu_ctx *c, int ncell, const int *level, const int *cell_inst,
-                       const std::vector<int> &nbr)
+// Pinned host staging for the uploads of setup, remesh and origins: copies
+// from it are plain asynchronous DMA, where pageable ones go through the
+// runtime's own staging call by call.  Every user synchronises the stream
+// before it returns, so no copy from the buffer is pending whenever it is
+// written; growing it synchronises first.
+static int pin_staging(amrgpu_ctx *c, size_t nint)
 {
-    std::vector<int> qi(ncell, 0);
-    if (cell_inst)
-        for (int q = 0; q < ncell; ++q) qi[q] = cell_inst[q];
+    if (nint <= c->h_pin_cap) return AMRGPU_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->h_pin) (void)hipHostFree(c->h_pin);
+    c->h_pin = nullptr;
+    c->h_pin_cap = 0;
+    const size_t cap = nint + nint / 2 + 64;
+    HIPCHK(hipHostMalloc((void **)&c->h_pin, sizeof(int) * cap, hipHostMallocDefault));
+    c->h_pin_cap = cap;
+    return AMRGPU_OK;
+}
+
+static int upload_mesh(amrgpu_ctx *c, int ncell, const int *level, const int *cell_inst, const int *nbr)
+{
+    int rc = pin_staging(c, 10 * (size_t)ncell);     // nbr, level, instance
+    if (rc) return rc;
+    int *pn = c->h_pin, *pl = pn + 8 * (size_t)ncell, *pq = pl + ncell;
+    memcpy(pn, nbr, sizeof(int) * 8 * (size_t)ncell);
+    memcpy(pl, level, sizeof(int) * (size_t)ncell);
+    if (cell_inst) memcpy(pq, cell_inst, sizeof(int) * (size_t)ncell);
+    else memset(pq, 0, sizeof(int) * (size_t)ncell);
     c->nquad = ncell;
     if (c->variant < 0 || kVariants[c->variant].n != c->n) c->variant = default_variant(c->n);
     c->nblocks = (ncell + kVariants[c->variant].ppb - 1) / kVariants[c->variant].ppb;
@@ -1672,9 +1696,9 @@ static int upload_mesh(amrgpu_ctx *c, int ncell, const int *level, const int *ce
         set_err("internal capacity error");
         return AMRGPU_ERR_ARG;
     }
-    HIPCHK(hipMemcpyAsync(c->d_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_level, level, sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_inst, qi.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_nbr, pn, sizeof(int) * 8 * (size_t)ncell, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_level, pl, sizeof(int) * (size_t)ncell, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_inst, pq, sizeof(int) * (size_t)ncell, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return AMRGPU_OK;
 }
@@ -1743,7 +1767,7 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme, int ninst,
     HIPCHK(hipMemcpyAsync(c->d_t, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_tstep, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_dt_cur, zeros.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
-    rc = upload_mesh(c, ncell, levels_of(ncell, cell_info).data(), cell_inst, nbr);
+    rc = upload_mesh(c, ncell, levels_of(ncell, cell_info).data(), cell_inst, nbr.data());
     if (rc) return rc;
     c->org_ok = false;
     if (geom) {
@@ -1792,8 +1816,8 @@ static int check_src(int ncell, const int *src, int nold, const char *who)
 }
 
 // Remap the state onto a validated new mesh and upload its table.
-static int remesh_to(amrgpu_ctx *c, int ncell, const int *level, const int *cell_inst,
-                     const std::vector<int> &nbr, const int *src)
+static int remesh_to(amrgpu_ctx *c, int ncell, const int *level, const int *cell_inst, const int *nbr,
+                     const int *src)
 {
     const int n2 = c->n * c->n, nold = c->nquad;
     int rc = AMRGPU_OK;
@@ -1803,9 +1827,14 @@ static int remesh_to(amrgpu_ctx *c, int ncell, const int *level, const int *cell
     const size_t old_state = (size_t)nold * 4 * n2;
     const bool fits = ncell <= c->cap_quad && (size_t)ncell * 4 * n2 <= c->cap_state;
     if (fits) {
-        // old state in U[cur], new state into the next buffer
+        // old state in U[cur], new state into the next buffer; the source
+        // map goes through the staging after the table's part of it
         const int nx = (c->cur + 1) % 3;
-        HIPCHK(hipMemcpyAsync(c->d_map, src, sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
+        rc = pin_staging(c, 11 * (size_t)ncell);
+        if (rc) return rc;
+        int *ps = c->h_pin + 10 * (size_t)ncell;
+        memcpy(ps, src, sizeof(int) * (size_t)ncell);
+        HIPCHK(hipMemcpyAsync(c->d_map, ps, sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(remap_kernel, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, c->stream,
                            c->U[c->cur], c->U[nx], c->d_map, ncells, n2);
         HIPCHK(hipGetLastError());
@@ -1854,7 +1883,7 @@ int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cel
     rc = build_nbr(ncell, nface, face_info, nbr);
     if (rc) return rc;                 // a refused call leaves the context as it was
     c->org_ok = false;                 // the new quadrants' coordinates come with amrgpu_set_geometry
-    return remesh_to(c, ncell, levels_of(ncell, cell_info).data(), cell_inst, nbr, src);
+    return remesh_to(c, ncell, levels_of(ncell, cell_info).data(), cell_inst, nbr.data(), src);
 }
 
 int amrgpu_remesh_nbr(amrgpu_ctx *c, int ncell, const int *level, const int *cell_inst, const int *nbr,
@@ -1885,13 +1914,18 @@ int amrgpu_remesh_nbr(amrgpu_ctx *c, int ncell, const int *level, const int *cel
         }
     }
     c->org_ok = false;                 // the new quadrants' coordinates come with amrgpu_set_origins
-    return remesh_to(c, ncell, level, cell_inst, std::vector<int>(nbr, nbr + 8 * (size_t)ncell), src);
+    return remesh_to(c, ncell, level, cell_inst, nbr, src);
 }
 
 int amrgpu_set_origins(amrgpu_ctx *c, const double *org)
 {
     if (!c || !c->nquad || !org) { set_err("amrgpu_set_origins: no mesh"); return AMRGPU_ERR_ARG; }
-    HIPCHK(hipMemcpyAsync(c->d_org, org, sizeof(double) * 2 * (size_t)c->nquad, hipMemcpyHostToDevice, c->stream));
+    // no copy from the staging is pending (its users synchronise first)
+    int rc = pin_staging(c, 4 * (size_t)c->nquad);
+    if (rc) return rc;
+    memcpy(c->h_pin, org, sizeof(double) * 2 * (size_t)c->nquad);
+    HIPCHK(hipMemcpyAsync(c->d_org, c->h_pin, sizeof(double) * 2 * (size_t)c->nquad, hipMemcpyHostToDevice,
+                          c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->org_ok = true;
     return AMRGPU_OK;
