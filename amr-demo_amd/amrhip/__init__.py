@@ -88,6 +88,7 @@ EXPORTS = {
     "amr_evolve": (_i, [_vp, _d, _i, _vp]),
     "amr_evolve_device": (_i, [_vp, _vp, _vp]),
     "amr_evolve_batch": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "amr_forest_host_parallel_enabled": (_i, []),
     "amr_forest_track_begin": (None, [_vp, _i]),
     "amr_forest_track_src": (_pi, [_vp]),
     "amr_forest_track_end": (_i, [_vp]),

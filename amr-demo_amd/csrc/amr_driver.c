@@ -332,10 +332,10 @@ int amr_evolve(amr_forest_t *f, double tol, int allow_coarsening, amr_evolve_sta
         int before = istep;
         /* soln is the forest's own patch array: updated in place
          * (external_p4est_to_soln / external_soln_to_p4est) */
-        rc = hpc_amr_evolve_solver(&dim, &uns, &nfields, &n, &ncell, &nghost, &nface, &dof,
-                                   cell_info, face_info, &nsend, &nrecv, &dummy, &dummy, &dummy,
-                                   geom, soln, &istep, &regrid, &visualize, &checkpoint, &evolve,
-                                   NULL, &total, &comp, &res);
+        rc = physics_go_gpu(&dim, &uns, &nfields, &n, &ncell, &nghost, &nface, &dof,
+                            cell_info, face_info, &nsend, &nrecv, &dummy, &dummy, &dummy,
+                            geom, soln, &istep, &regrid, &visualize, &checkpoint, &evolve,
+                            NULL, &total, &comp, &res);
         double t2 = now_s();
         s.wall_gpu += t2 - t1;
         s.wall_regrid += t1 - t0;
@@ -760,7 +760,9 @@ typedef struct {
 static void *regrid_worker(void *arg)
 {
     regrid_job_t *j = (regrid_job_t *)arg;
-    amr_forest_host_parallel(0);       /* the instances are the parallelism here */
+    /* the instances are the parallelism here; the caller's thread also runs
+     * this worker, so its own setting is restored on the way out */
+    const int was_parallel = amr_forest_host_parallel(0);
     for (;;) {
         pthread_mutex_lock(&j->lock);
         const int k = j->next++;
@@ -777,6 +779,7 @@ static void *regrid_worker(void *arg)
             tab_of(j->f[k], &j->tab[k]);
         }
     }
+    amr_forest_host_parallel(was_parallel);
     return NULL;
 }
 

@@ -55,7 +55,14 @@ static inline int qlen(int level) { return 1 << (AMR_MAXLEVEL - level); }
 static _Thread_local int t_parallel = 1;
 #define PAR_MIN 1024
 
-void amr_forest_host_parallel(int on) { t_parallel = on; }
+int amr_forest_host_parallel(int on)
+{
+    const int was = t_parallel;
+    t_parallel = on;
+    return was;
+}
+
+int amr_forest_host_parallel_enabled(void) { return t_parallel; }
 
 static inline uint64_t spread(uint32_t v)
 {

@@ -1840,12 +1840,15 @@ static int remesh_to(amrgpu_ctx *c, int ncell, const int *level, const int *cell
         HIPCHK(hipGetLastError());
         c->cur = nx;
     } else {
-        // keep the old state, time and dt in scratch allocations, regrow, remap
+        // keep the old state, time, dt and step start time in scratch
+        // allocations, regrow, remap (the residual of a remeshed state reads
+        // the step start time before any step rewrites it)
         double *keep = nullptr;
         int *map = nullptr;
-        std::vector<double> t(c->ninst), dt(c->ninst);
+        std::vector<double> t(c->ninst), dt(c->ninst), ts(c->ninst);
         HIPCHK(hipMemcpyAsync(t.data(), c->d_t, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(dt.data(), c->d_dt_cur, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(ts.data(), c->d_tstep, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMalloc(&keep, sizeof(double) * old_state));
         HIPCHK(hipMalloc(&map, sizeof(int) * ncell));
         HIPCHK(hipMemcpyAsync(keep, c->U[c->cur], sizeof(double) * old_state, hipMemcpyDeviceToDevice, c->stream));
@@ -1858,6 +1861,7 @@ static int remesh_to(amrgpu_ctx *c, int ncell, const int *level, const int *cell
         HIPCHK(hipMemcpyAsync(c->d_ip, c->h_ip.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(c->d_t, t.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(c->d_dt_cur, dt.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_tstep, ts.data(), sizeof(double) * ninst, hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(remap_kernel, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, c->stream,
                            keep, c->U[0], map, ncells, n2);
         HIPCHK(hipGetLastError());
@@ -2227,75 +2231,105 @@ int amrgpu_get_tile(amrgpu_ctx *c, int *ppb, int *minw, int *design)
     return AMRGPU_OK;
 }
 
-int amrgpu_autotune(amrgpu_ctx *c, int nsteps, int *nvar, int *ppb, int *minw, int *design,
-                    double *ms_per_step)
+// The autotuner's trials, each followed by a restore of the state copy in
+// d_stage and the per-instance time t0; the caller restores both once more
+// on every exit, so an error in a trial leaves the context as it was.
+static int restore_trial(amrgpu_ctx *c, int cur0, size_t nstate, const std::vector<double> &t0)
 {
-    if (!c || !c->nquad || nsteps <= 0) { set_err("amrgpu_autotune: no mesh"); return AMRGPU_ERR_ARG; }
-    // the trial steps advance the state: keep a device copy of it and the
-    // per-instance time, and restore both afterwards
-    const size_t nstate = (size_t)c->nquad * 4 * c->n * c->n;
-    std::vector<double> t0(c->ninst);
-    HIPCHK(hipMemcpyAsync(t0.data(), c->d_t, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_stage, c->U[c->cur], sizeof(double) * nstate, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->U[cur0], c->d_stage, sizeof(double) * nstate, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_t, t0.data(), sizeof(double) * c->ninst, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    const int cur0 = c->cur;
-    int best = -1, k = 0;
+    c->cur = cur0;
+    return AMRGPU_OK;
+}
+
+static int timed_trial(amrgpu_ctx *c, int nsteps, hipEvent_t e0, hipEvent_t e1, double *per)
+{
+    int rc = prime_run(c);
+    if (rc) return rc;
+    for (int s = 0; s < 2; ++s) if ((rc = one_step(c, true, false))) return rc;   // warm + capture
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int s = 0; s < nsteps; ++s) if ((rc = one_step(c, true, false))) return rc;
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *per = ms / nsteps;
+    return AMRGPU_OK;
+}
+
+static int autotune_trials(amrgpu_ctx *c, int nsteps, int cur0, size_t nstate, const std::vector<double> &t0,
+                           hipEvent_t e0, hipEvent_t e1, int *nvar, int *ppb, int *minw, int *design,
+                           double *ms_per_step)
+{
+    int best = -1, k = 0, rc;
     double best_ms = 1e300;
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
     for (int v = 0; v < kNumVariants; ++v) {
         if (kVariants[v].n != c->n) continue;
-        int rc = amrgpu_set_tile(c, kVariants[v].ppb, kVariants[v].minw, kVariants[v].design);
-        if (rc) return rc;
-        rc = prime_run(c);
-        if (rc) return rc;
-        for (int s = 0; s < 2; ++s) if ((rc = one_step(c, true, false))) return rc;   // warm + capture
-        HIPCHK(hipEventRecord(e0, c->stream));
-        for (int s = 0; s < nsteps; ++s) if ((rc = one_step(c, true, false))) return rc;
-        HIPCHK(hipEventRecord(e1, c->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        double per = ms / nsteps;
+        if ((rc = amrgpu_set_tile(c, kVariants[v].ppb, kVariants[v].minw, kVariants[v].design))) return rc;
+        double per = 0.0;
+        if ((rc = timed_trial(c, nsteps, e0, e1, &per))) return rc;
         if (ppb) ppb[k] = kVariants[v].ppb;
         if (minw) minw[k] = kVariants[v].minw;
         if (design) design[k] = kVariants[v].design;
         if (ms_per_step) ms_per_step[k] = per;
         ++k;
         if (per < best_ms) { best_ms = per; best = v; }
-        // restore the state for the next trial
-        HIPCHK(hipMemcpyAsync(c->U[cur0], c->d_stage, sizeof(double) * nstate, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(c->d_t, t0.data(), sizeof(double) * c->ninst, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        c->cur = cur0;
+        if ((rc = restore_trial(c, cur0, nstate, t0))) return rc;
     }
     if (nvar) *nvar = k;
-    int rc = amrgpu_set_tile(c, kVariants[best].ppb, kVariants[best].minw, kVariants[best].design);
-    if (rc) return rc;
+    if (best < 0) { set_err("amrgpu_autotune: no tile variant for n = %d", c->n); return AMRGPU_ERR_ARG; }
+    if ((rc = amrgpu_set_tile(c, kVariants[best].ppb, kVariants[best].minw, kVariants[best].design))) return rc;
     // launch mode for the chosen tile: direct launches against graph replay
     double per_mode[2] = {0.0, 0.0};
     for (int m = 0; m < 2; ++m) {
         c->graphs_enabled = (m == 1);
-        if ((rc = prime_run(c))) return rc;
-        for (int s = 0; s < 2; ++s) if ((rc = one_step(c, true, false))) return rc;
-        HIPCHK(hipEventRecord(e0, c->stream));
-        for (int s = 0; s < nsteps; ++s) if ((rc = one_step(c, true, false))) return rc;
-        HIPCHK(hipEventRecord(e1, c->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        per_mode[m] = ms / nsteps;
-        HIPCHK(hipMemcpyAsync(c->U[cur0], c->d_stage, sizeof(double) * nstate, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(c->d_t, t0.data(), sizeof(double) * c->ninst, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        c->cur = cur0;
+        if ((rc = timed_trial(c, nsteps, e0, e1, &per_mode[m]))) return rc;
+        if ((rc = restore_trial(c, cur0, nstate, t0))) return rc;
     }
     c->graphs_enabled = per_mode[1] < per_mode[0];
     if (!c->graphs_enabled) free_graphs(c);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return AMRGPU_OK;
+}
+
+int amrgpu_autotune(amrgpu_ctx *c, int nsteps, int *nvar, int *ppb, int *minw, int *design,
+                    double *ms_per_step)
+{
+    if (!c || !c->nquad || nsteps <= 0) { set_err("amrgpu_autotune: no mesh"); return AMRGPU_ERR_ARG; }
+    // the trial steps advance the state: keep a device copy of it and the
+    // per-instance time, and restore both afterwards, on every exit path
+    const size_t nstate = (size_t)c->nquad * 4 * c->n * c->n;
+    std::vector<double> t0(c->ninst);
+    HIPCHK(hipMemcpyAsync(t0.data(), c->d_t, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_stage, c->U[c->cur], sizeof(double) * nstate, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int cur0 = c->cur;
+    const bool graphs0 = c->graphs_enabled;
+    const int variant0 = c->variant, nblocks0 = c->nblocks;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = AMRGPU_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        set_err("amrgpu_autotune: hipEventCreate failed");
+        rc = AMRGPU_ERR_HIP;
+    } else {
+        rc = autotune_trials(c, nsteps, cur0, nstate, t0, e0, e1, nvar, ppb, minw, design, ms_per_step);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) {
+        // a failed trial: the tile and launch mode from before, the state
+        // and time from before (the error message is kept)
+        char msg[sizeof(g_err)];
+        memcpy(msg, g_err, sizeof(msg));
+        c->variant = variant0;
+        c->nblocks = nblocks0;
+        c->graphs_enabled = graphs0;
+        free_graphs(c);
+        (void)hipStreamSynchronize(c->stream);
+        (void)restore_trial(c, cur0, nstate, t0);
+        memcpy(g_err, msg, sizeof(msg));
+    }
+    return rc;
 }
 
 int amrgpu_set_graphs(amrgpu_ctx *c, int enable)

@@ -30,8 +30,9 @@ AMR_HIDDEN int amr_device_regrid(struct amr_forest *f, struct amrgpu_ctx *g, con
                                  int allow_coarsening, double *t_forest, double *t_remesh);
 AMR_HIDDEN double amr_now(void);
 /* Per-quadrant loops of the host forest run on OpenMP threads unless turned
- * off for the calling thread (the batch's per-instance threads do). */
-AMR_HIDDEN void amr_forest_host_parallel(int on);
+ * off for the calling thread (the batch's per-instance regrids do).  Returns
+ * the thread's previous setting, which the caller restores. */
+AMR_HIDDEN int amr_forest_host_parallel(int on);
 
 #ifdef __cplusplus
 }

@@ -5,7 +5,7 @@
  * amrgpu_* context; nothing here evaluates the scheme on the CPU.
  */
 #define _POSIX_C_SOURCE 200809L
-#include "amr_hip.h"
+#include "amr_hip_dropin.h"
 #include "amr_internal.h"
 
 #include <limits.h>

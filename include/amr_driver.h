@@ -97,8 +97,9 @@ typedef struct {
 } amr_evolve_stats_t;
 
 /* The evolve loop of src/amr/src/amr_evolve.c:15-73 on the host forest: per
- * interval build the boundary arrays (external_setup), call
- * hpc_amr_evolve_solver (the GPU physics module), and regrid when it raises
+ * interval build the boundary arrays (external_setup), call the GPU physics
+ * module through its seam, physics_go_gpu (where the reference calls
+ * hpc_amr_evolve_solver -> physics_evolve_interface), and regrid when it raises
  * the flag.  The physics settings come from physics_gpu_set_params /
  * hpc_amr_initialize_solver; tol is the density-gradient tagging tolerance.
  * Returns 0, or the physics module's error code. */

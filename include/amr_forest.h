@@ -47,6 +47,10 @@ void amr_forest_destroy(amr_forest_t *f);
 
 int amr_forest_num_quads(const amr_forest_t *f);
 int amr_forest_nelem_subgrid(const amr_forest_t *f);
+/* Whether the calling thread's per-quadrant forest loops run on OpenMP
+ * threads: on by default; a batch regrid turns it off on its own per-instance
+ * workers only, and leaves the caller's setting as it found it. */
+int amr_forest_host_parallel_enabled(void);
 const amr_quad_t *amr_forest_quads(const amr_forest_t *f);
 double *amr_forest_soln(amr_forest_t *f);     /* nquad * 4*n*n, reference layout */
 int *amr_forest_tags(amr_forest_t *f);        /* nquad */

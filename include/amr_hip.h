@@ -4,11 +4,17 @@
  *
  * Two layers:
  *
- * 1. Drop-in physics entry points.  Same names, argument lists and meaning
- *    as the reference physics module, so an AMR driver written against
- *    src/amr/src/amr_external.h binds them unchanged.  The only deviation is
- *    the communicator: this image has no MPI, and one GPU owns the whole
+ * 1. Drop-in physics entry points: the GPU seam physics_go_gpu that the
+ *    reference's physics_evolve_interface calls under _GPU_ (its header,
+ *    physics_go_gpu.h, is in this directory), and GPU counterparts of the
+ *    reference's residual and time-step functions.  Same argument lists and
+ *    meaning; the only deviation is the communicator: one GPU owns the whole
  *    forest, so the MPI_Comm argument is an opaque pointer that is ignored.
+ *    Nothing here re-declares a function the reference's physics module
+ *    defines, so this header can sit beside the reference's headers.  The
+ *    library also exports replacements of hpc_amr_initialize_solver and
+ *    hpc_amr_evolve_solver for drivers that link it instead of the
+ *    reference's physics module (amr_hip_dropin.h).
  *
  * 2. A device-resident context (amrgpu_*), with no reference counterpart,
  *    that keeps the state in HBM across steps, runs whole evolve intervals
@@ -50,16 +56,6 @@ typedef struct {
     int    verbose;
 } physics_gpu_params_t;
 
-/* Replaces hpc_amr_initialize_solver (src/physics/src/hpc_amr_initialize_solver.c:16-28
- * -> physics_initialize, src/physics/src/physics_initialize.c:16-58): reads
- * input.physics from the working directory with the same keywords, defaults,
- * CFL rescale and interval fix-up. */
-void hpc_amr_initialize_solver(int *rank, int *dim, double *mesh_scale, int *nlevels,
-                               int *nfields, int *nfringe, int *nelem_subgrid,
-                               int *extern_cell_size, int *extern_face_size,
-                               int *extern_edge_size, int *extern_node_size,
-                               int *extern_mpi_comm_size);
-
 /* Programmatic access to the same settings (no reference counterpart; the
  * reference writes the globals directly). */
 void physics_gpu_set_params(const physics_gpu_params_t *p);
@@ -81,16 +77,6 @@ int physics_go_gpu(int *dim, int *unstructured, int *nfields, int *nelem_subgrid
                    int *istep, int *regrid, int *visualize, int *checkpoint,
                    int *evolve_solution, void *mpicomm_ext,
                    double *total_time, double *compute_time, double *residual_time);
-
-/* Replaces hpc_amr_evolve_solver (src/physics/src/hpc_amr_evolve_solver.c:16-49,
- * declared at src/amr/src/amr_external.h:124-130); forwards to physics_go_gpu. */
-int hpc_amr_evolve_solver(int *dim, int *unstructured, int *nfields, int *nelem_subgrid,
-                          int *ncell, int *nghost, int *nface, int *dof,
-                          int *cell_info, int *face_info, int *nsend, int *nrecv,
-                          int *send, int *recv, int *sizes, double *geom, double *soln,
-                          int *istep, int *regrid, int *visualize, int *checkpoint,
-                          int *evolve_solution, void *mpicomm_ext,
-                          double *total_time, double *compute_time, double *residual_time);
 
 /* Replaces physics_rhs_structured (src/physics/src/physics_rhs_structured.c:19-22):
  * rhs[0:dof) = -R/V of soln, evaluated on the GPU (dim 2, nfields 4). */

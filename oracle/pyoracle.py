@@ -56,6 +56,7 @@ class Oracle:
             "orc_refine_2d": [_i, _pd, _pd, _pd, _pd, _pd],
             "orc_coarsen_2d": [_i, _pd, _pd, _pd, _pd, _pd],
             "orc_tag_feature": [_i, _pi, _pd, _i, _pd],
+            "orc_norm_L2_residual": [_i, _i, _pi, _pd],
             "orc_rk_generic": [_i, _i, _d, _pd, _pd, RHS_FN, _vp],
             "orc_bf16": [C.c_float],
             "orc_tagger_features": [_i, _i, _i, _pi, _pd, _vp],
@@ -63,7 +64,7 @@ class Oracle:
             "orc_mlp_grad": [_vp, _vp, _pi, _i, _vp, _i, _pd],
         }
         res = {"orc_run_steps": _d, "orc_run_steps_threaded": _d, "orc_rk_step": _d,
-               "orc_tag_feature": _i, "orc_rk_generic": _d, "orc_bf16": C.c_ushort,
+               "orc_tag_feature": _i, "orc_norm_L2_residual": _d, "orc_rk_generic": _d, "orc_bf16": C.c_ushort,
                "orc_tagger_features": _i, "orc_mlp_grad": _d}
         for k, a in sig.items():
             f = getattr(L, k)
@@ -115,6 +116,12 @@ class Oracle:
                                   s.size, dpt(s), dpt(out))
         return out
 
+    def norm(self, n, cell_info, res):
+        """sum of squares of res over the real patches of cell_info (no sqrt)"""
+        ci = np.ascontiguousarray(cell_info, np.int32)
+        r = np.ascontiguousarray(res, float)
+        return self.L.orc_norm_L2_residual(n, ci.size // 3, ipt(ci), dpt(r))
+
     def time_step(self, p, cell_info, geom, soln, dt0):
         ci = np.ascontiguousarray(cell_info, np.int32)
         g = np.ascontiguousarray(geom, float)
@@ -137,8 +144,10 @@ class Oracle:
                                       q.size, nsteps, dpt(q), C.byref(dt))
         return q, rn, dt.value
 
-    def rk_generic(self, scheme, dt, q0, rhs):
-        """one step of scheme on dq/dt = rhs(q) (rhs: numpy -> numpy)"""
+    def rk_generic(self, scheme, dt, q0, rhs, with_norm=False):
+        """one step of scheme on dq/dt = rhs(q) (rhs: numpy -> numpy); with
+        with_norm also the step's residual norm (sqrt of the last stage's sum
+        of squares)"""
         q = np.array(q0, dtype=float, copy=True)
         work = np.zeros(5 * q.size)
         n = q.size
@@ -149,8 +158,8 @@ class Oracle:
             ka[:] = rhs(qa.copy())
 
         f = RHS_FN(cb)
-        self.L.orc_rk_generic(scheme, n, dt, dpt(q), dpt(work), f, None)
-        return q
+        rn = self.L.orc_rk_generic(scheme, n, dt, dpt(q), dpt(work), f, None)
+        return (q, rn) if with_norm else q
 
     def ic(self, kind, n, geom, mach, alpha, beta, gamma, density, pressure, beta_l, xo, yo):
         q = np.zeros(4 * n * n)
@@ -230,6 +239,8 @@ class Reference:
         self._ic.argtypes = [_pi, _pi, _pi, _pi, _pd, _pd, _pd] + [_pd] * 6
         L.physics_rhs_structured.argtypes = [_pi, _pi, _pi, _pi, _pi, _pd, _pi, _pi, _pi, _pd, _pd]
         L.physics_time_step.argtypes = [_pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd]
+        L.physics_norm_L2_residual.argtypes = [_i, _i, _i, _i, _pi, _pd]
+        L.physics_norm_L2_residual.restype = _d
         L.solver_tag_feature.argtypes = [_pi, _pi, _pd, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pi]
         L.solver_refine_operator_2d.argtypes = [_pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd, _pd, _pd, _pd]
         L.solver_coarsen_operator_2d.argtypes = [_pi, _pi, _pi, _pi, _pd, _pd, _pd, _pd, _pd, _pd, _pd]
@@ -289,6 +300,13 @@ class Reference:
         self._ic(C.byref(a[0]), C.byref(a[1]), C.byref(a[2]), C.byref(a[3]), C.byref(d[0]),
                  dpt(np.ascontiguousarray(geom, float)), dpt(q), *[C.byref(x) for x in d[1:]])
         return q
+
+    def norm(self, n, cell_info, res):
+        """physics_norm_L2_residual: sum of squares (no sqrt) over the real
+        patches named by cell_info"""
+        ci = np.ascontiguousarray(cell_info, np.int32)
+        r = np.ascontiguousarray(res, float)
+        return self.L.physics_norm_L2_residual(2, 4, n, ci.size // 3, ipt(ci), dpt(r))
 
     def tag(self, methods, tols, n, q):
         m = np.ascontiguousarray(methods, np.int32)

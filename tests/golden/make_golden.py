@@ -17,6 +17,11 @@ Fixtures:
                  parameters)
   ops.npz        solver_tag_feature, solver_coarsen_operator_2d,
                  solver_refine_operator_2d
+  norm.npz       physics_norm_L2_residual on the golden meshes' residuals and
+                 on residual vectors with permuted patch offsets and ghost
+                 patches after the real ones
+
+    python tests/golden/make_golden.py norm     # norm.npz only (own seed)
 """
 import os
 import sys
@@ -159,10 +164,45 @@ def gen_ops(ref, rng):
                         tags=np.array(tags), refine_parent=parent, refine_children=np.array(ref.refine(n, parent)))
 
 
+def gen_norm(ref):
+    """sum of squares over the real patches, in cell order at each patch's
+    offset (physics_norm_L2_residual, src/physics/src/physics_norm_residual.c:15-56);
+    inputs: the committed golden meshes' residuals, and random residuals whose
+    magnitudes span many decades (so that the summation order shows in the
+    last bits) with cell_info offsets permuted and ghost patches appended"""
+    rng = np.random.default_rng(20261203)
+    cases = []
+    for name in ("uniform_periodic_n4", "uniform_periodic_n8", "hanging_walls_n4",
+                 "hanging_periodic_x_n8", "hanging_periodic_n2", "brick_3x2_n16"):
+        d = np.load(os.path.join(OUT, f"mesh_{name}.npz"))
+        cases.append((f"mesh_{name}", int(d["n"]), d["cell_info"], d["rhs"]))
+    for k, (n, ncell, nghost) in enumerate(((2, 7, 0), (4, 13, 3), (8, 21, 5), (16, 5, 2))):
+        nt = ncell + nghost
+        perm = rng.permutation(nt)[:ncell]
+        ci = np.zeros((ncell, 3), np.int32)
+        ci[:, 1] = rng.integers(0, 6, ncell)
+        ci[:, 2] = perm * 4 * n * n
+        res = rng.standard_normal(nt * 4 * n * n) * 10.0 ** rng.uniform(-8, 4, nt * 4 * n * n)
+        cases.append((f"random_n{n}_{k}", n, ci.reshape(-1), res))
+    out = {}
+    for name, n, ci, res in cases:
+        ref.configure(n)
+        val = ref.norm(n, ci, res)
+        out[f"{name}_n"] = np.int32(n)
+        out[f"{name}_cell_info"] = np.asarray(ci, np.int32)
+        out[f"{name}_res"] = np.asarray(res, float)
+        out[f"{name}_norm"] = np.float64(val)
+        print(f"norm {name}: n {n}, {len(ci) // 3} cells, sum of squares {val:.17g}")
+    np.savez_compressed(os.path.join(OUT, "norm.npz"), names=np.array([c[0] for c in cases]), **out)
+
+
 def main():
     if not pyoracle.ref_available():
         sys.exit("oracle/_ref/libamrref.so missing: run `make -C oracle ref` where /root/reference exists")
     ref = pyoracle.Reference()
+    if sys.argv[1:] == ["norm"]:
+        gen_norm(ref)
+        return
     rng = np.random.default_rng(20240611)
     gen_flux(ref, rng)
     gen_bcflux(ref, rng)
@@ -174,6 +214,7 @@ def main():
     gen_mesh(ref, rng, "brick_3x2_n16", 16, 3, 2, (0, 1), 0.5, 0, 2, (2, 0, 1, 1), 0.3, 2)
     gen_ic(ref, rng)
     gen_ops(ref, rng)
+    gen_norm(ref)
 
 
 if __name__ == "__main__":

@@ -53,6 +53,9 @@ def test_batch_equals_each_instance_alone(allow_coarsening, lag):
         steps.append(sa.steps)
     # the instances finish at different step counts: the batch really froze some
     assert len(set(steps)) > 1
+    # the batch regrid ran its first worker on this thread: the thread's own
+    # host-parallel setting is as it was
+    assert amrhip.lib().amr_forest_host_parallel_enabled() == 1
     assert tot.steps == max(steps) or tot.steps - max(steps) < 16
 
 

@@ -199,7 +199,7 @@ def test_rk_trajectory(gpu, oracle, scheme, n):
     dt_cap = 1.0 if scheme != 2 else 0.01
     setup(gpu, n, scheme, 10.0, (0, 0, 0, 0), cfl, dt_cap, ci, fi, geom, soln)
     nsteps = 10
-    gpu.run(nsteps)
+    rn = gpu.run(nsteps)
     q_gpu = gpu.soln()
     p = oracle.params(n, cfl=cfl, dt=dt_cap, h_base=10.0, scheme=scheme)
     q_ref, rn_ref, dt_ref = oracle.run(p, ci, fi, geom, soln, nsteps)
@@ -207,6 +207,10 @@ def test_rk_trajectory(gpu, oracle, scheme, n):
     assert err <= TRAJ_TOL, err
     t, dt = gpu.time()
     assert dt[0] == pytest.approx(dt_ref, rel=1e-12)
+    # the residual norm of the run's last stage (physics_norm_L2_residual
+    # after the SUM allreduce, then sqrt): the fused block sums of the GPU
+    # against the oracle's running sum
+    assert rn == pytest.approx(rn_ref, rel=1e-10), (rn, rn_ref)
 
 
 @pytest.mark.parametrize("n", [4, 8, 16])
@@ -348,7 +352,7 @@ def test_physics_go_gpu_interval(oracle, scheme, n):
     assert istep.value == 5 and rg.value == 1 and ev.value == 1
     p = oracle.params(n, cfl=0.3, dt=dt_in, h_base=10.0, scheme=scheme)
     q_ref, _, _ = oracle.run(p, ci, fi, geom, soln, 5)
-    assert np.max(np.abs(q - q_ref) / np.maximum(np.abs(q_ref), 1.0)) <= TRAJ_TOL
+    assert parity_err(q, q_ref) <= TRAJ_TOL
 
 
 def test_device_indicator_matches_host(gpu):
@@ -599,3 +603,32 @@ def test_position_walls_need_geometry_after_remesh(gpu):
         gpu.run(1)
     amrhip.check(amrhip.lib().amrgpu_set_geometry(gpu.h, amrhip.dpt(np.ascontiguousarray(geom))), "geom")
     gpu.run(1)
+
+
+def test_rhs_after_regrow_remesh_keeps_step_time(oracle):
+    """a remesh that outgrows the context's buffers (every quadrant of a
+    uniform level-1 forest refined: 16 -> 64 quadrants, capacity 36) keeps
+    the step start time that the moving-shock wall (type 5) reads, so the
+    residual right after it matches the oracle's at that time"""
+    f = amrhip.Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, 8, 1, 3)
+    f.apply_ic(amrhip.dmr_ic())
+    ci, fi, geom, soln = f.external()
+    nq0 = f.nquad
+    g = amrhip.Gpu()
+    try:
+        g.setup(8, 3, [amrhip.dmr_inst(1.0, 0.3, 1.0)], ci, fi, geom, soln)
+        t = 0.07
+        g.set_time([t])
+        f.track_begin(keep_data=True)
+        f.regrid_tags(np.ones(nq0, np.int32))
+        ok, src = f.track_end()
+        assert ok and f.nquad == 4 * nq0 > nq0 + nq0 // 4 + 16
+        ci2, fi2, geom2, soln2 = f.external()
+        g.remesh(ci2, fi2, src)
+        amrhip.check(amrhip.lib().amrgpu_set_geometry(g.h, amrhip.dpt(np.ascontiguousarray(geom2))), "geom")
+        assert np.array_equal(g.soln(), soln2)
+        ref = oracle.rhs(dmr_oracle_params(oracle, t), ci2, fi2, geom2, soln2)
+        assert parity_err(g.rhs(), ref) <= RHS_TOL
+        assert g.time()[0][0] == t
+    finally:
+        g.close()

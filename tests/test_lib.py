@@ -18,7 +18,7 @@ def header_functions():
         txt = open(h).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", txt, flags=re.M):
-            if m.group(0).startswith("typedef"):
+            if m.group(0).startswith(("typedef", "static")):   # header-only wrappers
                 continue
             if m.group(1) not in ("if", "while", "for", "return", "sizeof"):
                 names.add(m.group(1))

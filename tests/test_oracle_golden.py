@@ -68,3 +68,20 @@ def test_coarsen_and_tag_ops(oracle):
     for k, r in zip(kids, d["refine_children"]):
         assert np.array_equal(k, r) and np.array_equal(k, parent)
     _ = C
+
+
+def norm_cases():
+    d = np.load(os.path.join(GOLDEN, "norm.npz"))
+    return [str(x) for x in d["names"]]
+
+
+@pytest.mark.parametrize("name", norm_cases())
+def test_norm_L2_residual_bitwise(oracle, name):
+    """orc_norm_L2_residual against physics_norm_L2_residual: the golden
+    meshes' residuals, and random residuals spanning 12 decades with permuted
+    patch offsets and ghost patches (so the summation order and the offsets
+    both show in the bits)"""
+    d = np.load(os.path.join(GOLDEN, "norm.npz"))
+    n = int(d[f"{name}_n"])
+    got = oracle.norm(n, d[f"{name}_cell_info"], d[f"{name}_res"])
+    assert got == float(d[f"{name}_norm"])

@@ -184,3 +184,44 @@ def test_parity_metric_catches_planted_mistake(oracle):
     bad[k] *= -1.0
     assert parity_err(rhs, rhs) == 0.0
     assert parity_err(bad, rhs) > 1e-6
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16])
+def test_norm_closed_forms(oracle, n):
+    """the residual sum of squares counts exactly the real patches at their
+    cell_info offsets: all-ones gives 4 n^2 ncell exactly, ghost patches and
+    patches no cell names do not count, and the value equals the correctly
+    rounded sum (math.fsum) to within the rounding of a plain running sum"""
+    import math
+    rng = np.random.default_rng(n)
+    ncell, nghost = 11, 4
+    m = 4 * n * n
+    ci = np.zeros((ncell, 3), np.int32)
+    perm = rng.permutation(ncell + nghost)
+    ci[:, 2] = perm[:ncell] * m
+    ones = np.ones((ncell + nghost) * m)
+    assert oracle.norm(n, ci.reshape(-1), ones) == 4.0 * n * n * ncell
+    res = rng.standard_normal((ncell + nghost) * m)
+    real = np.concatenate([res[o:o + m] for o in ci[:, 2]])
+    exact = math.fsum(real * real)
+    got = oracle.norm(n, ci.reshape(-1), res)
+    assert abs(got - exact) <= 1e-13 * exact
+    poisoned = res.copy()
+    for g in perm[ncell:]:
+        poisoned[g * m:(g + 1) * m] = np.nan
+    assert oracle.norm(n, ci.reshape(-1), poisoned) == got
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4])
+def test_rk_norm_is_last_stage(oracle, scheme):
+    """the step's residual norm is the 2-norm of the LAST stage's residual
+    (rk_time_integrators.c keeps only the last physics_norm_L2_residual):
+    on L(Q) = lambda Q the last stage evaluates L at a known polynomial of
+    z = lambda dt times Q0 (RK1: Q0; SSP-RK2: 1 + z; SSP-RK3: 1 + z/2 + z^2/4;
+    classical RK4: 1 + z + z^2/2 + z^3/4)"""
+    lam, dt = -0.7, 0.3
+    z = lam * dt
+    last = {1: 1.0, 2: 1.0 + z, 3: 1.0 + z / 2 + z * z / 4, 4: 1.0 + z + z * z / 2 + z ** 3 / 4}[scheme]
+    q0 = np.array([1.0, -2.0, 0.5, 3.0])
+    _, rn = oracle.rk_generic(scheme, dt, q0, lambda q: lam * q, with_norm=True)
+    assert rn == pytest.approx(abs(lam) * abs(last) * np.linalg.norm(q0), rel=1e-14)
