@@ -37,6 +37,7 @@ EXPORTS = {
     "amrgpu_setup": (_i, [_vp, _i, _i, _i, _vp, _i, _pi, _pi, _i, _pi, _pd, _pd]),
     "amrgpu_upload_soln": (_i, [_vp, _pd]),
     "amrgpu_download_soln": (_i, [_vp, _pd]),
+    "amrgpu_download_quads": (_i, [_vp, _i, _i, _pd]),
     "amrgpu_rhs": (_i, [_vp, _pd]),
     "amrgpu_cfl": (_i, [_vp, _pd]),
     "amrgpu_run": (_i, [_vp, _i, _i, _pd]),
@@ -52,6 +53,7 @@ EXPORTS = {
     "amrgpu_get_tile": (_i, [_vp, _pi, _pi, _pi]),
     "amrgpu_tile_variants": (_i, [_i, _pi, _pi, _pi, _i]),
     "amrgpu_autotune": (_i, [_vp, _i, _pi, _pi, _pi, _pi, _pd]),
+    "amrgpu_feature_tags": (_i, [_vp, _i, _pi, _pd, _pi]),
     # include/amr_forest.h
     "amr_forest_new": (_vp, [_i, _i, _i, _i, _d, _d, _d, _i, _i, _i]),
     "amr_forest_destroy": (None, [_vp]),
@@ -121,6 +123,7 @@ EXPORTS = {
     "amr_mlp_predict_ctx": (_i, [_vp, _vp, _pi]),
     "amr_tagger_collect": (_i, [_vp, _vp, _vp, _pi]),
     "amr_mlp_tagger": (_i, [_vp, _vp, _vp, _pi]),
+    "amr_feature_tagger": (_i, [_vp, _vp, _vp, _pi]),
 }
 
 MLP_NPARAM = 64 * 384 + 64 + 64 * 64 + 64 + 3 * 64 + 3
@@ -156,6 +159,14 @@ class MlpTaggerStruct(C.Structure):
     """amr_mlp_tagger_t"""
     _fields_ = [("mlp", _vp), ("tol", _d), ("union_plain", _i), ("predicted", C.c_longlong * 3),
                 ("plain_refine", C.c_longlong), ("added_by_plain", C.c_longlong)]
+
+
+MAX_FEATURE_TAGS = 8
+
+
+class FeatureTaggerStruct(C.Structure):
+    """amr_feature_tagger_t"""
+    _fields_ = [("ntags", _i), ("methods", _i * MAX_FEATURE_TAGS), ("tols", _d * MAX_FEATURE_TAGS)]
 
 
 class EvolveOpts(C.Structure):
@@ -402,13 +413,15 @@ class Forest:
         """device-resident evolve loop; tagger: optional TAGGER_FN or MlpTagger"""
         o = EvolveOpts()
         o.tol, o.allow_coarsening, o.regrid_interval = tol, int(allow_coarsening), regrid_interval
-        if isinstance(tagger, MlpTagger):
-            o.tagger = C.cast(self.L.amr_mlp_tagger, _vp)
+        if isinstance(tagger, (MlpTagger, FeatureTagger)):
+            o.tagger = C.cast(tagger.fn(self.L), _vp)
             o.tagger_ctx = C.cast(C.pointer(tagger.s), _vp)
         else:
             o.tagger = C.cast(tagger, _vp) if tagger is not None else None
             o.tagger_ctx = tagger_ctx
-        if tile:
+        if tile == "auto":
+            o.tile_ppb = -1          # amrgpu_autotune on the run's own mesh
+        elif tile:
             o.tile_ppb, o.tile_minw, o.tile_design = tile
         if inst is not None:
             o.inst = C.pointer(inst)
@@ -440,16 +453,44 @@ def evolve_batch(forests, insts, tol, regrid_interval, allow_coarsening=True, ti
     ia = (InstParams * k)(*insts)
     o = EvolveOpts()
     o.tol, o.allow_coarsening, o.regrid_interval = tol, int(allow_coarsening), regrid_interval
-    if isinstance(tagger, MlpTagger):
-        o.tagger = C.cast(L.amr_mlp_tagger, _vp)
+    if isinstance(tagger, (MlpTagger, FeatureTagger)):
+        o.tagger = C.cast(tagger.fn(L), _vp)
         o.tagger_ctx = C.cast(C.pointer(tagger.s), _vp)
-    if tile:
+    if tile == "auto":
+        o.tile_ppb = -1
+    elif tile:
         o.tile_ppb, o.tile_minw, o.tile_design = tile
     o.regrid_lag = regrid_lag
     per = (EvolveStats * k)()
     tot = EvolveStats()
     check(L.amr_evolve_batch(fa, ia, k, C.byref(o), per, C.byref(tot)), "amr_evolve_batch")
     return list(per), tot
+
+
+def batch_external(forests):
+    """the reference-format arrays of several forests concatenated as one
+    batched mesh (forest k's quadrants after those of forests 0..k-1, face
+    cell ids shifted alike, soln offsets global) and each quadrant's
+    instance: (cell_info, face_info, geom, soln, cell_inst) for Gpu.setup"""
+    cis, fis, gs, ss, inst = [], [], [], [], []
+    off = 0
+    n = forests[0].n
+    for k, f in enumerate(forests):
+        if f.n != n:
+            raise ValueError("every forest of a batch has the same patch size")
+        ci, fi, geom, soln = f.external()
+        ci = ci.reshape(-1, 3).copy()
+        nq = ci.shape[0]
+        ci[:, 2] = 4 * n * n * (off + np.arange(nq, dtype=np.int64))
+        fi = fi.reshape(-1, 11).copy()
+        fi[:, 2] += off
+        ids = fi[:, 3:7]
+        fi[:, 3:7] = np.where(ids >= 0, ids + off, ids)
+        cis.append(ci.reshape(-1)); fis.append(fi.reshape(-1)); gs.append(geom); ss.append(soln)
+        inst.append(np.full(nq, k, np.int32))
+        off += nq
+    return (np.concatenate(cis).astype(np.int32), np.concatenate(fis).astype(np.int32), np.concatenate(gs),
+            np.concatenate(ss), np.concatenate(inst))
 
 
 def vortex_ic(strength=5.0, xo=5.0, yo=5.0, u=1.0, v=1.0, gamma=1.4, rho=1.0, p=1.0):
@@ -577,6 +618,15 @@ class Gpu:
     def indicator(self):
         out = np.empty(self.ndof // 4 // self.n2)
         check(self.L.amrgpu_indicator(self.h, dpt(out)), "amrgpu_indicator")
+        return out
+
+    def feature_tags(self, methods, tols):
+        """amrgpu_feature_tags: the reference's density-threshold tags
+        (method 1 below, 2 above) per quadrant of the current state"""
+        m = np.ascontiguousarray(methods, dtype=np.int32)
+        t = np.ascontiguousarray(tols, dtype=np.float64)
+        out = np.empty(self.ndof // 4 // self.n2, np.int32)
+        check(self.L.amrgpu_feature_tags(self.h, m.size, ipt(m), dpt(t), ipt(out)), "amrgpu_feature_tags")
         return out
 
     def remesh(self, cell_info, face_info, src, cell_inst=None):
@@ -854,6 +904,28 @@ class MlpTagger:
         self.s.tol = tol
         self.s.union_plain = int(union_plain)
 
+    @staticmethod
+    def fn(L):
+        return L.amr_mlp_tagger
+
     def stats(self):
         return {"predicted": list(self.s.predicted), "plain_refine": self.s.plain_refine,
                 "added_by_plain": self.s.added_by_plain}
+
+
+class FeatureTagger:
+    """amr_feature_tagger_t for evolve_device / evolve_batch(tagger=...): the
+    reference's density-threshold tags, methods[k] in {1: below, 2: above}"""
+
+    def __init__(self, methods, tols):
+        if len(methods) != len(tols) or len(methods) > MAX_FEATURE_TAGS:
+            raise ValueError("methods and tols: equal lengths, at most 8")
+        self.s = FeatureTaggerStruct()
+        self.s.ntags = len(methods)
+        for k, (m, t) in enumerate(zip(methods, tols)):
+            self.s.methods[k] = int(m)
+            self.s.tols[k] = float(t)
+
+    @staticmethod
+    def fn(L):
+        return L.amr_feature_tagger

@@ -22,15 +22,15 @@ from . import (Forest, Gpu, Mlp, MlpTagger, TAG_REFINE, Tagset, dmr_ic, dmr_inst
 ROOT_LEN = 1 << 30
 
 
-def riemann_params(t_end, regrid=10):
-    set_params(nelem_subgrid=8, time_scheme=3, time_steps=10 ** 9, dt=1.0, cfl=0.25 * 0.4 * 3,
+def riemann_params(t_end, regrid=10, n=8):
+    set_params(nelem_subgrid=n, time_scheme=3, time_steps=10 ** 9, dt=1.0, cfl=0.25 * 0.4 * 3,
                coarse_grid_h=1.0, regrid_interval=regrid, checkpoint_interval=2 ** 31 - 1,
                visualization_interval=2 ** 31 - 1, bc=(0, 0, 0, 0, 1, 1), t_end=t_end, verbose=0)
 
 
-def riemann_forest(seed, lmin, lmax, tol=0.1):
+def riemann_forest(seed, lmin, lmax, tol=0.1, n=8):
     ic = riemann4_ic(seed)
-    f = Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, 8, lmin, lmax)
+    f = Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, n, lmin, lmax)
     f.apply_ic(ic)
     f.regrid_initial(ic, tol)
     return f
@@ -124,11 +124,12 @@ def recall_at_equal_fraction(mlp, ts, tol=0.1, biases=np.arange(0.0, 4.01, 0.25)
 
 def density_on_grid(f, level):
     """piecewise-constant density of the forest's patches sampled on the
-    uniform grid of `level` quadrants (8x8 cells each): (N, ntx * N) array,
-    N = 8 * 2**level, for a brick of ntx x 1 trees"""
-    N = 8 * (1 << level)
+    uniform grid of `level` quadrants (n x n cells each, n the forest's patch
+    size): (N, ntx * N) array, N = n * 2**level, for a brick of ntx x 1 trees"""
+    n = f.n
+    N = n * (1 << level)
     out = np.zeros((N, f.ntx * N))
-    q = f.soln()                         # (nquad, 8, 8, 4) [quad][j][i][field]
+    q = f.soln()                         # (nquad, n, n, 4) [quad][j][i][field]
     quads = f.quads()
     lv = np.array([qq[1] for qq in quads])
     xs = np.array([qq[2] for qq in quads], np.int64)
@@ -139,25 +140,25 @@ def density_on_grid(f, level):
         idx = np.nonzero(lv == L)[0]
         r = 1 << (level - L)             # fine cells per patch cell
         trees = np.array([quads[i][0] for i in idx], np.int64)
-        x0 = (xs[idx] >> (30 - level)) * 8 + trees * N
-        y0 = (ys[idx] >> (30 - level)) * 8
-        rho = q[idx, :, :, 0]            # (k, 8, 8) [j][i]
-        blk = np.repeat(np.repeat(rho, r, axis=1), r, axis=2)   # (k, 8r, 8r)
+        x0 = (xs[idx] >> (30 - level)) * n + trees * N
+        y0 = (ys[idx] >> (30 - level)) * n
+        rho = q[idx, :, :, 0]            # (k, n, n) [j][i]
+        blk = np.repeat(np.repeat(rho, r, axis=1), r, axis=2)   # (k, nr, nr)
         for k in range(idx.size):
-            out[y0[k]:y0[k] + 8 * r, x0[k]:x0[k] + 8 * r] = blk[k]
+            out[y0[k]:y0[k] + n * r, x0[k]:x0[k] + n * r] = blk[k]
     return out
 
 
-def uniform_reference(seed, level, t_end, log=None):
+def uniform_reference(seed, level, t_end, log=None, n=8):
     """plain-method FP64 solution on the uniform forest of the finest level"""
-    riemann_params(t_end, regrid=2 ** 31 - 1)
+    riemann_params(t_end, regrid=2 ** 31 - 1, n=n)
     ic = riemann4_ic(seed)
-    f = Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, 8, level, level)
+    f = Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, n, level, level)
     f.apply_ic(ic)
     t0 = time.perf_counter()
     st = f.evolve_device(0.1, 0)
     if log:
-        log(f"uniform level-{level} reference: {f.nquad * 64} cells, {st.steps} steps, "
+        log(f"uniform level-{level} reference: {f.nquad * n * n} cells, {st.steps} steps, "
             f"{time.perf_counter() - t0:.1f} s")
     return density_on_grid(f, level), st
 
@@ -168,33 +169,33 @@ def l1_error(rho, ref):
 
 
 # ---- double-Mach reflection (config 5): [0,4] x [0,1], four unit trees ----
-def dmr_params(t_end, regrid=10, verbose=0):
+def dmr_params(t_end, regrid=10, verbose=0, n=8):
     """verbose = 1: the device loop prints a progress line every 10 s"""
-    set_params(nelem_subgrid=8, time_scheme=3, time_steps=10 ** 9, dt=1.0, cfl=0.25 * 0.4 * 3,
+    set_params(nelem_subgrid=n, time_scheme=3, time_steps=10 ** 9, dt=1.0, cfl=0.25 * 0.4 * 3,
                coarse_grid_h=1.0, regrid_interval=regrid, checkpoint_interval=2 ** 31 - 1,
                visualization_interval=2 ** 31 - 1, bc=(0, 0, 0, 0, 1, 1), t_end=t_end, verbose=verbose)
     p = get_params()
     return dmr_inst(1.0, p.cfl, p.dt, t_end)
 
 
-def dmr_forest(lmin, lmax, tol=0.1):
+def dmr_forest(lmin, lmax, tol=0.1, n=8):
     ic = dmr_ic()
-    f = Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, 8, lmin, lmax)
+    f = Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, n, lmin, lmax)
     f.apply_ic(ic)
     f.regrid_initial(ic, tol)
     return f
 
 
-def dmr_uniform_reference(level, t_end, log=None):
+def dmr_uniform_reference(level, t_end, log=None, n=8):
     """the plain solution on the uniform forest of `level`, advanced on one
     device context in chunks of 16 steps (the instance freezes at t_end, as
     in the evolve loop), logging progress every 30 s"""
-    inst = dmr_params(t_end, regrid=2 ** 31 - 1)
-    f = Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, 8, level, level)
+    inst = dmr_params(t_end, regrid=2 ** 31 - 1, n=n)
+    f = Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, n, level, level)
     f.apply_ic(dmr_ic())
     ci, fi, geom, soln = f.external()
     g = Gpu()
-    g.setup(8, 3, [inst], ci, fi, geom, soln)
+    g.setup(n, 3, [inst], ci, fi, geom, soln)
     t0 = last = time.perf_counter()
     steps = 0
     while True:
@@ -209,15 +210,15 @@ def dmr_uniform_reference(level, t_end, log=None):
     f.set_soln(np.asarray(g.soln()).reshape(-1))
     g.close()
     if log:
-        log(f"uniform level-{level} double-Mach reference: {f.nquad * 64} cells, {steps} steps, "
+        log(f"uniform level-{level} double-Mach reference: {f.nquad * n * n} cells, {steps} steps, "
             f"{time.perf_counter() - t0:.1f} s")
     return density_on_grid(f, level), steps
 
 
-def dmr_collect(lmin, lmax, t_end, tol=0.1, window=40, label_stride=5, tagset=None, log=None):
+def dmr_collect(lmin, lmax, t_end, tol=0.1, window=40, label_stride=5, tagset=None, log=None, n=8):
     ts = tagset if tagset is not None else Tagset()
-    inst = dmr_params(t_end)
-    f = dmr_forest(lmin, lmax, tol)
+    inst = dmr_params(t_end, n=n)
+    f = dmr_forest(lmin, lmax, tol, n=n)
     t0 = time.perf_counter()
     rows = f.collect(ts, tol, window=window, label_stride=label_stride, inst=inst)
     if log:
@@ -225,15 +226,19 @@ def dmr_collect(lmin, lmax, t_end, tol=0.1, window=40, label_stride=5, tagset=No
     return ts
 
 
-def dmr_deploy(lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None, lag=0, verbose=0):
-    inst = dmr_params(t_end, regrid=interval, verbose=verbose)
-    f = dmr_forest(lmin, lmax, tol)
+def dmr_deploy(lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None, lag=0, verbose=0,
+               n=8, tile=None):
+    """ref_level: level of the sampling grid in quadrants of this run's patch
+    size (the reference's grid has n * 2**ref_level cells per tree side)"""
+    inst = dmr_params(t_end, regrid=interval, verbose=verbose, n=n)
+    f = dmr_forest(lmin, lmax, tol, n=n)
     t0 = time.perf_counter()
-    st = f.evolve_device(tol, interval, tagger=tagger, inst=inst, regrid_lag=lag)
+    st = f.evolve_device(tol, interval, tagger=tagger, inst=inst, regrid_lag=lag, tile=tile)
     wall = time.perf_counter() - t0
     out = {
-        "regrid_interval": interval, "regrid_lag": lag, "steps": st.steps, "regrids": st.regrids, "t": st.t,
-        "peak_cells": st.peak_quads * 64, "final_cells": f.nquad * 64, "cell_stages": st.cell_stages,
+        "patch": n, "regrid_interval": interval, "regrid_lag": lag, "steps": st.steps, "regrids": st.regrids,
+        "t": st.t, "peak_cells": st.peak_quads * n * n, "final_cells": f.nquad * n * n,
+        "cell_stages": st.cell_stages,
         "wall_s": wall, "gpu_s": st.wall_gpu, "regrid_s": st.wall_regrid,
         "end_to_end_Mcell_upd_s": st.cell_stages / wall / 1e6,
     }
@@ -244,18 +249,18 @@ def dmr_deploy(lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_
     return out, f
 
 
-def deploy(seed, lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None, lag=0):
+def deploy(seed, lmin, lmax, t_end, interval, tagger=None, tol=0.1, ref=None, ref_level=None, lag=0, n=8):
     """one adaptive run on the device loop; end-to-end rate and L1 error;
     lag > 0 overlaps the host regrid with `lag` steps on the old mesh"""
-    riemann_params(t_end, regrid=interval)
-    f = riemann_forest(seed, lmin, lmax, tol)
+    riemann_params(t_end, regrid=interval, n=n)
+    f = riemann_forest(seed, lmin, lmax, tol, n=n)
     t0 = time.perf_counter()
     st = f.evolve_device(tol, interval, tagger=tagger, regrid_lag=lag)
     wall = time.perf_counter() - t0
     out = {
-        "regrid_interval": interval, "regrid_lag": lag,
+        "regrid_interval": interval, "regrid_lag": lag, "tol": tol,
         "steps": st.steps, "regrids": st.regrids, "t": st.t,
-        "peak_cells": st.peak_quads * 64, "final_cells": f.nquad * 64,
+        "peak_cells": st.peak_quads * n * n, "final_cells": f.nquad * n * n,
         "cell_stages": st.cell_stages, "wall_s": wall, "gpu_s": st.wall_gpu, "regrid_s": st.wall_regrid,
         "regrid_split_s": {"tags": st.wall_tags, "host_forest": st.wall_forest, "device_remesh": st.wall_remesh},
         "end_to_end_Mcell_upd_s": st.cell_stages / wall / 1e6,

@@ -483,6 +483,30 @@ int amr_device_regrid(amr_forest_t *f, amrgpu_ctx *g, const int *tags, int allow
     return rc;
 }
 
+/* tile_ppb < 0 in the options: the stage-kernel tile and launch mode are
+ * chosen by amrgpu_autotune on the loop's own mesh, at the start and again
+ * whenever the mesh has doubled since the last choice (the trial steps are
+ * undone, so the run is the one it would be with that tile set by hand).
+ * *tuned_at holds the cell count of the last choice (0: none yet). */
+static int autotune_tile(amrgpu_ctx *g, const amr_evolve_opts_t *o, int nquad, int *tuned_at)
+{
+    if (!o || o->tile_ppb >= 0) return AMRGPU_OK;
+    if (*tuned_at > 0 && nquad < 2 * *tuned_at) return AMRGPU_OK;
+    *tuned_at = nquad;
+    return amrgpu_autotune(g, 5, NULL, NULL, NULL, NULL, NULL);
+}
+
+int amr_feature_tagger(void *user, amrgpu_ctx *ctx, amr_forest_t *f, int *tags)
+{
+    (void)f;
+    const amr_feature_tagger_t *u = (const amr_feature_tagger_t *)user;
+    if (!u) {
+        amr_set_error("amr_feature_tagger: no settings");
+        return AMRGPU_ERR_ARG;
+    }
+    return amrgpu_feature_tags(ctx, u->ntags, u->methods, u->tols, tags);
+}
+
 int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_stats_t *st)
 {
     physics_gpu_params_t p;
@@ -498,6 +522,8 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
     rc = amr_device_setup(f, o ? o->inst : NULL, &g);
     if (rc) return rc;
     if (o && o->tile_ppb > 0) rc = amrgpu_set_tile(g, o->tile_ppb, o->tile_minw, o->tile_design);
+    int tuned_at = 0;
+    if (!rc) rc = autotune_tile(g, o, ncell, &tuned_at);
     double *ind = NULL;
     int *tags = NULL;
     int cap = 0, istep = 0;
@@ -598,6 +624,7 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *o, amr_evolve_st
         ncell = amr_forest_num_quads(f);
         s.regrids++;
         if (ncell > s.peak_quads) s.peak_quads = ncell;
+        if (!rc) rc = autotune_tile(g, o, ncell, &tuned_at);
         s.wall_regrid += now_s() - t2;
         if (istep >= p.time_steps || (p.t_end > 0.0 && t >= p.t_end)) break;   /* stop reached while lagging */
     }
@@ -701,11 +728,14 @@ static int batch_build(const ext_t *ext, int ninst, int n, int *base, batch_bufs
 
 /* Concatenated mesh tables of all forests from each forest's own table
  * tab[k] (quadrants of instance k at base[k] .. base[k+1]-1, neighbour ids
- * shifted by the same offsets), with the instance of every quadrant. */
-static int batch_build_tab(const tab_t *tab, int ninst, int *base, batch_bufs_t *b, int *ncell_out)
+ * shifted by the same offsets), with the instance of every quadrant; a
+ * dropped instance contributes no quadrants. */
+static int batch_build_tab(const tab_t *tab, const int *dropped, int ninst, int *base, batch_bufs_t *b,
+                           int *ncell_out)
 {
     size_t nc = 0;
     for (int k = 0; k < ninst; ++k) {
+        if (dropped[k]) continue;
         if (tab[k].rc) return AMRGPU_ERR_MESH;
         nc += (size_t)tab[k].nq;
     }
@@ -727,7 +757,7 @@ static int batch_build_tab(const tab_t *tab, int ninst, int *base, batch_bufs_t 
     }
     nc = 0;
     for (int k = 0; k < ninst; ++k) {
-        const int nq = tab[k].nq, off = (int)nc;
+        const int nq = dropped[k] ? 0 : tab[k].nq, off = (int)nc;
         base[k] = off;
         for (size_t e = 0; e < 8 * (size_t)nq; ++e) {
             const int v = tab[k].nbr[e];
@@ -744,9 +774,9 @@ static int batch_build_tab(const tab_t *tab, int ninst, int *base, batch_bufs_t 
 }
 
 /* Per-instance regrid work of one batch regrid, shared by the host threads:
- * instance k (taken in turn from a counter) is regridded by its tags when it
- * has not reached its stop, and its mesh table is rebuilt when it changed
- * (or was never built). */
+ * instance k (taken in turn from a counter) is regridded by its tags and its
+ * mesh table rebuilt when it has not reached its stop; one that has leaves
+ * the device mesh at this regrid and needs neither. */
 typedef struct {
     amr_forest_t **f;
     const int *done, *tags, *obase;
@@ -769,15 +799,12 @@ static void *regrid_worker(void *arg)
         pthread_mutex_unlock(&j->lock);
         if (k >= j->ninst) break;
         j->ok[k] = 1;
-        if (!j->done[k]) {
-            amr_forest_track_begin(j->f[k], 0);
-            const int r = amr_regrid_tags(j->f[k], j->tags + j->obase[k], j->allow);
-            j->ok[k] = amr_forest_track_end(j->f[k]) && r >= 0;
-        }
-        if (!j->done[k] || !j->tab[k].nbr || j->tab[k].rc) {
-            tab_free(&j->tab[k]);
-            tab_of(j->f[k], &j->tab[k]);
-        }
+        if (j->done[k]) continue;      /* leaves the device mesh at this regrid */
+        amr_forest_track_begin(j->f[k], 0);
+        const int r = amr_regrid_tags(j->f[k], j->tags + j->obase[k], j->allow);
+        j->ok[k] = amr_forest_track_end(j->f[k]) && r >= 0;
+        tab_free(&j->tab[k]);
+        tab_of(j->f[k], &j->tab[k]);
     }
     amr_forest_host_parallel(was_parallel);
     return NULL;
@@ -838,6 +865,7 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
     tab_t *tab = (tab_t *)calloc((size_t)ninst, sizeof(tab_t));
     int *ok = (int *)calloc((size_t)ninst, sizeof(int));
     int *obase = (int *)malloc(sizeof(int) * (size_t)(ninst + 1));
+    int *dropped = (int *)calloc((size_t)ninst, sizeof(int));   /* left the device mesh */
     batch_bufs_t b;
     memset(&b, 0, sizeof(b));
     double *ind = NULL;
@@ -845,7 +873,7 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
     int ncell = 0, nface = 0, rc = 0, istep = 0, cap = 0;
     amrgpu_ctx *g = NULL;
     double t0 = now_s();
-    if (!st || !base || !done || !t || !ext || !tab || !ok || !obase) rc = AMRGPU_ERR_ARG;
+    if (!st || !base || !done || !t || !ext || !tab || !ok || !obase || !dropped) rc = AMRGPU_ERR_ARG;
     for (int k = 0; k < ninst && !rc; ++k) rc = ext_of(f[k], &ext[k]);
     if (!rc) rc = batch_build(ext, ninst, n, base, &b, &ncell, &nface);
     if (!rc) {
@@ -861,6 +889,8 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
         b.cap_c = b.cap_f = 0;
     }
     if (!rc && o && o->tile_ppb > 0) rc = amrgpu_set_tile(g, o->tile_ppb, o->tile_minw, o->tile_design);
+    int tuned_at = 0;
+    if (!rc) rc = autotune_tile(g, o, ncell, &tuned_at);
     for (int k = 0; k < ninst && !rc; ++k) st[k].peak_quads = base[k + 1] - base[k];
     tot.peak_quads = ncell;
     tot.wall_regrid += now_s() - t0;
@@ -952,16 +982,22 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
             if (!ok[k]) rc = AMRGPU_ERR_MESH;
             if (!done[k]) st[k].regrids++;
         }
-        if (!rc) rc = batch_build_tab(tab, ninst, base, &b, &ncell);
+        /* instances that reached their stop leave the device mesh: their
+         * final state goes to their own forest now, and no later launch
+         * carries their cells */
         for (int k = 0; k < ninst && !rc; ++k) {
+            if (!done[k] || dropped[k]) continue;
+            rc = amrgpu_download_quads(g, obase[k], obase[k + 1] - obase[k], amr_forest_soln(f[k]));
+            st[k].t = t[k];
+            dropped[k] = 1;
+        }
+        if (!rc) rc = batch_build_tab(tab, dropped, ninst, base, &b, &ncell);
+        for (int k = 0; k < ninst && !rc; ++k) {
+            if (dropped[k]) continue;
             const int nk = base[k + 1] - base[k];
-            if (done[k]) {
-                for (int q = 0; q < nk; ++q) b.tsrc[base[k] + q] = obase[k] + q;
-            } else {
-                const int *s = amr_forest_track_src(f[k]);
-                for (int q = 0; q < nk; ++q)
-                    b.tsrc[base[k] + q] = s[q] >= 0 ? obase[k] + s[q] : -(obase[k] + (-s[q] - 1)) - 1;
-            }
+            const int *s = amr_forest_track_src(f[k]);
+            for (int q = 0; q < nk; ++q)
+                b.tsrc[base[k] + q] = s[q] >= 0 ? obase[k] + s[q] : -(obase[k] + (-s[q] - 1)) - 1;
             if (nk > st[k].peak_quads) st[k].peak_quads = nk;
         }
         double t4 = now_s();
@@ -969,6 +1005,7 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
         if (!rc) rc = amrgpu_remesh_nbr(g, ncell, b.lev, b.tinst, b.nbr, b.tsrc);
         if (!rc) rc = amrgpu_set_origins(g, b.org);
         tot.wall_remesh += now_s() - t4;
+        if (!rc) rc = autotune_tile(g, o, ncell, &tuned_at);
         tot.regrids++;
         if (ncell > tot.peak_quads) tot.peak_quads = ncell;
         tot.wall_regrid += now_s() - t2 - t_gpu_lag;
@@ -987,6 +1024,7 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
         if (!all) rc = AMRGPU_ERR_ARG;
         if (!rc) rc = amrgpu_download_soln(g, all);
         for (int k = 0; k < ninst && !rc; ++k) {
+            if (dropped[k]) continue;          /* written when it left the mesh */
             memcpy(amr_forest_soln(f[k]), all + 4 * (size_t)n2 * base[k],
                    sizeof(double) * 4 * (size_t)n2 * (size_t)(base[k + 1] - base[k]));
             st[k].t = t[k];
@@ -1000,5 +1038,6 @@ int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, con
     if (tab)
         for (int k = 0; k < ninst; ++k) tab_free(&tab[k]);
     free(st); free(base); free(done); free(t); free(ind); free(tags); free(ext); free(tab); free(ok); free(obase);
+    free(dropped);
     return rc;
 }

@@ -1162,6 +1162,56 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK)) void indicator_kernel(MeshDev 
     if (tid < PPB && (int)blockIdx.x * PPB + tid < m.nquad) ind[blockIdx.x * PPB + tid] = bits2d(pmax[tid]);
 }
 
+// The reference's feature tags (solver_tag_feature.F90:9-52 with
+// tag_density_below / tag_density_above, tagging_methods_module.F90:14-71):
+// a quadrant is tagged when, for some method k, one of its sub-cell
+// densities is <= tol[k] (method 1) or >= tol[k] (method 2).  "Some density
+// <= tol" is "the patch minimum <= tol" (fmin skips NaN, and a NaN density
+// satisfies neither comparison), so one min / max reduction per patch gives
+// the reference's flag exactly.  Methods 3 and 4 (vorticity, Q criterion) are
+// commented out in the reference and tag nothing, as any other value.
+struct FeatureTagArgs {
+    int ntags;
+    int method[AMRGPU_MAX_FEATURE_TAGS];
+    double tol[AMRGPU_MAX_FEATURE_TAGS];
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void feature_tag_kernel(int nquad, const double *U, FeatureTagArgs t,
+                                                          int *tags)
+{
+    constexpr int N2 = N * N;
+    constexpr int PPB = N2 >= 256 ? 1 : 256 / N2;
+    constexpr int W = N2 < 64 ? N2 : 64;            // lanes of one patch within a wave
+    __shared__ double smin[4], smax[4];
+    const int tid = threadIdx.x;
+    const int p = tid / N2, c = tid % N2;
+    const int quad = blockIdx.x * PPB + p;
+    const bool active = p < PPB && quad < nquad;
+    const double r = active ? U[(size_t)quad * NFLD * N2 + c] : NAN;
+    double mn = r, mx = r;
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) {           // segments of W aligned lanes
+        mn = fmin(mn, __shfl_xor(mn, o, 64));
+        mx = fmax(mx, __shfl_xor(mx, o, 64));
+    }
+    if constexpr (N2 > 64) {
+        static_assert(N2 == 256, "one patch of four waves");
+        if ((tid & 63) == 0) { smin[tid >> 6] = mn; smax[tid >> 6] = mx; }
+        __syncthreads();
+        mn = fmin(fmin(smin[0], smin[1]), fmin(smin[2], smin[3]));
+        mx = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+    }
+    if (active && c == 0) {
+        int tag = 0;
+        for (int k = 0; k < t.ntags; ++k) {
+            if (t.method[k] == 1 && mn <= t.tol[k]) tag = 1;
+            if (t.method[k] == 2 && mx >= t.tol[k]) tag = 1;
+        }
+        tags[quad] = tag;
+    }
+}
+
 // New state after a regrid, one thread per new sub-cell: src >= 0 copies
 // old quadrant src (solver_refine_operator_2d.F90:49-52 copies the parent);
 // src < 0 averages the old family -src-1.. (solver_coarsen_operator_2d.F90:
@@ -1584,6 +1634,14 @@ void amrgpu_destroy(amrgpu_ctx *c)
     delete c;
 }
 
+int amrgpu_staging(amrgpu_ctx *c, double **buf, size_t *ndoubles)
+{
+    if (!c || !c->d_stage) { set_err("amrgpu_staging: no mesh"); return AMRGPU_ERR_ARG; }
+    *buf = c->d_stage;
+    *ndoubles = c->cap_state;
+    return AMRGPU_OK;
+}
+
 int amrgpu_num_cells(const amrgpu_ctx *c) { return c ? c->nquad * c->n * c->n : 0; }
 int amrgpu_num_quads(const amrgpu_ctx *c) { return c ? c->nquad : 0; }
 
@@ -1802,6 +1860,34 @@ int amrgpu_indicator(amrgpu_ctx *c, double *ind)
     return AMRGPU_OK;
 }
 
+int amrgpu_feature_tags(amrgpu_ctx *c, int ntags, const int *methods, const double *tols, int *tags)
+{
+    if (!c || !tags || !c->nquad || ntags < 0 || ntags > AMRGPU_MAX_FEATURE_TAGS ||
+        (ntags > 0 && (!methods || !tols))) {
+        set_err("amrgpu_feature_tags: bad arguments");
+        return AMRGPU_ERR_ARG;
+    }
+    FeatureTagArgs t;
+    memset(&t, 0, sizeof(t));
+    t.ntags = ntags;
+    for (int k = 0; k < ntags; ++k) { t.method[k] = methods[k]; t.tol[k] = tols[k]; }
+    const int n2 = c->n * c->n, ppb = n2 >= 256 ? 1 : 256 / n2;
+    const int nblk = (c->nquad + ppb - 1) / ppb;
+    int *d_tags = c->d_map;      // regrid source map: free between remeshes
+    const double *U = c->U[c->cur];
+    switch (c->n) {
+    case 2: hipLaunchKernelGGL((feature_tag_kernel<2>), dim3(nblk), dim3(256), 0, c->stream, c->nquad, U, t, d_tags); break;
+    case 4: hipLaunchKernelGGL((feature_tag_kernel<4>), dim3(nblk), dim3(256), 0, c->stream, c->nquad, U, t, d_tags); break;
+    case 8: hipLaunchKernelGGL((feature_tag_kernel<8>), dim3(nblk), dim3(256), 0, c->stream, c->nquad, U, t, d_tags); break;
+    case 16: hipLaunchKernelGGL((feature_tag_kernel<16>), dim3(nblk), dim3(256), 0, c->stream, c->nquad, U, t, d_tags); break;
+    default: set_err("unsupported nelem_subgrid %d", c->n); return AMRGPU_ERR_ARG;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(tags, d_tags, sizeof(int) * c->nquad, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMRGPU_OK;
+}
+
 // The source map of a remesh refers to the old mesh only.
 static int check_src(int ncell, const int *src, int nold, const char *who)
 {
@@ -1870,7 +1956,30 @@ static int remesh_to(amrgpu_ctx *c, int ncell, const int *level, const int *cell
         (void)hipFree(map);
         c->cur = 0;
     }
-    return upload_mesh(c, ncell, level, cell_inst, nbr);
+    rc = upload_mesh(c, ncell, level, cell_inst, nbr);
+    if (rc) return rc;
+    // an instance left without quadrants (a batch drops those that reached
+    // their stop) no longer advances: no wave publishes its dt any more, so
+    // its last dt is cleared here and the time bookkeeping of the stages
+    // keeps its time as it is
+    if (cell_inst && c->ninst > 1) {
+        std::vector<char> has(c->ninst, 0);
+        for (int q = 0; q < ncell; ++q) has[cell_inst[q]] = 1;
+        bool empty = false;
+        for (int i = 0; i < c->ninst; ++i) empty = empty || !has[i];
+        if (empty) {
+            std::vector<double> dt(c->ninst);
+            HIPCHK(hipMemcpyAsync(dt.data(), c->d_dt_cur, sizeof(double) * c->ninst, hipMemcpyDeviceToHost,
+                                  c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int i = 0; i < c->ninst; ++i)
+                if (!has[i]) dt[i] = 0.0;
+            HIPCHK(hipMemcpyAsync(c->d_dt_cur, dt.data(), sizeof(double) * c->ninst, hipMemcpyHostToDevice,
+                                  c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+    }
+    return AMRGPU_OK;
 }
 
 int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst, int nface,
@@ -1969,6 +2078,24 @@ int amrgpu_download_soln(amrgpu_ctx *c, double *soln)
     const long long ncells = (long long)c->nquad * n2;
     hipLaunchKernelGGL(soa_to_aos_kernel, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, c->stream,
                        c->U[c->cur], c->d_stage, ncells, n2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(soln, c->d_stage, sizeof(double) * 4 * ncells, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMRGPU_OK;
+}
+
+int amrgpu_download_quads(amrgpu_ctx *c, int first, int count, double *soln)
+{
+    if (!c || !soln || first < 0 || count < 0 || first + (long long)count > c->nquad) {
+        set_err("amrgpu_download_quads: range [%d, %d + %d) outside the %d quadrants", first, first, count,
+                c ? c->nquad : 0);
+        return AMRGPU_ERR_ARG;
+    }
+    if (count == 0) return AMRGPU_OK;
+    const int n2 = c->n * c->n;
+    const long long ncells = (long long)count * n2;
+    hipLaunchKernelGGL(soa_to_aos_kernel, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, c->stream,
+                       c->U[c->cur] + (size_t)first * 4 * n2, c->d_stage, ncells, n2);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(soln, c->d_stage, sizeof(double) * 4 * ncells, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -5,6 +5,8 @@
 #ifndef AMR_INTERNAL_H
 #define AMR_INTERNAL_H
 
+#include <stddef.h>
+
 #include "amr_hip.h"
 
 #ifdef __cplusplus
@@ -29,6 +31,10 @@ AMR_HIDDEN int amr_device_setup(struct amr_forest *f, const amrgpu_inst_t *overr
 AMR_HIDDEN int amr_device_regrid(struct amr_forest *f, struct amrgpu_ctx *g, const int *tags,
                                  int allow_coarsening, double *t_forest, double *t_remesh);
 AMR_HIDDEN double amr_now(void);
+/* The context's staging buffer (device, *ndoubles doubles: one state vector
+ * of the current capacity), free between uploads and downloads; kernels on
+ * the context's stream may use it as scratch. */
+AMR_HIDDEN int amrgpu_staging(struct amrgpu_ctx *c, double **buf, size_t *ndoubles);
 /* Per-quadrant loops of the host forest run on OpenMP threads unless turned
  * off for the calling thread (the batch's per-instance regrids do).  Returns
  * the thread's previous setting, which the caller restores. */

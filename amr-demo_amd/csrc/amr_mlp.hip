@@ -118,6 +118,27 @@ __global__ __launch_bounds__(256) void features_kernel(const double *__restrict_
     for (int f = 0; f < 6; ++f) o[f * 64] = f2bf((float)feat[f]);
 }
 
+// 16x16 patches: the tagger reads each patch coarsened to 8x8, every coarse
+// cell the mean of its 2x2 block, ((a + b) + c) + d times 0.25 with a..d in
+// z-order (the summation of solver_coarsen_operator_2d.F90:49-50 applied
+// within a patch).  The neighbours are coarsened alike and keep their ids, so
+// features_kernel then runs unchanged on the coarsened state.
+__global__ __launch_bounds__(256) void pool16_kernel(const double *__restrict__ U, double *__restrict__ P,
+                                                     int nquad)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;     // one coarse cell
+    if (t >= (long long)nquad * 64) return;
+    const long long q = t >> 6;
+    const int c = (int)(t & 63), I = c & 7, J = c >> 3;
+    const double *s = U + q * 1024 + (2 * J) * 16 + 2 * I;
+    double *d = P + q * 256 + c;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const double *sf = s + f * 256;
+        d[f * 64] = 0.25 * (((sf[0] + sf[1]) + sf[16]) + sf[17]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // shared pieces of the network kernels
 // ---------------------------------------------------------------------------
@@ -803,8 +824,24 @@ int amrgpu_tagger_features(amrgpu_ctx *c, uint16_t *d_X)
     void *st;
     int rc = amrgpu_device_state(c, &U, &nbr, &nquad, &n, &st);
     if (rc) return rc;
-    if (n != 8) { amr_set_error("tagger features need 8x8 patches (got %d)", n); return AMRGPU_ERR_ARG; }
+    if (n != 8 && n != 16) {
+        amr_set_error("tagger features need 8x8 or 16x16 patches (got %d)", n);
+        return AMRGPU_ERR_ARG;
+    }
     if (!d_X) { amr_set_error("amrgpu_tagger_features: null output"); return AMRGPU_ERR_ARG; }
+    if (n == 16) {
+        // the features of the patches coarsened to 8x8, in the context's
+        // staging buffer (a quarter of one state vector)
+        double *P;
+        size_t cap;
+        rc = amrgpu_staging(c, &P, &cap);
+        if (rc) return rc;
+        if (cap < (size_t)nquad * 256) { amr_set_error("amrgpu_tagger_features: staging too small"); return AMRGPU_ERR_ARG; }
+        hipLaunchKernelGGL(pool16_kernel, dim3((unsigned)(((long long)nquad * 64 + 255) / 256)), dim3(256), 0,
+                           (hipStream_t)st, U, P, nquad);
+        MCHK(hipGetLastError());
+        U = P;
+    }
     hipLaunchKernelGGL(features_kernel, dim3((nquad + 3) / 4), dim3(256), 0, (hipStream_t)st, U, nbr, nquad, d_X);
     MCHK(hipGetLastError());
     return AMRGPU_OK;

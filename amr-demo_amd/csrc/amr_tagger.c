@@ -59,8 +59,8 @@ int amr_tagger_collect(amr_forest_t *f, const amr_collect_opts_t *o, amr_tagset_
         amr_set_error("amr_tagger_collect: window must be a positive multiple of label_stride");
         return AMRGPU_ERR_ARG;
     }
-    if (amr_forest_nelem_subgrid(f) != 8) {
-        amr_set_error("amr_tagger_collect: 8x8 patches required");
+    if (amr_forest_nelem_subgrid(f) != 8 && amr_forest_nelem_subgrid(f) != 16) {
+        amr_set_error("amr_tagger_collect: 8x8 or 16x16 patches required");
         return AMRGPU_ERR_ARG;
     }
     physics_gpu_params_t p;

@@ -117,7 +117,9 @@ typedef struct {
     int    regrid_interval;      /* steps between regrids (<= 0: never) */
     amr_tagger_fn tagger;        /* NULL: device density-gradient indicator */
     void  *tagger_ctx;
-    int    tile_ppb, tile_minw, tile_design;   /* stage-kernel tile (ppb <= 0: default) */
+    int    tile_ppb, tile_minw, tile_design;   /* stage-kernel tile (ppb == 0: default; ppb < 0:
+                                                  amrgpu_autotune on the run's mesh at the start
+                                                  and whenever the mesh has doubled since) */
     const amrgpu_inst_t *inst;   /* NULL: instance from the physics parameters; else these
                                     settings (e.g. the double-Mach walls and states) */
     int    regrid_lag;           /* 0: regrid at the tagging step (the reference's order);
@@ -125,6 +127,20 @@ typedef struct {
                                     a thread while the GPU advances k more steps on the old mesh,
                                     then the state k steps later is remapped (tags k steps old) */
 } amr_evolve_opts_t;
+
+/* The reference's own tag source, as a tagger for amr_evolve_device /
+ * amr_evolve_batch: solver_tag_feature's density thresholds
+ * (src/solver/src/solver_tag_feature.F90:9-52, methods 1 "density below"
+ * and 2 "density above" of src/solver/src/tagging_methods_module.F90:14-71)
+ * evaluated on the device by amrgpu_feature_tags.  Tag 1 refines; untagged
+ * families are coarsened (src/amr/src/amr_tag.c:202-220).  Pass the address
+ * of an amr_feature_tagger_t as tagger_ctx. */
+typedef struct {
+    int    ntags;                               /* <= AMRGPU_MAX_FEATURE_TAGS */
+    int    methods[AMRGPU_MAX_FEATURE_TAGS];
+    double tols[AMRGPU_MAX_FEATURE_TAGS];
+} amr_feature_tagger_t;
+int amr_feature_tagger(void *user, struct amrgpu_ctx *ctx, amr_forest_t *f, int *tags);
 
 /* The evolve loop with the state resident in HBM across regrids: tags are
  * computed on the device, the forest topology is regridded on the host in
@@ -140,8 +156,10 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *opts, amr_evolve
  * steps each instance that has not reached its stop is regridded on its own
  * forest by the device indicator (tracking mode) and the combined state is
  * remapped on the device.  An instance past its stop keeps its state (dt =
- * 0), so each instance evolves exactly as it would alone under
- * amr_evolve_device with the same settings.  All forests must have the same
+ * 0) and leaves the device mesh at the next regrid (its final state is
+ * written to its forest then), so each instance evolves exactly as it would
+ * alone under amr_evolve_device with the same settings and later launches
+ * carry only the cells of instances still advancing.  All forests must have the same
  * patch size; time scheme and step limit come from the physics parameters.
  * per_inst[ninst] (may be NULL) receives each instance's statistics
  * (cell_stages counted while it has not reached its stop at the start of a

@@ -135,13 +135,26 @@ int amrgpu_setup(amrgpu_ctx *c, int nelem_subgrid, int time_scheme,
                  int nface, const int *face_info, const double *geom, const double *soln);
 int amrgpu_upload_soln(amrgpu_ctx *c, const double *soln);
 int amrgpu_download_soln(amrgpu_ctx *c, double *soln);
+/* The state of quadrants [first, first + count) only, in the same layout
+ * (soln holds 4 n^2 count doubles). */
+int amrgpu_download_quads(amrgpu_ctx *c, int first, int count, double *soln);
 /* Project-defined density-gradient indicator of the current state, one
  * value per quadrant (host out); tagging is ind > tolerance. */
 int amrgpu_indicator(amrgpu_ctx *c, double *ind);
+/* The reference's feature tags of the current state, one flag per quadrant
+ * (host out): solver_tag_feature (src/solver/src/solver_tag_feature.F90:9-52)
+ * over ntags methods, tag_density_below (1: some sub-cell density <=
+ * tols[k]) and tag_density_above (2: some sub-cell density >= tols[k]) of
+ * src/solver/src/tagging_methods_module.F90:14-71; tags[q] = 1 if any method
+ * fires, else 0 (other method numbers tag nothing, as in the reference).
+ * 0 <= ntags <= AMRGPU_MAX_FEATURE_TAGS. */
+#define AMRGPU_MAX_FEATURE_TAGS 8
+int amrgpu_feature_tags(amrgpu_ctx *c, int ntags, const int *methods, const double *tols, int *tags);
 /* Replace the mesh after a regrid without leaving the device: src[ncell]
  * from amr_forest_track_src (copy of an old quadrant, or mean of an old
  * family); cell_info/face_info/cell_inst describe the new mesh.  Instance
- * settings and time are kept. */
+ * settings and time are kept; an instance the new mesh leaves without
+ * quadrants stops advancing (its time stays where it is). */
 int amrgpu_remesh(amrgpu_ctx *c, int ncell, const int *cell_info, const int *cell_inst, int nface,
                   const int *face_info, const int *src);
 /* The same remesh given the device mesh table directly (levels, instances

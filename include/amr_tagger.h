@@ -24,6 +24,9 @@
  * index-aligned edge cell (the mean of the two when the side has two finer
  * neighbours), or the cell itself on a domain boundary.  Each feature is
  * computed in FP64 (IEEE divide), rounded to FP32, then to BF16 (nearest even).
+ * A 16x16 patch is read coarsened to 8x8 (each coarse cell the mean of its
+ * 2x2 block, ((a + b) + c) + d times 0.25 with a..d in z-order), its
+ * neighbours coarsened alike: the same 384 features of the coarsened state.
  *
  * Parameters (FP32 master copy, AMR_MLP_NPARAM floats):
  *   W1[64][384], b1[64], W2[64][64], b2[64], W3[3][64], b3[3]
@@ -81,8 +84,8 @@ int amr_tagset_get(const amr_tagset_t *s, int first, int n, uint16_t *X, int *y,
 /* Rows [first, first+n) of `src` appended to `dst` (device to device). */
 int amr_tagset_append_from(amr_tagset_t *dst, const amr_tagset_t *src, int first, int n);
 
-/* Features of every quadrant of the context's current state (8x8 patches
- * only) into X (device pointer, nquad x 384 BF16 bits), on the context's
+/* Features of every quadrant of the context's current state (8x8 or 16x16
+ * patches) into X (device pointer, nquad x 384 BF16 bits), on the context's
  * stream. */
 int amrgpu_tagger_features(struct amrgpu_ctx *c, uint16_t *d_X);
 /* Same, copied to the host (nquad x 384). */

@@ -62,6 +62,30 @@ static double edge_rho(const int *nb, const double *soln, int q, int s, int ec, 
 int orc_tagger_features(int n, int ncell, int nface, const int *face_info, const double *soln,
                         unsigned short *X)
 {
+    if (n == 16) {
+        /* 16x16 patches: the features of the patch coarsened to 8x8, every
+         * coarse cell the mean of its 2x2 block, ((a + b) + c) + d times 0.25
+         * with a..d in z-order (the summation of the reference's coarsen
+         * operator, solver_coarsen_operator_2d.F90:49-50, within one patch);
+         * the neighbours coarsened alike, so the 8x8 definition applies */
+        double *P = (double *)malloc(sizeof(double) * 256 * (size_t)(ncell > 0 ? ncell : 1));
+        if (!P) return -1;
+        for (int q = 0; q < ncell; ++q) {
+            const double *Q = soln + (size_t)1024 * q;
+            for (int J = 0; J < 8; ++J)
+                for (int I = 0; I < 8; ++I)
+                    for (int f = 0; f < 4; ++f) {
+                        const double a = Q[f + 4 * (2 * I + 16 * (2 * J))];
+                        const double b = Q[f + 4 * (2 * I + 1 + 16 * (2 * J))];
+                        const double c = Q[f + 4 * (2 * I + 16 * (2 * J + 1))];
+                        const double d = Q[f + 4 * (2 * I + 1 + 16 * (2 * J + 1))];
+                        P[(size_t)256 * q + f + 4 * (I + 8 * J)] = 0.25 * (((a + b) + c) + d);
+                    }
+        }
+        const int rc = orc_tagger_features(8, ncell, nface, face_info, P, X);
+        free(P);
+        return rc;
+    }
     if (n != 8) return -1;
     /* neighbours per side from the face list: full faces pair the two sides,
      * a hanging face gives the coarse side both fine quadrants (in face order)

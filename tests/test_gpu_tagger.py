@@ -45,6 +45,28 @@ def test_features_bitwise(oracle, seed):
     g.close()
 
 
+def test_features_bitwise_16(oracle):
+    """16x16 patches: the features of the patches coarsened to 8x8, bitwise
+    equal to the oracle (hanging faces, a periodic pair, walls)"""
+    rng = np.random.default_rng(5)
+    f = amrhip.Forest(2, 1, (1, 0), 0.0, 0.0, 0.5, 16, 1, 3)
+    for _ in range(2):
+        f.refine((rng.uniform(size=f.nquad) < 0.35).astype(np.int32))
+        f.balance()
+    s = f.soln()
+    s[..., 0] = rng.uniform(0.2, 2.0, s.shape[:-1])
+    s[..., 1:3] = rng.uniform(-1.0, 1.0, s.shape[:-1] + (2,))
+    s[..., 3] = rng.uniform(1.0, 4.0, s.shape[:-1])
+    ci, fi, geom, soln = f.external()
+    g = amrhip.Gpu()
+    g.setup(16, 3, [amrhip.make_inst(0.5, (0, 0, 0, 0), cfl=0.3, dt_cap=1.0)], ci, fi, geom, soln)
+    got = g.tagger_features()
+    exp = oracle.tagger_features(16, ci, fi, soln)
+    print(f"16x16 features nquad={f.nquad}", flush=True)
+    assert np.array_equal(got, exp)
+    g.close()
+
+
 def torch_fp32_forward(P, X):
     torch = pytest.importorskip("torch")
     x = torch.tensor(bf16_val(X))

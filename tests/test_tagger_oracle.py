@@ -234,3 +234,58 @@ def test_features_hanging_side_uses_both_fine_neighbours(oracle):
         sk = np.abs(q[k, :, :, 0]).max()
         for j in range(8):
             assert X[k, 4, j, 0] == bf16_bits(np.float32((q[k, j, 1, 0] - q[0, j, 7, 0]) / sk))
+
+
+def pool16(soln16):
+    """16x16 patches (reference layout) coarsened to 8x8: ((a + b) + c) + d
+    times 0.25 over each 2x2 block in z-order"""
+    q = np.asarray(soln16).reshape(-1, 16, 16, 4)
+    a, b = q[:, 0::2, 0::2], q[:, 0::2, 1::2]
+    c, d = q[:, 1::2, 0::2], q[:, 1::2, 1::2]
+    return (0.25 * (((a + b) + c) + d)).reshape(-1)
+
+
+def test_features_16_equal_features_of_coarsened_patches(oracle):
+    """a 16x16 forest's features are those of the same forest with 8x8
+    patches holding the 2x2 means, neighbours included (hanging faces and
+    walls)"""
+    rng = np.random.default_rng(16)
+    out = []
+    for n in (16, 8):
+        f = amrhip.Forest(2, 1, (1, 0), 0.0, 0.0, 1.0, n, 1, 3)
+        r = np.random.default_rng(3)
+        for _ in range(2):
+            f.refine((r.uniform(size=f.nquad) < 0.4).astype(np.int32))
+            f.balance()
+        out.append((f, f.external()))
+    (f16, (ci16, fi16, _, s16)), (f8, (ci8, fi8, _, _)) = out
+    assert f16.quads() == f8.quads() and np.array_equal(fi16, fi8)
+    s16[:] = rng.uniform(0.2, 2.0, s16.shape)
+    X16 = oracle.tagger_features(16, ci16, fi16, s16)
+    X8 = oracle.tagger_features(8, ci8, fi8, pool16(s16))
+    assert np.array_equal(X16, X8)
+
+
+def test_features_16_constant_and_linear(oracle):
+    f = amrhip.Forest(1, 1, (0, 0), 0.0, 0.0, 1.0, 16, 1, 1)
+    ci, fi, geom, soln = f.external()
+    q = soln.reshape(4, 16, 16, 4)
+    ii = np.arange(16)[None, :]
+    q[:] = 0.0
+    q[..., 0] = 2.0
+    q[..., 1] = 0.5 * ii                   # pooled: I + 0.25, max 7.25
+    q[..., 2] = -1.0
+    q[..., 3] = 4.0
+    X = oracle.tagger_features(16, ci, fi, soln).reshape(4, 6, 8, 8).astype(np.uint32)
+    one = bf16_bits(np.float32(1.0))
+    assert np.all(X[:, 0] == one) and np.all(X[:, 3] == one)
+    assert np.all(X[:, 2] == bf16_bits(np.float32(-1.0)))
+    exp1 = bf16_bits(((np.arange(8) + 0.25) / 7.25).astype(np.float32))
+    assert np.all(X[:, 1] == exp1[None, None, :])
+    assert np.all(X[:, 4] == 0) and np.all(X[:, 5] == 0)
+    # linear density along y: interior pooled differences are 4 fine steps
+    q[..., 0] = 1.0 + 0.125 * np.arange(16)[:, None]     # exact in binary; pooled 1 + 0.125 (2J + 0.5)
+    X = oracle.tagger_features(16, ci, fi, soln).reshape(4, 6, 8, 8)
+    s0 = 1.0 + 0.125 * 14.5                              # the largest pooled density (J = 7)
+    assert np.all(X[:, 5, 1:7, :] == bf16_bits(np.float32(0.5 / s0)))
+    assert np.all(X[:, 4] == 0)
