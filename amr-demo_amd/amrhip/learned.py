@@ -215,6 +215,61 @@ def dmr_uniform_reference(level, t_end, log=None, n=8):
     return density_on_grid(f, level), steps
 
 
+def dmr_track(level, t_end, every=512, log=None, n=8):
+    """the plain scheme on the uniform double-Mach forest of `level`, with
+    the state inspected every `every` steps: time, minimum density and
+    pressure, the largest flow speed and where it sits, and the speed of the
+    flow behind the Mach stem for comparison.  Stops at t_end or at the
+    first non-finite state (amrgpu_run's NaN check).  Returns the records
+    and whether t_end was reached."""
+    inst = dmr_params(t_end, regrid=2 ** 31 - 1, n=n)
+    f = Forest(4, 1, (0, 0), 0.0, 0.0, 1.0, n, level, level)
+    f.apply_ic(dmr_ic())
+    ci, fi, geom, soln = f.external()
+    g = Gpu()
+    g.setup(n, 3, [inst], ci, fi, geom, soln)
+    N = n << level
+    h = 1.0 / N
+    steps, recs, reached = 0, [], False
+    while True:
+        try:
+            g.run(every)
+        except FloatingPointError:
+            recs.append({"steps": steps + every, "t": None, "nan": True})
+            if log:
+                log(f"  level {level}: non-finite residual within steps {steps}..{steps + every}")
+            break
+        steps += every
+        t = float(g.time()[0][0])
+        f.set_soln(np.asarray(g.soln()).reshape(-1))
+        q = f.soln()
+        rho = q[..., 0]
+        u, v = q[..., 1] / rho, q[..., 2] / rho
+        p = 0.4 * (q[..., 3] - 0.5 * rho * (u * u + v * v))
+        speed = np.hypot(u, v)
+        k = int(np.nanargmax(speed))
+        kq, kj, ki = np.unravel_index(k, speed.shape)
+        tree, lv, x, y = f.quads()[kq]
+        size = 1.0 / (1 << lv)
+        xs = tree + (x / ROOT_LEN) + (ki + 0.5) * size / n
+        ys = (y / ROOT_LEN) + (kj + 0.5) * size / n
+        rec = {"steps": steps, "t": t, "rho_min": float(np.nanmin(rho)), "p_min": float(np.nanmin(p)),
+               "speed_max": float(speed.flat[k]), "at": [round(xs, 4), round(ys, 4)],
+               "rho_at_speed_max": float(rho.flat[k]), "cells": f.nquad * n * n, "h": h}
+        recs.append(rec)
+        if log:
+            log(f"  level {level} step {steps}: t={t:.4f} rho_min={rec['rho_min']:.3f} p_min={rec['p_min']:.3f} "
+                f"|u|max={rec['speed_max']:.2f} at ({xs:.3f}, {ys:.4f}) rho there {rec['rho_at_speed_max']:.3f}")
+        if not np.isfinite(rho).all():
+            rec["nan"] = True
+            break
+        if t >= t_end:
+            reached = True
+            break
+    g.close()
+    return recs, reached
+
+
 def dmr_collect(lmin, lmax, t_end, tol=0.1, window=40, label_stride=5, tagset=None, log=None, n=8):
     ts = tagset if tagset is not None else Tagset()
     inst = dmr_params(t_end, n=n)

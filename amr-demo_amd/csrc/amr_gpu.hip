@@ -500,6 +500,22 @@ __device__ __forceinline__ void load_prim(const double *lds, int p, int t, Prim 
     P.H = (P.rE + P.p) * ir;     // E + p/rho
 }
 
+// The same record oriented along a face normal chosen per lane: offn / offt
+// are the LDS array offsets of the normal and tangential momenta.
+template <int N, int PPB>
+__device__ __forceinline__ void load_oprim(const double *lds, int p, int t, int offn, int offt, OPrim &P)
+{
+    using TL = Tile<N, PPB>;
+    const double *b = lds + p * TL::PRIM_PER_PATCH + t;
+    P.r = b[0]; P.rn = b[offn]; P.rt = b[offt]; P.rE = b[3 * TL::NPOS];
+    P.p = b[4 * TL::NPOS];
+    const double ir = b[5 * TL::NPOS];
+    P.sr = b[6 * TL::NPOS];
+    P.un = P.rn * ir;
+    P.ut = P.rt * ir;
+    P.H = (P.rE + P.p) * ir;
+}
+
 template <int N, int PPB, int MINW, int DESIGN>
 __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(StageArgs a)
 {
@@ -728,7 +744,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         // load of its own record (two independent flux chains per thread);
         // one more round covers the -x and -y patch edges
         double FA[4] = {0, 0, 0, 0}, FB[4] = {0, 0, 0, 0}, FC[4] = {0, 0, 0, 0};
-        int dC = -1;
+        int dC = -1, fnC = 1;     // FC: normal-frame flux; fnC its normal-momentum field
         if (active) {
             const double h = UNI ? hw : ph[p];
             const int na1 = pnbr[p][2], nb1 = pnbr[p][3], na3 = pnbr[p][6], nb3 = pnbr[p][7];
@@ -785,24 +801,32 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 const double hf = nb >= 0 ? 0.5 * h : h;
                 const int tl = TL::GHOST + side * N + k;   // ghost of side 0 or 2
                 const int tr = xs ? k * N : k;             // first cell of row / column k
-                Prim L, R;
-                load_prim<N, PPB>(lds, tp, tl, L);
-                load_prim<N, PPB>(lds, tp, tr, R);
-                flux_face_dir(L, R, hf, xs, FC);
+                // x and y faces mixed in one wave: the records are read
+                // oriented along each lane's normal by choosing the LDS
+                // arrays (the normal momentum is rho u on x faces, rho v on
+                // y faces), and the flux is stored back the same way, in
+                // place of selecting every value
+                const int offn = xs ? TL::NPOS : 2 * TL::NPOS, offt = 3 * TL::NPOS - offn;
+                OPrim L, R;
+                load_oprim<N, PPB>(lds, tp, tl, offn, offt, L);
+                load_oprim<N, PPB>(lds, tp, tr, offn, offt, R);
+                flux_normal(L, R, hf, FC);
                 if (na < 0) {
                     const double nrm[2] = {xs ? -h : 0.0, xs ? 0.0 : -h};
-                    const double q[4] = {R.r, R.ru, R.rv, R.rE};
+                    const double q[4] = {R.r, xs ? R.rn : R.rt, xs ? R.rt : R.rn, R.rE};
+                    double F[4];
                     flux_boundary(a.m.ip[pinst[tp]], side, nrm, q, a.m.org + 2 * (size_t)pquad[tp],
-                                  xs ? 0.0 : k + 0.5, xs ? k + 0.5 : 0.0, h, ptime[tp], FC);
-                    FC[0] = -FC[0]; FC[1] = -FC[1]; FC[2] = -FC[2]; FC[3] = -FC[3];
+                                  xs ? 0.0 : k + 0.5, xs ? k + 0.5 : 0.0, h, ptime[tp], F);
+                    FC[0] = -F[0]; FC[1] = -(xs ? F[1] : F[2]); FC[2] = -(xs ? F[2] : F[1]); FC[3] = -F[3];
                 } else if (nb >= 0) {
-                    Prim F2;
-                    load_prim<N, PPB>(lds, tp, TL::SECOND + side * N + k, F2);
+                    OPrim F2;
+                    load_oprim<N, PPB>(lds, tp, TL::SECOND + side * N + k, offn, offt, F2);
                     double G[4];
-                    flux_face_dir(F2, R, hf, xs, G);
+                    flux_normal(F2, R, hf, G);
                     FC[0] += G[0]; FC[1] += G[1]; FC[2] += G[2]; FC[3] += G[3];
                 }
                 dC = tp * TL::FLUX_PER_PATCH + (xs ? k * (N + 1) : NFLD * N * (N + 1) + k);
+                fnC = xs ? 1 : 2;
             }
         }
         if (active && needB) {   // base state: loaded late, its latency overlaps the flux gather
@@ -821,8 +845,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             }
         }
         if (dC >= 0) {
-#pragma unroll
-            for (int f = 0; f < 4; ++f) lds[dC + f * stride] = FC[f];
+            lds[dC] = FC[0];
+            lds[dC + fnC * stride] = FC[1];
+            lds[dC + (3 - fnC) * stride] = FC[2];
+            lds[dC + 3 * stride] = FC[3];
         }
         __syncthreads();
     } else {
@@ -2261,9 +2287,18 @@ int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_
             cur = step_next(c, cur);
         }
     }
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    // every event is destroyed on every exit path
+    struct Events {
+        std::vector<hipEvent_t> e;
+        ~Events() { for (auto &x : e) if (x) (void)hipEventDestroy(x); }
+    } evs;
+    std::vector<StageArgs> st;
+    build_step(c, c->cur, st, false);
+    const int ns = (int)st.size();
+    evs.e.assign(2 + (size_t)nsteps * (ns + 1), nullptr);
+    for (auto &e : evs.e) HIPCHK(hipEventCreate(&e));
+    hipEvent_t e0 = evs.e[0], e1 = evs.e[1];
+    hipEvent_t *ev = evs.e.data() + 2;
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipEventRecord(e0, c->stream));
     for (int s = 0; s < nsteps; ++s) {
@@ -2277,11 +2312,6 @@ int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_
     if (ms_total) *ms_total = ms;
 
     // per-stage kernel durations: direct launches bracketed by events
-    std::vector<StageArgs> st;
-    build_step(c, c->cur, st, false);
-    const int ns = (int)st.size();
-    std::vector<hipEvent_t> ev((size_t)nsteps * (ns + 1));
-    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     for (int s = 0; s < nsteps; ++s) {
         int nx = build_step(c, c->cur, st, false);
         if (c->scheme == 1) {
@@ -2307,9 +2337,6 @@ int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_
         if (avg_stage_ms) avg_stage_ms[k] = tot / nsteps;
     }
     if (nstages) *nstages = ns;
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     // one more step carrying the residual norm: NaN check of the result
     rc = one_step(c, false, true);
     if (rc) return rc;

@@ -1,48 +1,55 @@
 #!/usr/bin/env python3
 """Benchmark of the FP64 finite-volume Euler hot path on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): the 2D
-four-quadrant Riemann problem on [0,1]^2, states from
-numpy.random.default_rng(seed) (rho log-uniform [0.1,2], u,v uniform [-1,1],
-p uniform [0.1,2]), 8x8 FP64 sub-grid patches on an adaptive forest with
-levels 3-7 refined by the density-gradient indicator, regrid every 10 steps,
-2:1 balance, transmissive boundaries, SSP-RK3 with input CFL 0.4.
+Headline workload (--workload batch, the default): 64 independent
+four-quadrant Riemann problems on [0,1]^2 (seeds 1000..1063, states from
+numpy.random.default_rng(seed): rho log-uniform [0.1,2], u,v uniform [-1,1],
+p uniform [0.1,2]), each with its own finest level drawn from {5,6,7,8}
+(numpy.random.default_rng(2024)), refined by the density-gradient indicator
+every 10 steps with 2:1 balance, transmissive boundaries, SSP-RK3 with input
+CFL 0.4, FP64.
 
-Setup (untimed): the adaptive problem is run on the GPU path through the C
-evolve loop with the state resident in HBM (device indicator and remesh, host
-forest topology) to t = --t-end; the mesh and state reached there are the
-benchmark input, resident in HBM.  The end-to-end rate of that adaptive run is
-reported beside the headline: the run is made twice, and adaptive_run gives
-the second (steady state) with the first, which pays the runtime's one-time
-set-up, beside it (with --host-loop also the interval-by-interval path
-through hpc_amr_evolve_solver).
+Setup (untimed): the 64 problems are advanced together to t = --batch-t-end
+in one device context (amr_evolve_batch: one launch per RK stage over every
+instance that has not reached its stop, instances leaving the device mesh at
+the regrid after their stop), once with 8x8 and once with 16x16 patches
+(16x16 one level coarser at both ends: the same finest sub-cell size), the
+stage-kernel tile autotuned on the run's own mesh.  The patch size with the
+shorter time to t_end is kept ("autotuner" in the output, both runs beside
+it); the 64 meshes and states it reached are the benchmark input, resident
+in HBM (about 6-10 M cells, far larger than the 256 MB Infinity Cache).
 
 A step = one SSP-RK3 step (3 fused residual + stage-update launches, CFL
-folded into the last) over that mesh.  value = sub-grid cells x 3 stages x
-steps / wall time of the timed region (HIP events; the stages launched
-directly from the stream or replayed from a captured graph, whichever the
+folded into the last) over all 64 instances, each with its own dt.
+value = sub-grid cells x 3 stages x steps / wall time of the timed region
+(HIP events on the launch stream; direct launches or graph replay, as the
 autotuner measured faster).
 
 roofline: the fused stage kernel; algorithmic bytes = 85.33 B per cell-stage
-(stage 1 reads U^n, writes U^1; stages 2-3 read U^s and U^n, write U^s+1;
-32 B per state), achieved = bytes per launch / average launch duration, the
-latter = the timed region (HIP events on the launch stream) / launches.
-traffic = FETCH_SIZE + WRITE_SIZE per launch from the committed rocprofv3
-summary of the same mesh (profiles/stage_kernel_<workload>_traffic.txt).
+(stage 1 reads U^n and writes U^1; stages 2-3 read U^s and U^n and write
+U^s+1; 32 B per state), achieved = bytes per launch / average launch
+duration = the timed region / launches.  traffic = FETCH_SIZE (scaled by the
+copy kernel's reading of a known size) + WRITE_SIZE per launch from the
+committed rocprofv3 summary of the same mesh
+(profiles/stage_kernel_<workload>_traffic.txt), when its cell count matches.
 
-learned: the learned-tagger comparison (amrhip.learned): snapshots from
-seeded runs, BF16-MFMA training on the GPU, held-out recall, and plain vs
-learned deployment on a finer seeded run with L1 density errors against the
-uniform finest-level plain solution (--no-learned skips it).
+cpu_baseline: the C oracle (restatement of the reference CPU path: AoS
+patches, face loop + element loop, k-form SSP-RK3) on the host, one worker per
+core on contiguous quadrant ranges (as many as the GPU's share of the host
+allows: at most 16), on a bounded number of steps of the same 64-instance mesh
+(one dt for all: the cost per cell-stage is the same); its -O2 and
+-O3 -march=native builds both reported, the faster as `value`.
 
-cpu_baseline: the CPU oracle (C restatement of the reference CPU path,
-AoS patches, face loop + element loop, k-form SSP-RK3) on the host cores,
-one worker per core on contiguous quadrant ranges, on a bounded number of
-steps of the same mesh; the faster of its -O2 and -O3 -march=native builds.
+Secondary lines: the single adaptive Riemann problem (levels 3-7, 8x8, its
+207k-cell mesh at t = 0.25: cache-resident, so a throughput figure only),
+the learned-tagger comparison, the batched-mode comparison (one by one vs
+batched, plain vs learned, with L1 errors) and the double-Mach reflection.
+--workload riemann4 / vortex put those single meshes in the headline instead.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -54,12 +61,95 @@ sys.path.insert(0, os.path.join(ROOT, "amr-demo_amd"))
 METRIC = "Mcell-updates/s (RK stage) on 1 GPU, %HBM peak; plain vs learned+batched at equal L1"
 HBM_PEAK_GBS = 8000.0
 B_ALG_STAGE = (64.0 + 96.0 + 96.0) / 3.0     # bytes per cell-stage, SSP-RK3 Shu-Osher
+CFL_EFF = 0.25 * 0.4 * 3                       # input CFL 0.4, SSP-RK3 (physics_initialize.c:193)
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+# ---------------------------------------------------------------------------
+# the 64-instance batch (headline)
+# ---------------------------------------------------------------------------
+def batch_problem(k):
+    rng = np.random.default_rng(2024)
+    lmaxs = [int(x) for x in rng.choice([5, 6, 7, 8], size=k)]
+    return [1000 + i for i in range(k)], lmaxs
+
+
+def batch_forests(L, seeds, lmaxs, lmin, tol, n):
+    """16x16 patches one level coarser at both ends (the same finest
+    sub-cell size as 8x8 at lmax)"""
+    d = {8: 0, 16: 1}[n]
+    return [L.riemann_forest(s, lmin - d, lm - d, tol, n=n) for s, lm in zip(seeds, lmaxs)]
+
+
+def run_batch(amrhip, forests, insts, tol, interval, n, tagger=None, lag=0):
+    t0 = time.perf_counter()
+    per, tot = amrhip.evolve_batch(forests, insts, tol, interval, tile="auto", tagger=tagger, regrid_lag=lag)
+    wall = time.perf_counter() - t0
+    useful = sum(p.cell_stages for p in per)
+    r = {"patch": n, "wall_s": round(wall, 3), "steps": tot.steps, "regrids": tot.regrids,
+         "peak_cells": tot.peak_quads * n * n, "final_cells": sum(f.nquad for f in forests) * n * n,
+         "gpu_s": round(tot.wall_gpu, 3), "regrid_s": round(tot.wall_regrid, 3),
+         "regrid_split_s": {"tags": round(tot.wall_tags, 3), "host_forest": round(tot.wall_forest, 3),
+                            "device_remesh": round(tot.wall_remesh, 3)},
+         # cells of instances that have not reached their stop (an instance
+         # leaves the launches at the regrid after its stop)
+         "useful_cell_stages": useful, "launched_cell_stages": tot.cell_stages,
+         "end_to_end_Mcell_upd_s": useful / wall / 1e6,
+         "gpu_Mcell_upd_s": useful / tot.wall_gpu / 1e6,
+         "gpu_hbm_frac": useful / tot.wall_gpu * B_ALG_STAGE / (HBM_PEAK_GBS * 1e9)}
+    return r, per
+
+
+def batch_headline(amrhip, args):
+    """runs the 64 problems to t_end with each candidate patch size, keeps the
+    faster; returns (chosen n, the final forests of every run by patch size,
+    the runs, (seeds, finest levels))"""
+    from amrhip import learned as L
+    seeds, lmaxs = batch_problem(args.batch_instances)
+    runs, forests = {}, {}
+    # one small run first: the runtime's one-time set-up stays out of the timings
+    L.riemann_params(0.01, regrid=10)
+    L.riemann_forest(0, 2, 4, args.tol).evolve_device(args.tol, 10)
+    for n in args.batch_patches:
+        L.riemann_params(args.batch_t_end, regrid=10, n=n)
+        p = amrhip.get_params()
+        insts = [amrhip.make_inst(1.0, (0, 0, 0, 0), p.cfl, p.dt, args.batch_t_end) for _ in seeds]
+        fb = batch_forests(L, seeds, lmaxs, args.lmin, args.tol, n)
+        r, _ = run_batch(amrhip, fb, insts, args.tol, 10, n)
+        runs[n], forests[n] = r, fb
+        log(f"batch of {len(seeds)}, {n}x{n} patches: {r['wall_s']:.2f} s to t={args.batch_t_end}, "
+            f"{r['steps']} steps, {r['regrids']} regrids, peak {r['peak_cells']} cells, final {r['final_cells']}, "
+            f"{r['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s end to end, {r['gpu_Mcell_upd_s']:.0f} over GPU time")
+    best = min(runs, key=lambda k: runs[k]["wall_s"])
+    return best, forests, runs, (seeds, lmaxs)
+
+
+def save_forests(forests, d):
+    os.makedirs(d, exist_ok=True)
+    for k, f in enumerate(forests):
+        f.save(os.path.join(d, f"inst_{k:03d}.bin"))
+
+
+def load_forests(amrhip, d, k):
+    return [amrhip.Forest.load(os.path.join(d, f"inst_{i:03d}.bin"))[0] for i in range(k)]
+
+
+# ---------------------------------------------------------------------------
+# single adaptive Riemann problem (the 207k-cell secondary line)
+# ---------------------------------------------------------------------------
 def build_riemann_mesh(amrhip, seed, lmin, lmax, n, tol, t_end, regrid, device=True):
     """run the adaptive Riemann problem on the GPU path to t_end; return the
     forest (mesh + state) and run statistics.  device=True keeps the state in
@@ -70,7 +160,7 @@ def build_riemann_mesh(amrhip, seed, lmin, lmax, n, tol, t_end, regrid, device=T
     f.apply_ic(ic)
     f.regrid_initial(ic, tol)
     amrhip.set_params(nelem_subgrid=n, time_scheme=3, time_steps=10 ** 9, dt=1.0,
-                      cfl=0.25 * 0.4 * 3, coarse_grid_h=1.0, regrid_interval=regrid,
+                      cfl=CFL_EFF, coarse_grid_h=1.0, regrid_interval=regrid,
                       checkpoint_interval=2 ** 31 - 1, visualization_interval=2 ** 31 - 1,
                       bc=(0, 0, 0, 0, 1, 1), t_end=t_end, verbose=0)
     t0 = time.perf_counter()
@@ -79,31 +169,44 @@ def build_riemann_mesh(amrhip, seed, lmin, lmax, n, tol, t_end, regrid, device=T
     return f, st, wall
 
 
-def cpu_baseline(ci, fi, geom, soln, n, seconds, threads, h_base):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    best = None
-    for path, label in ((pyoracle.ORACLE_SO, "-O2"), (pyoracle.ORACLE_NATIVE_SO, "-O3 -march=native")):
-        if not os.path.exists(path):
-            continue
-        orc = pyoracle.Oracle(path)
-        p = orc.params(n, bc=(0, 0, 0, 0), cfl=0.3, dt=1.0, h_base=h_base, scheme=3)
-        t0 = time.perf_counter()
-        orc.run(p, ci, fi, geom, soln, 1, threads=threads)
-        one = max(time.perf_counter() - t0, 1e-4)
-        k = max(1, int(seconds / 2 / one))
-        t0 = time.perf_counter()
-        orc.run(p, ci, fi, geom, soln, k, threads=threads)
-        dt = time.perf_counter() - t0
-        cells = soln.size // 4
-        v = cells * 3 * k / dt / 1e6
-        log(f"cpu baseline {label}: {k} steps, {dt:.2f} s, {v:.2f} Mcell-updates/s on {threads} threads")
-        if best is None or v > best[0]:
-            best = (v, label, k, dt)
-    return best
+def time_steps(amrhip, n, insts, arrays, steps, warmup, tile=None, autotune=True):
+    """Gpu context on the given mesh: autotuned tile, warm-up, timed steps"""
+    ci, fi, geom, soln, cinst = arrays
+    g = amrhip.Gpu()
+    g.setup(n, 3, insts, ci, fi, geom, soln, cinst)
+    tune = None
+    if tile:
+        g.set_tile(*tile)
+    elif autotune:
+        tune = g.autotune(10)
+        log("autotune (ppb, minw, design, ms/step): " +
+            ", ".join(f"({a},{b},{c},{d:.4f})" for a, b, c, d in tune))
+    if warmup:
+        g.run(warmup)
+    ms, stage_ms = g.bench(steps)
+    return g, tune, ms, stage_ms
 
 
-def measured_traffic(workload, cells, tile):
+def roofline_of(cells, ms, steps, n, tile, launch, stage_ms):
+    avg_stage = ms / (3 * steps)
+    bytes_launch = cells * B_ALG_STAGE
+    achieved = bytes_launch / (avg_stage * 1e-3) / 1e9
+    return {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": f"stage_kernel<{n},{tile[0]},{tile[1]},{tile[2]}>",
+        "bytes_per_launch": int(bytes_launch),
+        "avg_launch_us": round(avg_stage * 1e3, 3),
+        "avg_launch_from": f"timed region / launches (HIP events on the launch stream, {launch})",
+        "stage_launch_us_bracketed": [round(s * 1e3, 3) for s in stage_ms],
+    }
+
+
+def measured_traffic(workload, cells):
     """HBM bytes per stage-kernel launch from the committed rocprofv3 --pmc
     summary of this workload (FETCH_SIZE and WRITE_SIZE per dispatch, scaled
     by the copy kernel's reading of a known size), when it was taken on the
@@ -124,15 +227,105 @@ def measured_traffic(workload, cells, tile):
         return None
 
 
+def cpu_baseline(ci, fi, geom, soln, n, seconds, threads, h_base):
+    """the C oracle on the host: each build timed on a bounded number of steps"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    lines = {}
+    for path, label in ((pyoracle.ORACLE_SO, "-O2"), (pyoracle.ORACLE_NATIVE_SO, "-O3 -march=native")):
+        if not os.path.exists(path):
+            continue
+        orc = pyoracle.Oracle(path)
+        p = orc.params(n, bc=(0, 0, 0, 0), cfl=CFL_EFF, dt=1.0, h_base=h_base, scheme=3)
+        t0 = time.perf_counter()
+        orc.run(p, ci, fi, geom, soln, 1, threads=threads)
+        one = max(time.perf_counter() - t0, 1e-4)
+        k = max(1, int(seconds / 2 / one))
+        t0 = time.perf_counter()
+        orc.run(p, ci, fi, geom, soln, k, threads=threads)
+        dt = time.perf_counter() - t0
+        cells = soln.size // 4
+        v = cells * 3 * k / dt / 1e6
+        log(f"cpu baseline {label}: {k} steps, {dt:.2f} s, {v:.2f} Mcell-updates/s on {threads} threads")
+        lines[label] = {"value": round(v, 3), "steps": k, "seconds": round(dt, 2)}
+    return lines
+
+
+# ---------------------------------------------------------------------------
+# secondary experiments
+# ---------------------------------------------------------------------------
+def riemann_line(amrhip, args):
+    """the single adaptive Riemann problem (levels 3-7, 8x8) to t = 0.25 on
+    the state-resident loop, then timed steps on the ~207k-cell mesh reached
+    (cache-resident: a throughput figure, not an HBM one)"""
+    f_cold, st_cold, wall_cold = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, 8, args.tol,
+                                                    args.t_end, 10)
+    f, st, wall = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, 8, args.tol, args.t_end, 10)
+    if (st.steps, st.regrids, f.nquad) != (st_cold.steps, st_cold.regrids, f_cold.nquad):
+        raise RuntimeError("two identical adaptive runs differ")
+    ci, fi, geom, soln = f.external()
+    cells = f.nquad * 64
+    g, tune, ms, stage_ms = time_steps(amrhip, 8, [amrhip.make_inst(1.0, (0, 0, 0, 0), CFL_EFF, 1.0, 0.0)],
+                                       (ci, fi, geom, soln, None), args.steps, args.warmup)
+    tile = g.tile()
+    launch = "graph replay" if g.graphs() else "direct stream launches"
+    g.close()
+    value = cells * 3 * args.steps / (ms * 1e-3) / 1e6
+    rl = roofline_of(cells, ms, args.steps, 8, tile, launch, stage_ms)
+    rl["note"] = "the mesh (~6.6 MB per state vector) sits in the 256 MB Infinity Cache: not an HBM figure"
+    log(f"riemann4 207k mesh: {value:.0f} Mcell-updates/s, {ms / args.steps * 1e3:.1f} us/step")
+    return {
+        "workload": f"riemann4 seed {args.seed}, adaptive levels {args.lmin}-{args.lmax}, 8x8 FP64 patches, "
+                    f"mesh at t={st.t:.3f}",
+        "cells": cells, "value_Mcell_upd_s": round(value, 3), "ms_per_step": round(ms / args.steps, 6),
+        "roofline": rl,
+        "adaptive_run": {
+            "loop": "state resident in HBM, device indicator + remesh, host topology",
+            "steps": st.steps, "regrids": st.regrids, "t": round(st.t, 6),
+            "cell_stages": st.cell_stages, "wall_s": round(wall, 3),
+            "gpu_s": round(st.wall_gpu, 3), "regrid_s": round(st.wall_regrid, 3),
+            "end_to_end_Mcell_updates_per_s": round(st.cell_stages / wall / 1e6, 3),
+            "first_run_in_process_wall_s": round(wall_cold, 3),
+        },
+    }
+
+
+def equal_l1_plain(L, args, lmax, target, ref, interval=10, lag=0, log_fn=log):
+    """plain runs with the indicator tolerance bisected (in log tol) until
+    their L1 error is within 1 % of `target`; returns the matching run (or
+    the closest one found) and the search trail"""
+    lo, hi = 0.005, 0.5         # small tol refines more (lower L1)
+    trail, best = [], None
+    for _ in range(9):
+        tol = float(np.sqrt(lo * hi))
+        r, _ = L.deploy(args.seed, args.lmin, lmax, args.t_end, interval, None, tol, ref, lmax, lag)
+        trail.append({"tol": round(tol, 5), "l1": r["l1_density"], "wall_s": round(r["wall_s"], 3)})
+        log_fn(f"  equal-L1 search: plain tol {tol:.4f}: L1 {r['l1_density']:.5e} (target {target:.5e}), "
+               f"{r['wall_s']:.2f} s")
+        if best is None or abs(r["l1_density"] - target) < abs(best["l1_density"] - target):
+            best = r
+        if abs(r["l1_density"] - target) <= 0.01 * target:
+            break
+        if r["l1_density"] > target:
+            hi = tol
+        else:
+            lo = tol
+    return best, trail
+
+
 def learned_experiment(amrhip, args):
     """plain indicator every 10 steps vs the learned tagger (with the plain
     indicator as fallback) every --learned-interval steps, on a seeded
-    Riemann run with levels lmin..--learned-lmax: end-to-end cell-updates/s
-    and L1 density error against the plain-method FP64 solution on the
-    uniform finest level.  The network is trained on the GPU from snapshots
-    of other seeds (1..k); recall is measured on two held-out seeds."""
+    Riemann run with levels lmin..--learned-lmax: end-to-end cell-updates/s,
+    wall time and L1 density error against the plain-method FP64 solution on
+    the uniform finest level.  The network is trained on the GPU from
+    snapshots of other seeds (1..k); recall is measured on two held-out
+    seeds.  Plain every 40 runs with the learned run's regrid lag, so the
+    network's own contribution is isolated; a plain run with its tolerance
+    tuned to the learned run's L1 (within 1 %) gives the wall-time-to-solution
+    comparison at equal accuracy."""
     from amrhip import learned as L
-    lmax, iv = args.learned_lmax, args.learned_interval
+    lmax, iv, lag = args.learned_lmax, args.learned_interval, args.learned_lag
     t0 = time.perf_counter()
     ts = L.collect(range(1, 1 + args.learned_seeds), lmin=args.lmin, lmax=args.lmax, t_end=args.t_end,
                    tol=args.tol, window=iv, log=log)
@@ -148,17 +341,18 @@ def learned_experiment(amrhip, args):
         f"vs indicator-now {rec['recall_indicator_now']:.3f}")
     ref, _ = L.uniform_reference(args.seed, lmax, args.t_end, log=log)
     runs = {}
-    for name, interval, tagger, lag in (("plain_every_10", 10, None, 0), (f"plain_every_{iv}", iv, None, 0),
-                                        (f"learned_every_{iv}", iv, amrhip.MlpTagger(m, args.tol, True),
-                                         args.learned_lag)):
-        r, _ = L.deploy(args.seed, args.lmin, lmax, args.t_end, interval, tagger, args.tol, ref, lmax, lag)
+    for name, interval, tagger, lg in (("plain_every_10", 10, None, 0),
+                                       (f"plain_every_{iv}_lag_{lag}", iv, None, lag),
+                                       (f"learned_every_{iv}_lag_{lag}", iv,
+                                        amrhip.MlpTagger(m, args.tol, True), lag)):
+        r, _ = L.deploy(args.seed, args.lmin, lmax, args.t_end, interval, tagger, args.tol, ref, lmax, lg)
         runs[name] = r
         log(f"{name}: {r['steps']} steps, {r['regrids']} regrids, peak {r['peak_cells']} cells, "
             f"{r['wall_s']:.2f} s, {r['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s end to end, "
             f"L1 {r['l1_density']:.5e}")
     base = runs["plain_every_10"]
-    lr = runs[f"learned_every_{iv}"]
-    return {
+    lr = runs[f"learned_every_{iv}_lag_{lag}"]
+    out = {
         "deployed_workload": f"riemann4 seed {args.seed}, levels {args.lmin}-{lmax}, 8x8 FP64, t={args.t_end}",
         "reference": f"plain FP64 on the uniform level-{lmax} forest ({ref.shape[0]}^2 cells)",
         "training": {"rows": ts.size, "label_counts": L.class_counts(ts.get(features=False)[1]),
@@ -169,71 +363,93 @@ def learned_experiment(amrhip, args):
         "runs": runs,
         "speedup_end_to_end": lr["end_to_end_Mcell_upd_s"] / base["end_to_end_Mcell_upd_s"],
         "l1_ratio": lr["l1_density"] / base["l1_density"],
-    }, m
+    }
+    if not args.no_equal_l1:
+        eq, trail = equal_l1_plain(L, args, lmax, lr["l1_density"], ref)
+        out["plain_every_10_at_learned_l1"] = {"run": eq, "search": trail,
+                                               "l1_within_1pct": abs(eq["l1_density"] / lr["l1_density"] - 1)
+                                               <= 0.01,
+                                               "wall_time_ratio_plain_over_learned": eq["wall_s"] / lr["wall_s"]}
+        log(f"equal L1: plain every 10 at tol {eq['tol']:.4f}: L1 {eq['l1_density']:.5e}, {eq['wall_s']:.2f} s "
+            f"vs learned {lr['wall_s']:.2f} s ({eq['wall_s'] / lr['wall_s']:.2f}x)")
+    return out, m
 
 
 def dmr_experiment(amrhip, args, model):
-    """config 5 at reduced depth: the Mach-10 double-Mach reflection on
-    [0,4] x [0,1] (levels 2..--dmr-lmax, 8x8 FP64 patches, to --dmr-t-end),
-    plain indicator every 10 steps vs the learned tagger (fine-tuned on
-    double-Mach snapshots of a run one level coarser) every
-    --learned-interval steps; L1 density error against the plain solution on
-    the uniform finest level"""
+    """config 5 at the deepest depth the plain scheme carries to t = 0.2: the
+    Mach-10 double-Mach reflection on [0,4] x [0,1] with 8x8 patches (levels
+    2..--dmr-lmax) and with 16x16 patches (one level coarser at both ends: the
+    same finest cells), plain indicator every 10 steps vs the learned tagger
+    (fine-tuned on double-Mach snapshots of a run one level coarser, 8x8 then
+    16x16) every --learned-interval steps; L1 density error against the plain
+    solution on the uniform finest level (the same grid for both patch
+    sizes)"""
     from amrhip import learned as L
     lmax, iv, t_end = args.dmr_lmax, args.learned_interval, args.dmr_t_end
-    out = {"workload": f"double-Mach reflection, Mach 10 at 60 deg, [0,4]x[0,1], levels 2-{lmax}, "
-                       f"8x8 FP64 patches, t={t_end}"}
-    if model is not None:
-        t0 = time.perf_counter()
-        ts = L.dmr_collect(2, lmax - 1, t_end, tol=args.tol, window=iv, log=log)
-        losses = model.train(ts, epochs=10, batch=4096, lr=1e-3, cw=(1.0, 1.0, 2.0), seed=7)
-        out["fine_tune"] = {"rows": ts.size, "epochs": 10, "loss_first_last": [float(losses[0]), float(losses[-1])],
-                            "wall_s": round(time.perf_counter() - t0, 2)}
-        ts.close()
+    out = {"workload": f"double-Mach reflection, Mach 10 at 60 deg, [0,4]x[0,1], finest level {lmax} of 8x8 "
+                       f"patches (16x16: level {lmax - 1}), FP64, t={t_end}"}
     ref, _ = L.dmr_uniform_reference(lmax, t_end, log=log)
-    runs = [("plain_every_10", 10, None)]
-    if model is not None:
-        runs.append((f"learned_every_{iv}", iv, amrhip.MlpTagger(model, args.tol, True)))
-    for name, interval, tagger in runs:
-        r, _ = L.dmr_deploy(2, lmax, t_end, interval, tagger, args.tol, ref, lmax,
-                            args.learned_lag if tagger is not None else 0,
-                            verbose=1 if lmax >= 9 else 0)      # progress lines on long runs
-        out[name] = r
-        log(f"double-Mach {name}: {r['steps']} steps, {r['regrids']} regrids, peak {r['peak_cells']} cells, "
-            f"{r['wall_s']:.2f} s, {r['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s, L1 {r['l1_density']:.4e}")
-    if model is not None:
-        a, b = out["plain_every_10"], out[f"learned_every_{iv}"]
-        out["speedup_end_to_end"] = b["end_to_end_Mcell_upd_s"] / a["end_to_end_Mcell_upd_s"]
-        out["l1_ratio"] = b["l1_density"] / a["l1_density"]
+    for n in args.dmr_patches:
+        d = {8: 0, 16: 1}[n]
+        res = {}
+        if model is not None:
+            t0 = time.perf_counter()
+            ts = L.dmr_collect(2 - d, lmax - 1 - d, t_end, tol=args.tol, window=iv, log=log, n=n)
+            losses = model.train(ts, epochs=10, batch=4096, lr=1e-3, cw=(1.0, 1.0, 2.0), seed=7)
+            res["fine_tune"] = {"rows": ts.size, "epochs": 10,
+                                "loss_first_last": [float(losses[0]), float(losses[-1])],
+                                "wall_s": round(time.perf_counter() - t0, 2)}
+            ts.close()
+        runs = [("plain_every_10", 10, None)]
+        if model is not None:
+            runs.append((f"learned_every_{iv}", iv, amrhip.MlpTagger(model, args.tol, True)))
+        for name, interval, tagger in runs:
+            r, _ = L.dmr_deploy(2 - d, lmax - d, t_end, interval, tagger, args.tol, ref, lmax - d,
+                                args.learned_lag if tagger is not None else 0,
+                                verbose=1 if lmax >= 9 else 0, n=n, tile="auto")
+            res[name] = r
+            log(f"double-Mach {n}x{n} {name}: {r['steps']} steps, {r['regrids']} regrids, peak {r['peak_cells']} "
+                f"cells, {r['wall_s']:.2f} s, {r['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s, L1 {r['l1_density']:.4e}")
+        if model is not None:
+            a, b = res["plain_every_10"], res[f"learned_every_{iv}"]
+            res["speedup_end_to_end"] = b["end_to_end_Mcell_upd_s"] / a["end_to_end_Mcell_upd_s"]
+            res["wall_time_ratio_plain_over_learned"] = a["wall_s"] / b["wall_s"]
+            res["l1_ratio"] = b["l1_density"] / a["l1_density"]
+        out[f"patch_{n}"] = res
     return out
 
 
-def batched_experiment(amrhip, args, model):
-    """K independent Riemann problems (seeds 1000+k, max level drawn from
-    {5,6,7,8}, levels lmin..max) advanced together in one device context
-    (amr_evolve_batch, one launch per RK stage for all), against the same
-    problems run one after another on the plain path; per-instance final
-    states compared bit for bit; then the batch with the learned tagger
-    every --learned-interval steps.  Rates count the cells of instances that
-    have not reached their stop."""
+def batch_l1(L, forests, seeds, lmaxs, t_end, levels, cache):
+    """mean L1 density error over the instances whose finest level is in
+    `levels`, each against the plain solution on its uniform finest level
+    (computed once per instance and cached)"""
+    errs = []
+    for f, s, lm in zip(forests, seeds, lmaxs):
+        if lm not in levels:
+            continue
+        if s not in cache:
+            cache[s], _ = L.uniform_reference(s, lm, t_end)
+        errs.append(L.l1_error(L.density_on_grid(f, lm), cache[s]))
+    return float(np.mean(errs)), len(errs)
+
+
+def batched_experiment(amrhip, args, model, plain_run, plain_forests, problem):
+    """the same 64 problems run one after another on the plain path (8x8,
+    every 10 steps) against the batch (plain_run: the headline's 8x8 batch
+    run), per-instance final states compared bit for bit; then the batch with
+    the learned tagger every --learned-interval steps.  Rates count the cells
+    of instances that have not reached their stop; L1 errors are averaged
+    over the instances with finest level 5-7 against their uniform plain
+    solutions."""
     from amrhip import learned as L
-    rng = np.random.default_rng(2024)
-    K = args.batch_instances
-    lmaxs = [int(x) for x in rng.choice([5, 6, 7, 8], size=K)]
-    seeds = [1000 + k for k in range(K)]
+    seeds, lmaxs = problem
     t_end = args.batch_t_end
-
-    def forests():
-        return [L.riemann_forest(s, args.lmin, lm, args.tol) for s, lm in zip(seeds, lmaxs)]
-
-    # one by one, plain every 10 steps; the batch instances take the same
-    # settings (effective CFL 0.25 * 0.4 * 3 as computed from the input)
     L.riemann_params(t_end, regrid=10)
     p = amrhip.get_params()
     insts = [amrhip.make_inst(p.coarse_grid_h, tuple(p.bc[:4]), p.cfl, p.dt, t_end) for _ in seeds]
-    out = {"instances": K, "max_levels": lmaxs, "t_end": t_end}
+    out = {"instances": len(seeds), "max_levels": lmaxs, "t_end": t_end}
     alone, wall_alone, cs_alone = [], 0.0, 0.0
-    for fa in forests():
+    for fa in batch_forests(L, seeds, lmaxs, args.lmin, args.tol, 8):
         t0 = time.perf_counter()
         st = fa.evolve_device(args.tol, 10)
         wall_alone += time.perf_counter() - t0
@@ -241,78 +457,83 @@ def batched_experiment(amrhip, args, model):
         alone.append(fa)
     out["sequential_plain_every_10"] = {"wall_s": round(wall_alone, 3), "cell_stages": cs_alone,
                                         "end_to_end_Mcell_upd_s": cs_alone / wall_alone / 1e6}
-    log(f"batch: {K} instances one by one: {wall_alone:.2f} s, {cs_alone / wall_alone / 1e6:.0f} Mcell-upd/s")
-    runs = [("batched_plain_every_10", 10, None)]
+    log(f"batch: {len(seeds)} instances one by one: {wall_alone:.2f} s, {cs_alone / wall_alone / 1e6:.0f} Mcell-upd/s")
+    same = [bool(a.quads() == b.quads() and np.array_equal(a.soln(), b.soln())) for a, b in zip(alone, plain_forests)]
+    r = dict(plain_run)
+    r["instances_bitwise_equal_to_alone"] = int(sum(same))
+    out["batched_plain_every_10"] = r
+    del alone
+    levels = (5, 6, 7)
+    cache = {}
+    r["l1_density_mean_levels_5_7"], nl1 = batch_l1(L, plain_forests, seeds, lmaxs, t_end, levels, cache)
     if model is not None:
-        runs.append((f"batched_learned_every_{args.learned_interval}", args.learned_interval,
-                     amrhip.MlpTagger(model, args.tol, True)))
-    for name, iv, tagger in runs:
-        fb = forests()
-        t0 = time.perf_counter()
-        per, tot = amrhip.evolve_batch(fb, insts, args.tol, iv, tagger=tagger,
-                                       regrid_lag=args.learned_lag if tagger is not None else 0)
-        wall = time.perf_counter() - t0
-        useful = sum(p.cell_stages for p in per)
-        r = {"wall_s": round(wall, 3), "steps": tot.steps, "regrids": tot.regrids,
-             "peak_cells": tot.peak_quads * 64, "gpu_s": round(tot.wall_gpu, 3),
-             "regrid_s": round(tot.wall_regrid, 3),
-             "regrid_split_s": {"tags": round(tot.wall_tags, 3), "host_forest": round(tot.wall_forest, 3),
-                                "device_remesh": round(tot.wall_remesh, 3)},
-             "cell_stages": useful,
-             "end_to_end_Mcell_upd_s": useful / wall / 1e6,
-             "launched_Mcell_upd_s": tot.cell_stages / wall / 1e6,
-             # the stage launches alone: every launched cell-stage over the
-             # run's GPU time, and that rate's algorithmic bytes against HBM peak
-             "gpu_Mcell_upd_s": tot.cell_stages / tot.wall_gpu / 1e6,
-             "gpu_hbm_frac": tot.cell_stages / tot.wall_gpu * B_ALG_STAGE / (HBM_PEAK_GBS * 1e9)}
-        if tagger is None:
-            same = [bool(a.quads() == b.quads() and np.array_equal(a.soln(), b.soln())) for a, b in zip(alone, fb)]
-            r["instances_bitwise_equal_to_alone"] = int(sum(same))
-        out[name] = r
-        log(f"{name}: {wall:.2f} s, {r['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s end to end, "
-            f"{tot.regrids} regrids, peak {tot.peak_quads * 64} cells")
-    base = out["sequential_plain_every_10"]["end_to_end_Mcell_upd_s"]
-    for name, _, _ in runs:
-        out[name]["speedup_vs_sequential_plain"] = out[name]["end_to_end_Mcell_upd_s"] / base
+        iv, lag = args.learned_interval, args.learned_lag
+        fb = batch_forests(L, seeds, lmaxs, args.lmin, args.tol, 8)
+        L.riemann_params(t_end, regrid=iv)
+        rl, _ = run_batch(amrhip, fb, insts, args.tol, iv, 8, tagger=amrhip.MlpTagger(model, args.tol, True),
+                          lag=lag)
+        rl["l1_density_mean_levels_5_7"], _ = batch_l1(L, fb, seeds, lmaxs, t_end, levels, cache)
+        out[f"batched_learned_every_{iv}_lag_{lag}"] = rl
+        log(f"batched learned: {rl['wall_s']:.2f} s, {rl['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s end to end, "
+            f"L1 {rl['l1_density_mean_levels_5_7']:.4e} vs plain {r['l1_density_mean_levels_5_7']:.4e}")
+        rl["speedup_vs_sequential_plain"] = rl["end_to_end_Mcell_upd_s"] / out["sequential_plain_every_10"][
+            "end_to_end_Mcell_upd_s"]
+        rl["wall_time_ratio_sequential_plain_over_batched_learned"] = wall_alone / rl["wall_s"]
+    out["l1_instances"] = nl1
+    r["speedup_vs_sequential_plain"] = r["end_to_end_Mcell_upd_s"] / out["sequential_plain_every_10"][
+        "end_to_end_Mcell_upd_s"]
     return out
 
 
+# ---------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--lmin", type=int, default=3)
     ap.add_argument("--lmax", type=int, default=7)
-    ap.add_argument("--n", type=int, default=8)
+    ap.add_argument("--n", type=int, default=8, help="patch size of the vortex / riemann4 workloads")
     ap.add_argument("--tol", type=float, default=0.1)
     ap.add_argument("--t-end", type=float, default=0.25)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile", type=str, default="", help="ppb,minw,design (skips the autotuner)")
-    ap.add_argument("--workload", choices=["riemann4", "vortex"], default="riemann4",
-                    help="riemann4: the adaptive Riemann problem the metric is quoted on; "
-                         "vortex: uniform periodic isentropic vortex at --level (4 = config 1)")
+    ap.add_argument("--workload", choices=["batch", "riemann4", "vortex"], default="batch",
+                    help="batch: the 64-instance batch (headline); riemann4: the single adaptive Riemann "
+                         "problem's 207k-cell mesh; vortex: uniform periodic isentropic vortex at --level")
     ap.add_argument("--level", type=int, default=4)
-    ap.add_argument("--host-loop", action="store_true",
-                    help="also time the interval-by-interval host-regrid loop")
+    ap.add_argument("--batch-instances", type=int, default=64)
+    ap.add_argument("--batch-t-end", type=float, default=0.25)
+    ap.add_argument("--batch-patches", type=str, default="8,16",
+                    help="patch sizes the batch autotuner compares")
+    ap.add_argument("--save-mesh", type=str, default="", help="batch: write the timed meshes here")
+    ap.add_argument("--load-mesh", type=str, default="",
+                    help="batch: time the meshes written by --save-mesh (no batch runs)")
+    ap.add_argument("--load-patch", type=int, default=8, help="patch size of the meshes --load-mesh reads")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the 207k-cell Riemann line")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-tagger comparison")
+    ap.add_argument("--no-equal-l1", action="store_true", help="skip the equal-L1 tolerance search")
     ap.add_argument("--learned-seeds", type=int, default=8, help="training runs (seeds 1..k)")
     ap.add_argument("--learned-lmax", type=int, default=9,
                     help="finest level of the deployed run (9: about 1M cells at peak)")
     ap.add_argument("--learned-interval", type=int, default=40)
     ap.add_argument("--learned-epochs", type=int, default=30)
     ap.add_argument("--learned-lag", type=int, default=10,
-                    help="learned runs: steps the GPU advances while the host regrids (0: none)")
+                    help="runs every --learned-interval steps: steps the GPU advances while the host regrids")
     ap.add_argument("--no-dmr", action="store_true", help="skip the double-Mach reflection comparison")
     ap.add_argument("--dmr-lmax", type=int, default=7)
     ap.add_argument("--dmr-t-end", type=float, default=0.2)
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances comparison")
-    ap.add_argument("--batch-instances", type=int, default=64)
-    ap.add_argument("--batch-t-end", type=float, default=0.25)
+    ap.add_argument("--dmr-patches", type=str, default="8,16")
+    ap.add_argument("--dmr-track", type=int, default=0,
+                    help="only: run the uniform double-Mach forest of this level (8x8) with the state "
+                         "inspected every 2048 steps, print the records as JSON and exit")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-mode comparison")
     args = ap.parse_args()
+    args.batch_patches = [int(x) for x in args.batch_patches.split(",") if x]
+    args.dmr_patches = [int(x) for x in args.dmr_patches.split(",") if x]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: the benchmark runs on exactly one GPU (the whole forest lives on one "
               "MI355X); sharding over several GPUs is designed but not implemented, so --gpus N "
@@ -322,65 +543,61 @@ def main():
 
     import amrhip
     amrhip.lib()
-    if args.workload == "riemann4":
-        log(f"setup: adaptive Riemann seed={args.seed} levels {args.lmin}-{args.lmax} n={args.n} "
-            f"to t={args.t_end}")
-        # the first run in the process pays the runtime's one-time set-up
-        # (code-object load, first allocations); the second is the steady state
-        f_cold, st_cold, wall_cold = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, args.n, args.tol,
-                                                   args.t_end, 10, device=True)
-        f, st, wall = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, args.n, args.tol,
-                                         args.t_end, 10, device=True)
-        if (st.steps, st.regrids, f.nquad) != (st_cold.steps, st_cold.regrids, f_cold.nquad):
-            raise RuntimeError("two identical adaptive runs differ")
-        host_run = None
-        if args.host_loop:
-            _, sth, wallh = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, args.n,
-                                               args.tol, args.t_end, 10, device=False)
-            host_run = (sth, wallh)
-        h_base, bc = 1.0, (0, 0, 0, 0)
-        workload = (f"riemann4 seed {args.seed}, adaptive levels {args.lmin}-{args.lmax}, "
-                    f"{args.n}x{args.n} FP64 patches, SSP-RK3 CFL 0.4, mesh at t={st.t:.3f}")
+    if args.dmr_track:
+        from amrhip import learned as L
+        recs, reached = L.dmr_track(args.dmr_track, args.dmr_t_end, every=2048, log=log)
+        print(json.dumps({"dmr_track_level": args.dmr_track, "t_end": args.dmr_t_end, "reached": reached,
+                          "records": recs}), flush=True)
+        return
+    tile_arg = tuple(int(x) for x in args.tile.split(",")) if args.tile else None
+    batch_runs, problem, plain_forests = None, None, None
+    if args.workload == "batch":
+        if args.load_mesh:
+            n = args.load_patch
+            forests = load_forests(amrhip, args.load_mesh, args.batch_instances)
+            log(f"loaded {len(forests)} meshes ({n}x{n}) from {args.load_mesh}")
+        else:
+            n, by_patch, batch_runs, problem = batch_headline(amrhip, args)
+            forests, plain_forests = by_patch[n], by_patch.get(8)
+            if args.save_mesh:
+                save_forests(forests, args.save_mesh)
+        arrays = amrhip.batch_external(forests)
+        insts = [amrhip.make_inst(1.0, (0, 0, 0, 0), CFL_EFF, 1.0, 0.0) for _ in forests]
+        nq = arrays[0].size // 3
+        lv = np.concatenate([f.levels() for f in forests])
+        workload = (f"{len(forests)} batched riemann4 problems (seeds 1000-{999 + len(forests)}, finest level "
+                    f"from {{5,6,7,8}} at 8x8), {n}x{n} FP64 patches, SSP-RK3 CFL 0.4, meshes at t={args.batch_t_end}")
+        wl_key, h_base = "batch64", 1.0
+    elif args.workload == "riemann4":
+        n = args.n
+        f, st, _ = build_riemann_mesh(amrhip, args.seed, args.lmin, args.lmax, n, args.tol, args.t_end, 10)
+        ci, fi, geom, soln = f.external()
+        arrays = (ci, fi, geom, soln, None)
+        insts = [amrhip.make_inst(1.0, (0, 0, 0, 0), CFL_EFF, 1.0, 0.0)]
+        nq, lv = f.nquad, f.levels()
+        workload = (f"riemann4 seed {args.seed}, adaptive levels {args.lmin}-{args.lmax}, {n}x{n} FP64 patches, "
+                    f"SSP-RK3 CFL 0.4, mesh at t={st.t:.3f}")
+        wl_key, h_base = "riemann4", 1.0
     else:
-        f = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 10.0, args.n, args.level, args.level)
+        n = args.n
+        f = amrhip.Forest(1, 1, (1, 1), 0.0, 0.0, 10.0, n, args.level, args.level)
         f.apply_ic(amrhip.vortex_ic(5.0, 5.0, 5.0))
-        st, wall, host_run = None, 0.0, None
-        st_cold, wall_cold = None, 0.0
-        h_base, bc = 10.0, (0, 0, 0, 0)
+        ci, fi, geom, soln = f.external()
+        arrays = (ci, fi, geom, soln, None)
+        insts = [amrhip.make_inst(10.0, (0, 0, 0, 0), CFL_EFF, 1.0, 0.0)]
+        nq, lv = f.nquad, f.levels()
         workload = (f"isentropic vortex strength 5, periodic [0,10]^2, uniform level {args.level}, "
-                    f"{args.n}x{args.n} FP64 patches, SSP-RK3 CFL 0.4")
-    ci, fi, geom, soln = f.external()
-    nq = f.nquad
-    cells = nq * args.n * args.n
-    lv = f.levels()
-    if st is not None:
-        log(f"setup done: {st.steps} steps, {st.regrids} regrids, t={st.t:.4f}, peak {st.peak_quads} "
-            f"quads, final {nq} quads / {cells} cells, levels {lv.min()}-{lv.max()}, {wall:.2f} s "
-            f"(gpu {st.wall_gpu:.2f} s, regrid+boundary {st.wall_regrid:.2f} s)")
-    else:
-        log(f"setup done: {nq} quads / {cells} cells")
-
-    g = amrhip.Gpu()
-    g.setup(args.n, 3, [amrhip.make_inst(h_base, bc, 0.3, 1.0, 0.0)], ci, fi, geom, soln)
-    tune = None
-    if args.tile:
-        g.set_tile(*(int(x) for x in args.tile.split(",")))
-    elif not args.no_autotune:
-        tune = g.autotune(10)
-        log("autotune (ppb, minw, design, ms/step): " +
-            ", ".join(f"({a},{b},{c},{d:.4f})" for a, b, c, d in tune))
+                    f"{n}x{n} FP64 patches, SSP-RK3 CFL 0.4")
+        wl_key, h_base = f"vortex_l{args.level}_n{n}", 10.0
+    cells = nq * n * n
+    log(f"timed mesh: {nq} quads / {cells} cells, levels {lv.min()}-{lv.max()}")
+    g, tune, ms, stage_ms = time_steps(amrhip, n, insts, arrays, args.steps, args.warmup, tile_arg,
+                                       not args.no_autotune)
     tile = g.tile()
     launch = "graph replay" if g.graphs() else "direct stream launches"
-    if args.warmup:
-        g.run(args.warmup)
-    ms, stage_ms = g.bench(args.steps)
+    g.close()
     ms_per_step = ms / args.steps
     value = cells * 3 * args.steps / (ms * 1e-3) / 1e6
-    # average launch duration over the timed region (HIP events on the
-    # context's stream around the steps): region time / launches
-    avg_stage = ms / (3 * args.steps)
-    bytes_launch = cells * B_ALG_STAGE
-    achieved = bytes_launch / (avg_stage * 1e-3) / 1e9
     log(f"timed: {args.steps} steps, {ms:.2f} ms ({ms_per_step * 1e3:.1f} us/step); stage kernel "
         f"avg {[round(s * 1e3, 2) for s in stage_ms]} us; {value:.1f} Mcell-updates/s")
     try:
@@ -388,8 +605,12 @@ def main():
     except Exception as e:   # noqa: BLE001
         log(f"copy probe failed: {e}")
         probe = None
-    g.close()
-
+    roof = roofline_of(cells, ms, args.steps, n, tile, launch, stage_ms)
+    roof["copy_probe_GBs"] = round(probe, 1) if probe else None
+    traffic = measured_traffic(wl_key, cells)
+    if traffic is not None:
+        roof["traffic"] = traffic[0]
+        roof["traffic_tile"] = traffic[1]
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -405,71 +626,40 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": workload,
-            "quads": nq, "cells": cells, "peak_quads_in_run": st.peak_quads if st else nq,
-            "levels": [int(lv.min()), int(lv.max())], "parallelism": "1 GPU, whole forest",
+            "quads": nq, "cells": cells, "patch": n,
+            "levels": [int(lv.min()), int(lv.max())], "parallelism": "1 GPU, every instance in one launch",
             "tile": {"patches_per_workgroup": tile[0], "min_waves_per_simd": tile[1],
                      "design": ["face task list", "paired faces"][tile[2]],
                      "autotuned": tune is not None, "launch": launch},
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": f"stage_kernel<{args.n},{tile[0]},{tile[1]},{tile[2]}>",
-            "bytes_per_launch": int(bytes_launch),
-            "avg_launch_us": round(avg_stage * 1e3, 3),
-            "avg_launch_from": f"timed region / launches (HIP events, {launch})",
-            "stage_launch_us_bracketed": [round(s * 1e3, 3) for s in stage_ms],
-            "copy_probe_GBs": round(probe, 1) if probe else None,
-        },
+        "roofline": roof,
     }
-    if st is not None:
-        out["adaptive_run"] = {
-            "loop": "state resident in HBM, device indicator + remesh, host topology",
-            "steps": st.steps, "regrids": st.regrids, "t": round(st.t, 6),
-            "cell_stages": st.cell_stages, "wall_s": round(wall, 3),
-            "gpu_s": round(st.wall_gpu, 3), "regrid_s": round(st.wall_regrid, 3),
-            "regrid_split_s": {"tags": round(st.wall_tags, 3), "host_forest": round(st.wall_forest, 3),
-                               "device_remesh": round(st.wall_remesh, 3)},
-            "end_to_end_Mcell_updates_per_s": round(st.cell_stages / wall / 1e6, 3),
-            "first_run_in_process": {
-                "wall_s": round(wall_cold, 3), "regrid_s": round(st_cold.wall_regrid, 3),
-                "end_to_end_Mcell_updates_per_s": round(st_cold.cell_stages / wall_cold / 1e6, 3),
-                "note": "includes the runtime's one-time set-up; the figures above are the second, "
-                        "identical run",
-            },
-        }
-        if host_run is not None:
-            sth, wallh = host_run
-            out["adaptive_run_host_regrid"] = {
-                "loop": "hpc_amr_evolve_solver per interval, host regrid",
-                "steps": sth.steps, "regrids": sth.regrids, "wall_s": round(wallh, 3),
-                "end_to_end_Mcell_updates_per_s": round(sth.cell_stages / wallh / 1e6, 3),
-            }
-    traffic = measured_traffic(args.workload, cells, tile)
-    if traffic is not None:
-        out["roofline"]["traffic"] = traffic[0]
-        out["roofline"]["traffic_tile"] = traffic[1]
+    if batch_runs is not None:
+        out["config"]["autotuner_patch_choice"] = {
+            "rule": f"shorter batch run to t={args.batch_t_end} (tile autotuned in each run)",
+            "chosen": n, "runs": {str(k): v for k, v in batch_runs.items()}}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        best = cpu_baseline(ci, fi, geom, soln, args.n, args.cpu_seconds, threads, h_base)
-        if best:
-            v, label, k, dt = best
+        lines = cpu_baseline(*arrays[:4], n, args.cpu_seconds, threads, h_base)
+        if lines:
+            best = max(lines, key=lambda k: lines[k]["value"])
+            b = lines[best]
             out["cpu_baseline"] = {
-                "value": round(v, 3), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
-                "sample": f"{k} SSP-RK3 steps of the same {cells}-cell mesh ({dt:.1f} s), "
-                          f"C oracle {label}, {threads} threads on contiguous quadrant ranges",
+                "value": b["value"], "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+                "sample": f"{b['steps']} SSP-RK3 steps of the same {cells}-cell mesh ({b['seconds']} s), "
+                          f"C oracle {best} (faster build), {threads} threads on contiguous quadrant ranges",
+                "builds": lines, "cpu_model": cpu_model(), "host_logical_cpus": os.cpu_count(),
+                "threads_note": "one worker per core of the GPU's share of the host (at most 16)",
             }
-            out["gpu_over_cpu"] = round(value / v, 1)
+            out["gpu_over_cpu"] = round(value / b["value"], 1)
+    if not args.no_secondary and args.workload == "batch" and not args.load_mesh:
+        out["riemann4_single"] = riemann_line(amrhip, args)
     model = None
-    if args.workload == "riemann4" and args.n == 8 and not args.no_learned:
+    if not args.no_learned and not args.load_mesh:
         out["learned"], model = learned_experiment(amrhip, args)
-    if args.workload == "riemann4" and args.n == 8 and not args.no_batch:
-        out["batched"] = batched_experiment(amrhip, args, model)
-    if args.workload == "riemann4" and args.n == 8 and not args.no_dmr:
+    if not args.no_batch and plain_forests is not None:
+        out["batched"] = batched_experiment(amrhip, args, model, batch_runs[8], plain_forests, problem)
+    if not args.no_dmr and not args.load_mesh:
         out["double_mach"] = dmr_experiment(amrhip, args, model)
     if model is not None:
         model.close()
