@@ -54,6 +54,7 @@ EXPORTS = {
     "amrgpu_tile_variants": (_i, [_i, _pi, _pi, _pi, _i]),
     "amrgpu_autotune": (_i, [_vp, _i, _pi, _pi, _pi, _pi, _pd]),
     "amrgpu_feature_tags": (_i, [_vp, _i, _pi, _pd, _pi]),
+    "amrgpu_cfl_noalloc": (_i, [_i, _i, _i, _pd, _pd, _vp, _vp, _vp, _vp, _vp]),
     # include/amr_forest.h
     "amr_forest_new": (_vp, [_i, _i, _i, _i, _d, _d, _d, _i, _i, _i]),
     "amr_forest_destroy": (None, [_vp]),

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "amr_hip.h"
@@ -1076,10 +1077,13 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK)) void cfl_kernel(MeshDev m, con
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int p = tid / N2, c = tid % N2;
     const int quad = lb * PPB + p;
-    const bool active = quad < m.nquad;
+    // a quadrant whose instance id is outside [0, ninst) is skipped (ids
+    // from a caller's device array cannot be checked on the host)
+    const bool active = quad < m.nquad && (unsigned)m.inst[quad] < (unsigned)m.ninst;
     if (tid < PPB) {
         int qq = lb * PPB + tid;
-        pinst[tid] = qq < m.nquad ? m.inst[qq] : -1;
+        const int in = qq < m.nquad ? m.inst[qq] : -1;
+        pinst[tid] = (unsigned)in < (unsigned)m.ninst ? in : -1;
     }
     __syncthreads();
     int uinst = pinst[0];
@@ -2496,6 +2500,113 @@ int amrgpu_set_graphs(amrgpu_ctx *c, int enable)
 }
 
 int amrgpu_get_graphs(const amrgpu_ctx *c) { return c && c->graphs_enabled ? 1 : 0; }
+
+// ---------------------------------------------------------------------------
+// amrgpu_cfl_noalloc: the per-instance CFL minimum of amrgpu_cfl for callers
+// that cannot allocate.  Its device memory beyond the caller's inputs and
+// outputs is the file-scope array below (instance table + CFL sub-slots for
+// at most AMRGPU_NOALLOC_MAX_INST instances, 130 KB), which each GPU the
+// library runs on holds until the process ends.  Host code reaches it only
+// through the address hipGetSymbolAddress returns on that device.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t NOALLOC_IP_BYTES = ((sizeof(InstDev) * AMRGPU_NOALLOC_MAX_INST + 255) / 256) * 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_IP_BYTES + sizeof(unsigned long long) * AMRGPU_NOALLOC_MAX_INST * NSLOT;
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_BYTES / sizeof(unsigned long long)];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES];     // the array's address on each device
+std::mutex g_scratch_lock;
+
+struct NoallocInst {
+    double h_base[AMRGPU_NOALLOC_MAX_INST];
+    double cfl[AMRGPU_NOALLOC_MAX_INST];
+};
+
+// instance table (the fields cfl_kernel reads) and every sub-slot reset, in
+// one kernel: no memset node if the caller captures the launches
+__global__ void noalloc_prep_kernel(InstDev *ip, unsigned long long *slot, int ninst, NoallocInst p)
+{
+    for (int i = threadIdx.x; i < ninst; i += blockDim.x) {
+        InstDev d;
+        memset(&d, 0, sizeof(d));
+        d.h_base = p.h_base[i];
+        d.cfl = p.cfl[i];
+        ip[i] = d;
+    }
+    for (int i = threadIdx.x; i < ninst * NSLOT; i += blockDim.x) slot[i] = DBL_INF_BITS;
+}
+
+__global__ void noalloc_finish_kernel(const unsigned long long *slot, int ninst, double *dt)
+{
+    // one wave per instance
+    const int i = blockIdx.x;
+    if (i < ninst) {
+        const double m = slot_min(slot, i);
+        if (threadIdx.x == 0) dt[i] = m;
+    }
+}
+
+int noalloc_scratch(void **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) { set_err("amrgpu_cfl_noalloc: device %d", dev); return AMRGPU_ERR_ARG; }
+    std::lock_guard<std::mutex> hold(g_scratch_lock);
+    if (!g_scratch_ptr[dev]) {
+        // the array's own name is a host-side stand-in: only this address,
+        // taken once per device with that device current, is valid in
+        // kernels and copies
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return AMRGPU_OK;
+}
+}  // namespace
+
+int amrgpu_cfl_noalloc(int nelem_subgrid, int nquad, int ninst, const double *h_base, const double *cfl,
+                       const int *d_level, const int *d_inst, const double *d_U, double *d_dt, void *stream)
+{
+    if (ninst < 1 || ninst > AMRGPU_NOALLOC_MAX_INST) {
+        set_err("amrgpu_cfl_noalloc: %d instances (1 to %d)", ninst, AMRGPU_NOALLOC_MAX_INST);
+        return AMRGPU_ERR_ARG;
+    }
+    if (nquad < 1 || !h_base || !cfl || !d_level || !d_inst || !d_U || !d_dt || ppb_of(nelem_subgrid) == 0) {
+        set_err("amrgpu_cfl_noalloc: bad arguments");
+        return AMRGPU_ERR_ARG;
+    }
+    void *base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    InstDev *ip = (InstDev *)base;
+    unsigned long long *slot = (unsigned long long *)((char *)base + NOALLOC_IP_BYTES);
+    NoallocInst p;
+    memset(&p, 0, sizeof(p));
+    for (int i = 0; i < ninst; ++i) { p.h_base[i] = h_base[i]; p.cfl[i] = cfl[i]; }
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(noalloc_prep_kernel, dim3(1), dim3(256), 0, st, ip, slot, ninst, p);
+    HIPCHK(hipGetLastError());
+    MeshDev m;
+    memset(&m, 0, sizeof(m));
+    m.nquad = nquad;
+    m.ninst = ninst;
+    m.level = d_level;
+    m.inst = d_inst;
+    m.ip = ip;
+    const int ppb = ppb_of(nelem_subgrid), nblk = (nquad + ppb - 1) / ppb;
+    switch (nelem_subgrid) {
+    case 2: hipLaunchKernelGGL((cfl_kernel<2, 16>), dim3(nblk), dim3(64), 0, st, m, d_U, slot); break;
+    case 4: hipLaunchKernelGGL((cfl_kernel<4, 8>), dim3(nblk), dim3(128), 0, st, m, d_U, slot); break;
+    case 8: hipLaunchKernelGGL((cfl_kernel<8, 4>), dim3(nblk), dim3(256), 0, st, m, d_U, slot); break;
+    default: hipLaunchKernelGGL((cfl_kernel<16, 1>), dim3(nblk), dim3(256), 0, st, m, d_U, slot); break;
+    }
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(noalloc_finish_kernel, dim3(ninst), dim3(64), 0, st, slot, ninst, d_dt);
+    HIPCHK(hipGetLastError());
+    return AMRGPU_OK;
+}
 
 int amrgpu_copy_probe(long long nbytes, int reps, double *gbps)
 {

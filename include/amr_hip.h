@@ -221,6 +221,25 @@ int amrgpu_num_quads(const amrgpu_ctx *c);
 int amrgpu_device_state(amrgpu_ctx *c, const double **U, const int **nbr, int *nquad, int *n,
                         void **stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the per-instance CFL minimum of amrgpu_cfl, for a state
+ * the caller keeps on the device, with no allocation of any kind.  Inputs:
+ * d_U the state in the context layout U[quad][field][j][i] (FP64, n x n
+ * patches, nquad quadrants), d_level / d_inst each quadrant's level and
+ * instance (int32, device), h_base[ninst] / cfl[ninst] per instance (host,
+ * passed by value); output d_dt[ninst] (device): min over the instance's
+ * cells of cfl sqrt(V) / (|u| + c), bit for bit amrgpu_cfl's value.  The
+ * work goes on `stream` (NULL: the default stream) and is not waited for.
+ * Scratch is one static device array of the library (instance table + CFL
+ * sub-slots, about 130 KB, held by each GPU until the process ends); it
+ * keeps the last call's contents and is re-initialised by every call, so
+ * calls that overlap on one GPU race.  1 <= ninst <= AMRGPU_NOALLOC_MAX_INST,
+ * else AMRGPU_ERR_ARG. */
+#define AMRGPU_NOALLOC_MAX_INST 64
+int amrgpu_cfl_noalloc(int nelem_subgrid, int nquad, int ninst, const double *h_base, const double *cfl,
+                       const int *d_level, const int *d_inst, const double *d_U, double *d_dt, void *stream);
+
 /* Device-side helpers for the HBM roofline probe: copy nbytes between two
  * device buffers allocated by the library, timed with events. */
 int amrgpu_copy_probe(long long nbytes, int reps, double *gbps);

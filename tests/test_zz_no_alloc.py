@@ -26,7 +26,36 @@ def forest(seed, n, lmin, lmax):
     return f
 
 
-def check(torch, n, forests, h_base, cfl):
+class DeviceBuffers:
+    """device copies of host arrays through the HIP runtime the library
+    itself links (test plumbing: the entry point under test allocates
+    nothing, so its inputs and outputs come from here)"""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.ptrs = []
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        p = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(p), C.c_size_t(max(a.nbytes, 8))) == 0
+        self.ptrs.append(p)
+        assert self.hip.hipMemcpy(p, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) == 0   # H2D
+        return p
+
+    def download(self, p, like):
+        out = np.empty_like(like)
+        assert self.hip.hipDeviceSynchronize() == 0
+        assert self.hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), p, C.c_size_t(out.nbytes), 2) == 0   # D2H
+        return out
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+        self.ptrs = []
+
+
+def check(n, forests, h_base, cfl):
     L = amrhip.lib()
     insts = [amrhip.make_inst(h, (0, 0, 0, 0), c, 1.0) for h, c in zip(h_base, cfl)]
     ci, fi, geom, soln, cinst = amrhip.batch_external(forests)
@@ -35,35 +64,33 @@ def check(torch, n, forests, h_base, cfl):
     ref = g.cfl()
     U, nq, nn, st = C.c_void_p(), C.c_int(), C.c_int(), C.c_void_p()
     assert L.amrgpu_device_state(g.h, C.byref(U), None, C.byref(nq), C.byref(nn), C.byref(st)) == 0
-    level = torch.tensor(np.concatenate([f.levels() for f in forests]), dtype=torch.int32, device="cuda")
-    inst = torch.tensor(cinst, dtype=torch.int32, device="cuda")
-    dt = torch.full((len(forests),), -1.0, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
+    dev = DeviceBuffers()
+    level = dev.upload(np.concatenate([f.levels() for f in forests]).astype(np.int32))
+    inst = dev.upload(np.asarray(cinst, np.int32))
+    init = np.full(len(forests), -1.0)
+    dt = dev.upload(init)
     hb = np.ascontiguousarray(h_base, np.float64)
     cf = np.ascontiguousarray(cfl, np.float64)
     print(f"noalloc call: n={n}, {nq.value} quads, {len(forests)} instances", flush=True)
-    rc = L.amrgpu_cfl_noalloc(n, nq.value, len(forests), dpt(hb), dpt(cf), C.c_void_p(level.data_ptr()),
-                              C.c_void_p(inst.data_ptr()), U, C.c_void_p(dt.data_ptr()), None)
+    rc = L.amrgpu_cfl_noalloc(n, nq.value, len(forests), dpt(hb), dpt(cf), level, inst, U, dt, None)
     assert rc == 0, amrhip.last_error()
-    torch.cuda.synchronize()
-    got = dt.cpu().numpy()
+    got = dev.download(dt, init)
+    dev.free()
     g.close()
     assert np.array_equal(got, ref), (got, ref)
     return got
 
 
 def test_noalloc_cfl_equals_amrgpu_cfl_on_two_calls():
-    torch = pytest.importorskip("torch")
     big = [forest(1, 8, 1, 3), forest(2, 8, 2, 4), forest(3, 8, 1, 2)]
-    first = check(torch, 8, big, [0.5, 0.5, 0.25], [0.3, 0.2, 0.3])
+    first = check(8, big, [0.5, 0.5, 0.25], [0.3, 0.2, 0.3])
     small = [forest(4, 8, 1, 2), forest(5, 8, 1, 1)]
-    second = check(torch, 8, small, [0.5, 1.0], [0.3, 0.3])
+    second = check(8, small, [0.5, 1.0], [0.3, 0.3])
     assert len(second) == 2 and np.all(np.isfinite(first)) and np.all(second > 0)
 
 
 def test_noalloc_cfl_16x16():
-    torch = pytest.importorskip("torch")
-    check(torch, 16, [forest(6, 16, 1, 2), forest(7, 16, 1, 3)], [0.5, 0.5], [0.3, 0.3])
+    check(16, [forest(6, 16, 1, 2), forest(7, 16, 1, 3)], [0.5, 0.5], [0.3, 0.3])
 
 
 def test_noalloc_cfl_rejects_too_many_instances():
