@@ -91,6 +91,7 @@ struct StageArgs {
     double *t_step;      // written at the step's first stage (equal values)
     unsigned long long *slot;
     double *norm_part;
+    int per_block;       // persistent design: patches per workgroup (a contiguous range)
 };
 
 __device__ __forceinline__ double bits2d(unsigned long long b) { return __longlong_as_double((long long)b); }
@@ -1064,6 +1065,373 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     if (a.norm_part) sum_reduce_store<TL::BLK>(sq, &a.norm_part[blockIdx.x], red);
 }
 
+
+
+// ---------------------------------------------------------------------------
+// Persistent, prefetching stage kernel (DESIGN 2, 16x16 patches): a grid of
+// about as many workgroups as the GPU holds at once, each walking a
+// contiguous Morton range of patches (per_block of them; the XCD-aware block
+// order keeps each XCD's ranges contiguous).  While one patch is evaluated,
+// the next patch's own state and halo cells travel into the other half of a
+// double-buffered LDS area by direct global-to-LDS loads (no registers), so
+// a patch's memory latency is paid behind the arithmetic of the patch before
+// it.  Per patch the arithmetic is stage_kernel's paired-face design,
+// operation for operation (the results are bitwise those of DESIGN 1).  The
+// CFL minimum is kept per wave across patches of one instance and folded
+// into the instance's sub-slot once per instance run; the residual sum of
+// squares once per workgroup.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dma16(const double *src, double *lds)    // 16 B per lane, lane-contiguous
+{
+    __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)src,
+                                     (void __attribute__((address_space(3))) *)lds, 16, 0, 0);
+}
+__device__ __forceinline__ void dma4(const unsigned *src, double *lds)   // 4 B per lane, lane-contiguous
+{
+    __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)src,
+                                     (void __attribute__((address_space(3))) *)lds, 4, 0, 0);
+}
+
+// edge cell of the neighbour across side sd at position k (ec) and the own
+// edge cell (oc), cell index j * N + i
+template <int N>
+__device__ __forceinline__ void edge_cells(int sd, int k, int &ec, int &oc)
+{
+    switch (sd) {
+    case 0: ec = k * N + (N - 1); oc = k * N;           break;
+    case 1: ec = k * N + 0;       oc = k * N + (N - 1); break;
+    case 2: ec = (N - 1) * N + k; oc = k;               break;
+    default: ec = 0 * N + k;      oc = (N - 1) * N + k; break;
+    }
+}
+
+// Issue the loads of patch q into one LDS buffer: its own state (4 N^2
+// doubles, contiguous: 1 KB per wave instruction) and its halo, the index-
+// aligned edge cells of the face neighbours (own edge cells on a wall; both
+// fine neighbours on a coarse side), gathered as 32-bit halves (a lane pair
+// per double).  Every wave issues its share; the caller waits for them
+// (vmcnt) and synchronises before the buffer is read.
+template <int N>
+__device__ __forceinline__ void prefetch_patch(const double *Us, int q, const int nb[8], bool hang, double *raw,
+                                               double *gh, int tid)
+{
+    constexpr int N2 = N * N, NG = 4 * N, NW = N2 / 64;
+    const int lane = tid & 63, w = tid >> 6;
+    const double *own = Us + (size_t)q * NFLD * N2;
+    constexpr int CH = NFLD * N2 / 128;                 // 1 KB chunks
+#pragma unroll
+    for (int j = w; j < CH; j += NW) dma16(own + j * 128 + lane * 2, raw + j * 128);
+    constexpr int HCH = NFLD * NG / 32;                 // 32 doubles per instruction
+#pragma unroll
+    for (int i = w; i < HCH; i += NW) {
+        const int d = 32 * i + (lane >> 1);
+        const int f = d / NG, hh = d % NG, sd = hh / N, k = hh % N;
+        const int na = sd == 0 ? nb[0] : sd == 1 ? nb[2] : sd == 2 ? nb[4] : nb[6];
+        int ec, oc;
+        edge_cells<N>(sd, k, ec, oc);
+        const double *sa = na < 0 ? own + f * N2 + oc : Us + ((size_t)na * NFLD + f) * N2 + ec;
+        dma4((const unsigned *)sa + (lane & 1), gh + 32 * i);
+        if (hang) {
+            const int nbb = sd == 0 ? nb[1] : sd == 1 ? nb[3] : sd == 2 ? nb[5] : nb[7];
+            if (nbb >= 0)
+                dma4((const unsigned *)(Us + ((size_t)nbb * NFLD + f) * N2 + ec) + (lane & 1),
+                     gh + NFLD * NG + 32 * i);
+        }
+    }
+}
+
+template <int N, int MINW>
+__global__ __launch_bounds__(N * N, MINW) void stage_kernel_pp(StageArgs a)
+{
+    using TL = Tile<N, 1>;
+    constexpr int N2 = N * N, NG = 4 * N, NW = N2 / 64;
+    static_assert(N2 % 64 == 0 && NG <= N2, "one patch of whole waves, one halo cell per thread at most");
+    __shared__ double lds[TL::LDS_DOUBLES];
+    __shared__ __attribute__((aligned(16))) double raw[2][NFLD * N2];
+    __shared__ __attribute__((aligned(16))) double gh[2][2 * NFLD * NG];
+    __shared__ double red[NW];
+    typedef const int __attribute__((address_space(4))) cint4;
+    typedef const InstDev __attribute__((address_space(4))) cinst4;
+    typedef const double __attribute__((address_space(4))) cdbl4;
+
+    const int tid = threadIdx.x, c = tid;
+    const int ci = c % N, cj = c / N;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int q0 = lb * a.per_block;
+    const int q1 = min(q0 + a.per_block, a.m.nquad);
+    const bool needB = (a.mode == MODE_SO && a.U0) || a.mode == MODE_RK4_ACC || a.mode == MODE_RK4_FINAL;
+    const cint4 *nbrc = (const cint4 *)a.m.nbr;
+    const cinst4 *ipc = (const cinst4 *)a.m.ip;
+
+    // time bookkeeping of the step's second stage (the t_inst / slot roles)
+    if ((a.role & ROLE_TIME) && blockIdx.x == 0) {
+        for (int i = tid; i < a.m.ninst; i += N2) a.t_inst[i] += a.dt_cur[i];
+        for (int i = tid; i < a.m.ninst * NSLOT; i += N2) a.slot[i] = DBL_INF_BITS;
+    }
+
+    int nbw[8];
+    bool hangw = false;
+    if (q0 < q1) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) nbw[k] = nbrc[(size_t)q0 * 8 + k];
+        hangw = nbw[1] >= 0 || nbw[3] >= 0 || nbw[5] >= 0 || nbw[7] >= 0;
+        prefetch_patch<N>(a.Us, q0, nbw, hangw, raw[0], gh[0], tid);
+    }
+    double sq = 0.0;
+    double cmin = INFINITY;          // CFL minimum of the current instance run (per wave)
+    int cinst = -1;
+    int buf = 0;
+    for (int q = q0; q < q1; ++q, buf ^= 1) {
+        // this patch's halo and state are in buffer buf once every wave's
+        // loads are complete
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        const int levw = ((const cint4 *)a.m.level)[q];
+        const int inw = ((const cint4 *)a.m.inst)[q];
+        const bool firstw = q == 0 || ((const cint4 *)a.m.inst)[q - 1] != inw;
+        double B[4] = {0.0, 0.0, 0.0, 0.0};
+        const size_t base = (size_t)q * NFLD * N2 + c;
+        if (needB) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) B[f] = a.U0[base + f * N2];
+        }
+        unsigned long long slv[(NSLOT + 63) / 64];
+        if ((a.role & ROLE_DT) && a.use_cfl) {
+            const unsigned long long *sp = a.slot + (size_t)inw * NSLOT;
+            const int lane = tid & 63;
+#pragma unroll
+            for (int k = 0; k < (NSLOT + 63) / 64; ++k)
+                slv[k] = lane + 64 * k < NSLOT ? sp[lane + 64 * k] : DBL_INF_BITS;
+        }
+        // the next patch's neighbour ids (scalar loads, used below)
+        int nbn[8];
+        const bool more = q + 1 < q1;
+        const int qn = more ? q + 1 : q;         // a valid index either way
+#pragma unroll
+        for (int k = 0; k < 8; ++k) nbn[k] = nbrc[(size_t)qn * 8 + k];
+        // records of the own cells and of the halo
+        const double *rw = raw[buf];
+        const double *gw = gh[buf];
+        store_rec<N, 1>(lds, 0, c, rw[c], rw[N2 + c], rw[2 * N2 + c], rw[3 * N2 + c]);
+        if (c < NG) {
+            const int sd = c / N;
+            const int nbb = sd == 0 ? nbw[1] : sd == 1 ? nbw[3] : sd == 2 ? nbw[5] : nbw[7];
+            if (nbb >= 0)
+                store_rec<N, 1>(lds, 0, TL::SECOND + c, gw[NFLD * NG + c], gw[NFLD * NG + NG + c],
+                                gw[NFLD * NG + 2 * NG + c], gw[NFLD * NG + 3 * NG + c]);
+            store_rec<N, 1>(lds, 0, TL::GHOST + c, gw[c], gw[NG + c], gw[2 * NG + c], gw[3 * NG + c]);
+        }
+        // patch parameters behind the instance id
+        const double hw = ldexp(ipc[inw].h_base, -levw) / (double)N;
+        double dtw = 0.0, tw = 0.0;
+        if (a.role & ROLE_DT) {
+            double dt = ipc[inw].dt_cap;
+            if (a.use_cfl) {
+                double m = INFINITY;
+#pragma unroll
+                for (int k = 0; k < (NSLOT + 63) / 64; ++k) m = fmin(m, bits2d(slv[k]));
+                m = wave_min(m);
+                if (m < dt) dt = m;
+            }
+            const double ti = ((const cdbl4 *)a.t_inst)[inw];
+            const double t_end = ipc[inw].t_end;
+            if (t_end > 0.0) {
+                double rem = t_end - ti;
+                if (rem < 0.0) rem = 0.0;
+                if (rem < dt) dt = rem;
+            }
+            dtw = dt;
+            tw = ti;
+            if (c == 0 && firstw) {          // one writer per instance
+                a.dt_cur[inw] = dt;
+                a.t_step[inw] = ti;
+            }
+        } else {
+            dtw = a.mode != MODE_RHS ? ((const cdbl4 *)a.dt_cur)[inw] : 0.0;
+            tw = ((const cdbl4 *)a.m.t_step)[inw];
+        }
+        // the next patch's loads go out now, into the other buffer (read
+        // last by the patch before this one, which every wave has finished)
+        if (more) {
+            const bool hangn = nbn[1] >= 0 || nbn[3] >= 0 || nbn[5] >= 0 || nbn[7] >= 0;
+            prefetch_patch<N>(a.Us, q + 1, nbn, hangn, raw[buf ^ 1], gh[buf ^ 1], tid);
+        }
+        __syncthreads();
+
+        // ---- face fluxes (stage_kernel DESIGN 1, one patch) ----------------
+        double FA[4], FB[4], FC[4] = {0, 0, 0, 0};
+        int dC = -1, fnC = 1;
+        {
+            const double h = hw;
+            const int na1 = nbw[2], nb1 = nbw[3], na3 = nbw[6], nb3 = nbw[7];
+            const bool ex = (ci == N - 1), ey = (cj == N - 1);
+            const double hx = (ex && nb1 >= 0) ? 0.5 * h : h;
+            const double hy = (ey && nb3 >= 0) ? 0.5 * h : h;
+            Prim L, RX, RY;
+            load_prim<N, 1>(lds, 0, c, L);
+            load_prim<N, 1>(lds, 0, ex ? TL::GHOST + 1 * N + cj : c + 1, RX);
+            load_prim<N, 1>(lds, 0, ey ? TL::GHOST + 3 * N + ci : c + N, RY);
+            flux_face<true>(L, RX, hx, FA);
+            flux_face<false>(L, RY, hy, FB);
+            if (ex && (na1 < 0 || nb1 >= 0)) {
+                if (na1 < 0) {
+                    const double nrm[2] = {h, 0.0};
+                    const double qq[4] = {L.r, L.ru, L.rv, L.rE};
+                    flux_boundary(a.m.ip[inw], 1, nrm, qq, a.m.org + 2 * (size_t)q, (double)N, cj + 0.5, h, tw, FA);
+                } else {
+                    Prim F2;
+                    load_prim<N, 1>(lds, 0, TL::SECOND + 1 * N + cj, F2);
+                    double G[4];
+                    flux_face<true>(L, F2, hx, G);
+                    FA[0] += G[0]; FA[1] += G[1]; FA[2] += G[2]; FA[3] += G[3];
+                }
+            }
+            if (ey && (na3 < 0 || nb3 >= 0)) {
+                if (na3 < 0) {
+                    const double nrm[2] = {0.0, h};
+                    const double qq[4] = {L.r, L.ru, L.rv, L.rE};
+                    flux_boundary(a.m.ip[inw], 3, nrm, qq, a.m.org + 2 * (size_t)q, ci + 0.5, (double)N, h, tw, FB);
+                } else {
+                    Prim F2;
+                    load_prim<N, 1>(lds, 0, TL::SECOND + 3 * N + ci, F2);
+                    double G[4];
+                    flux_face<false>(L, F2, hy, G);
+                    FB[0] += G[0]; FB[1] += G[1]; FB[2] += G[2]; FB[3] += G[3];
+                }
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        {
+            // the patch-edge round on one wave, rotating with the patch
+            const int et = tid - (2 * N <= 64 ? ((lb + q) % NW) * 64 : 0);
+            if (et >= 0 && et < 2 * N) {
+                const double h = hw;
+                const bool xs = et < N;
+                const int k = xs ? et : et - N, side = xs ? 0 : 2;
+                const int na = xs ? nbw[0] : nbw[4], nb = xs ? nbw[1] : nbw[5];
+                const double hf = nb >= 0 ? 0.5 * h : h;
+                const int tl = TL::GHOST + side * N + k;
+                const int tr = xs ? k * N : k;
+                const int offn = xs ? TL::NPOS : 2 * TL::NPOS, offt = 3 * TL::NPOS - offn;
+                OPrim L, R;
+                load_oprim<N, 1>(lds, 0, tl, offn, offt, L);
+                load_oprim<N, 1>(lds, 0, tr, offn, offt, R);
+                flux_normal(L, R, hf, FC);
+                if (na < 0) {
+                    const double nrm[2] = {xs ? -h : 0.0, xs ? 0.0 : -h};
+                    const double qq[4] = {R.r, xs ? R.rn : R.rt, xs ? R.rt : R.rn, R.rE};
+                    double F[4];
+                    flux_boundary(a.m.ip[inw], side, nrm, qq, a.m.org + 2 * (size_t)q, xs ? 0.0 : k + 0.5,
+                                  xs ? k + 0.5 : 0.0, h, tw, F);
+                    FC[0] = -F[0]; FC[1] = -(xs ? F[1] : F[2]); FC[2] = -(xs ? F[2] : F[1]); FC[3] = -F[3];
+                } else if (nb >= 0) {
+                    OPrim F2;
+                    load_oprim<N, 1>(lds, 0, TL::SECOND + side * N + k, offn, offt, F2);
+                    double G[4];
+                    flux_normal(F2, R, hf, G);
+                    FC[0] += G[0]; FC[1] += G[1]; FC[2] += G[2]; FC[3] += G[3];
+                }
+                dC = xs ? k * (N + 1) : NFLD * N * (N + 1) + k;
+                fnC = xs ? 1 : 2;
+            }
+        }
+        __syncthreads();   // all reads of the record tile are done
+        const int stride = N * (N + 1);
+        {
+            const int dA = cj * (N + 1) + ci + 1;
+            const int dB = NFLD * N * (N + 1) + (cj + 1) * N + ci;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                lds[dA + f * stride] = FA[f];
+                lds[dB + f * stride] = FB[f];
+            }
+        }
+        if (dC >= 0) {
+            lds[dC] = FC[0];
+            lds[dC + fnC * stride] = FC[1];
+            lds[dC + (3 - fnC) * stride] = FC[2];
+            lds[dC + 3 * stride] = FC[3];
+        }
+        __syncthreads();
+
+        // ---- per-cell residual and stage update ---------------------------
+        double U[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) U[f] = rw[f * N2 + c];
+        double k4[4], out[4];
+        const double ninvV = ldexp(ipc[inw].ninv_vol0, 2 * levw);
+        const double cflh = ldexp(ipc[inw].cfl_h0, -levw);
+        {
+            const double *FX = lds;
+            const double *FY = FX + NFLD * N * (N + 1);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                double Rf = (FX[f * stride + cj * (N + 1) + ci + 1] - FX[f * stride + cj * (N + 1) + ci]) +
+                            (FY[f * stride + (cj + 1) * N + ci] - FY[f * stride + cj * N + ci]);
+                k4[f] = Rf * ninvV;
+                sq += k4[f] * k4[f];
+            }
+        }
+        const double dt = a.mode != MODE_RHS ? dtw : 0.0;
+        switch (a.mode) {
+        case MODE_SO:
+            if (a.U0) {
+#pragma unroll
+                for (int f = 0; f < 4; ++f) out[f] = a.c0 * B[f] + a.c1 * U[f] + (a.c2 * dt) * k4[f];
+            } else {
+#pragma unroll
+                for (int f = 0; f < 4; ++f) out[f] = a.c1 * U[f] + (a.c2 * dt) * k4[f];
+            }
+            break;
+        case MODE_RK4_ACC:
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                double acc = a.first ? k4[f] : a.Acc[base + f * N2] + 2.0 * k4[f];
+                a.Acc[base + f * N2] = acc;
+                out[f] = B[f] + (a.c2 * dt) * k4[f];
+            }
+            break;
+        case MODE_RK4_FINAL:
+#pragma unroll
+            for (int f = 0; f < 4; ++f) out[f] = B[f] + (dt / 6.0) * (a.Acc[base + f * N2] + k4[f]);
+            break;
+        case MODE_LSERK:
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                double dq = a.first ? dt * k4[f] : a.Acc[base + f * N2] * a.c1 + dt * k4[f];
+                a.Acc[base + f * N2] = dq;
+                out[f] = U[f] + a.c2 * dq;
+            }
+            break;
+        default:   // MODE_RHS
+#pragma unroll
+            for (int f = 0; f < 4; ++f) out[f] = k4[f];
+            break;
+        }
+        if (a.mode != MODE_RHS && dt == 0.0) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) out[f] = U[f];
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f) __builtin_nontemporal_store(out[f], &a.Uout[base + f * N2]);
+        if (a.role & ROLE_CFL) {
+            // the running minimum of this wave, folded into the instance's
+            // sub-slot when the instance changes (uniform across the block)
+            if (inw != cinst) {
+                if (cinst >= 0) cfl_reduce<N2>(cmin, cinst, true, cinst, a.slot, red);
+                cinst = inw;
+                cmin = INFINITY;
+            }
+            cmin = fmin(cmin, wave_min(cfl_candidate_h(out, cflh)));
+        }
+        // the next patch's ids
+#pragma unroll
+        for (int k = 0; k < 8; ++k) nbw[k] = nbn[k];
+    }
+    if ((a.role & ROLE_CFL) && cinst >= 0) cfl_reduce<N2>(cmin, cinst, true, cinst, a.slot, red);
+    if (a.norm_part) sum_reduce_store<N2>(sq, &a.norm_part[blockIdx.x], red);
+}
+
 // Standalone CFL minimum (physics_time_step): one thread per cell.
 template <int N, int PPB>
 __global__ __launch_bounds__((Tile<N, PPB>::BLK)) void cfl_kernel(MeshDev m, const double *U,
@@ -1345,6 +1713,7 @@ struct amrgpu_ctx {
     std::vector<InstDev> h_ip;
     int *h_pin = nullptr;      // pinned staging of the mesh uploads (ints)
     size_t h_pin_cap = 0;
+    int cu_count = 256;        // compute units of the context's device
 };
 
 namespace {
@@ -1370,8 +1739,35 @@ const TileVariant kVariants[] = {
     {8, 2, 1, 0}, {8, 4, 1, 0}, {8, 4, 4, 0}, {8, 2, 1, 1}, {8, 4, 1, 1}, {8, 4, 4, 1},
     {16, 1, 1, 0}, {16, 1, 4, 0}, {16, 1, 1, 1}, {16, 1, 4, 1},
     {8, 1, 1, 1}, {8, 1, 4, 1}, {8, 2, 4, 1}, {8, 1, 5, 1},
+    {16, 1, 3, 2},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+// Persistent design: the workgroups resident at once (LDS bounds a 16x16
+// workgroup to 3 per CU, registers to MINW), each given a contiguous range
+// of patches.
+constexpr int PERSIST_WG_PER_CU = 3;
+
+void persistent_grid(const amrgpu_ctx *c, int nquad, int minw, int *nblk, int *per)
+{
+    const int gmax = std::max(8, c->cu_count * std::min(minw, PERSIST_WG_PER_CU));
+    const int p = std::max(1, (nquad + gmax - 1) / gmax);
+    *per = p;
+    *nblk = (nquad + p - 1) / p;
+}
+
+// Workgroups of one stage launch for tile variant v (the residual-norm
+// partials are one per workgroup).
+int grid_of(const amrgpu_ctx *c, int v, int nquad)
+{
+    if (kVariants[v].design == 2) {
+        int nblk, per;
+        persistent_grid(c, nquad, kVariants[v].minw, &nblk, &per);
+        return nblk;
+    }
+    return (nquad + kVariants[v].ppb - 1) / kVariants[v].ppb;
+}
+
 
 // Stage-kernel tile used until amrgpu_set_tile / amrgpu_autotune picks one;
 // for n = 8 the paired-faces tile with two patches per workgroup, the
@@ -1398,6 +1794,18 @@ int ppb_of(int n)
     case 16: return 1;
     default: return 0;
     }
+}
+
+template <int N, int MINW>
+int launch_pp(amrgpu_ctx *c, const StageArgs &a0)
+{
+    StageArgs a = a0;
+    int nblk, per;
+    persistent_grid(c, c->nquad, MINW, &nblk, &per);
+    a.per_block = per;
+    hipLaunchKernelGGL((stage_kernel_pp<N, MINW>), dim3(nblk), dim3(N * N), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return AMRGPU_OK;
 }
 
 template <int N, int PPB, int MINW, int D>
@@ -1430,6 +1838,7 @@ int launch_stage(amrgpu_ctx *c, const StageArgs &a)
     case 15: return launch_v<8, 1, 4, 1>(c, a);
     case 16: return launch_v<8, 2, 4, 1>(c, a);
     case 17: return launch_v<8, 1, 5, 1>(c, a);
+    case 18: return launch_pp<16, 3>(c, a);
     default: set_err("bad tile variant %d", c->variant); return AMRGPU_ERR_ARG;
     }
 }
@@ -1644,6 +2053,10 @@ const char *amrgpu_last_error(void) { return g_err; }
 amrgpu_ctx *amrgpu_create(void)
 {
     amrgpu_ctx *c = new amrgpu_ctx();
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+        c->cu_count = ncu;
     const char *ge = getenv("AMRGPU_GRAPHS");
     if (ge) c->graphs_enabled = ge[0] == '1';
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1779,7 +2192,7 @@ static int upload_mesh(amrgpu_ctx *c, int ncell, const int *level, const int *ce
     else memset(pq, 0, sizeof(int) * (size_t)ncell);
     c->nquad = ncell;
     if (c->variant < 0 || kVariants[c->variant].n != c->n) c->variant = default_variant(c->n);
-    c->nblocks = (ncell + kVariants[c->variant].ppb - 1) / kVariants[c->variant].ppb;
+    c->nblocks = grid_of(c, c->variant, ncell);
     if (c->nblocks > c->cap_blocks || (size_t)ncell * 4 * c->n * c->n > c->cap_state) {
         set_err("internal capacity error");
         return AMRGPU_ERR_ARG;
@@ -2357,7 +2770,7 @@ int amrgpu_set_tile(amrgpu_ctx *c, int ppb, int minw, int design)
             HIPCHK(hipStreamSynchronize(c->stream));
             free_graphs(c);
             c->variant = v;
-            c->nblocks = (c->nquad + ppb - 1) / ppb;
+            c->nblocks = grid_of(c, v, c->nquad);
             return AMRGPU_OK;
         }
     }

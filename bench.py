@@ -313,6 +313,21 @@ def equal_l1_plain(L, args, lmax, target, ref, interval=10, lag=0, log_fn=log):
     return best, trail
 
 
+def interpolated_wall(trail, target):
+    """wall time at the target L1, linear between the closest runs on either
+    side of it (L1 is not monotone in the tolerance at the 1 % level, so the
+    bisection may not land within 1 %); None without a run on each side"""
+    below = [t for t in trail if t["l1"] <= target]
+    above = [t for t in trail if t["l1"] > target]
+    if not below or not above:
+        return None
+    a = max(below, key=lambda t: t["l1"])
+    b = min(above, key=lambda t: t["l1"])
+    w = (target - a["l1"]) / (b["l1"] - a["l1"])
+    return {"wall_s": a["wall_s"] + w * (b["wall_s"] - a["wall_s"]), "between_tol": [a["tol"], b["tol"]],
+            "between_l1": [a["l1"], b["l1"]]}
+
+
 def learned_experiment(amrhip, args):
     """plain indicator every 10 steps vs the learned tagger (with the plain
     indicator as fallback) every --learned-interval steps, on a seeded
@@ -366,10 +381,14 @@ def learned_experiment(amrhip, args):
     }
     if not args.no_equal_l1:
         eq, trail = equal_l1_plain(L, args, lmax, lr["l1_density"], ref)
-        out["plain_every_10_at_learned_l1"] = {"run": eq, "search": trail,
-                                               "l1_within_1pct": abs(eq["l1_density"] / lr["l1_density"] - 1)
-                                               <= 0.01,
-                                               "wall_time_ratio_plain_over_learned": eq["wall_s"] / lr["wall_s"]}
+        rec = {"run": eq, "search": trail,
+               "l1_within_1pct": abs(eq["l1_density"] / lr["l1_density"] - 1) <= 0.01,
+               "wall_time_ratio_plain_over_learned": eq["wall_s"] / lr["wall_s"]}
+        iw = interpolated_wall(trail, lr["l1_density"])
+        if iw is not None:
+            iw["wall_time_ratio_plain_over_learned"] = iw["wall_s"] / lr["wall_s"]
+            rec["interpolated_at_learned_l1"] = iw
+        out["plain_every_10_at_learned_l1"] = rec
         log(f"equal L1: plain every 10 at tol {eq['tol']:.4f}: L1 {eq['l1_density']:.5e}, {eq['wall_s']:.2f} s "
             f"vs learned {lr['wall_s']:.2f} s ({eq['wall_s'] / lr['wall_s']:.2f}x)")
     return out, m
@@ -629,7 +648,7 @@ def main():
             "quads": nq, "cells": cells, "patch": n,
             "levels": [int(lv.min()), int(lv.max())], "parallelism": "1 GPU, every instance in one launch",
             "tile": {"patches_per_workgroup": tile[0], "min_waves_per_simd": tile[1],
-                     "design": ["face task list", "paired faces"][tile[2]],
+                     "design": ["face task list", "paired faces", "persistent prefetch"][tile[2]],
                      "autotuned": tune is not None, "launch": launch},
         },
         "roofline": roof,

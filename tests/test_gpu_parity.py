@@ -632,3 +632,53 @@ def test_rhs_after_regrow_remesh_keeps_step_time(oracle):
         assert g.time()[0][0] == t
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("scheme", [3, 4, 5])
+def test_persistent_design_equals_paired_faces(oracle, scheme):
+    """the persistent, prefetching 16x16 kernel (several patches per
+    workgroup, the next patch's state and halo loaded into LDS while the
+    current one is evaluated) against the one-patch-per-workgroup paired-face
+    kernel it restates, on a batch of three instances (adaptive forests with
+    hanging faces and walls, and a periodic one) large enough that every
+    workgroup walks several patches: residual and five steps to the parity
+    bars (the source arithmetic is the same, but each kernel is compiled on
+    its own and may contract a*b + c differently; measured: bit for bit),
+    and the residual of each instance against the oracle"""
+    n = 16
+    forests = [riemann_forest(n, seed=21, lmin=3, lmax=7), riemann_forest(n, seed=22, lmin=3, lmax=7),
+               vortex_forest(n, 5)]
+    insts = [amrhip.make_inst(1.0, (2, 1, 0, 1), 0.3, 1.0, 0.0), amrhip.make_inst(1.0, (0, 0, 1, 1), 0.25, 1.0, 0.0),
+             amrhip.make_inst(10.0, (0, 0, 0, 0), 0.3, 1.0, 0.0)]
+    arrays = amrhip.batch_external(forests)
+    nq = arrays[0].size // 3
+    log(f"persistent vs paired: scheme {scheme}, {nq} quads")
+    assert nq > 3 * 768      # several patches per resident workgroup (768 of them on 256 CUs)
+    out = {}
+    for tile in [(1, 4, 1)] + [v for v in amrhip.tile_variants(n) if v[2] >= 2]:
+        g = amrhip.Gpu()
+        g.setup(n, scheme, insts, *arrays[:4], arrays[4])
+        g.set_tile(*tile)
+        r = g.rhs()
+        g.run(5)
+        out[tile] = (r, g.soln(), g.time())
+        g.close()
+    ref = out[(1, 4, 1)]
+    assert len(out) >= 2
+    # the residual of each instance against the oracle's
+    exp, off = [], 0
+    for f, ip in zip(forests, insts):
+        ci, fi, geom, soln = f.external()
+        p = oracle.params(n, bc=tuple(ip.bc), cfl=ip.cfl, dt=1.0, h_base=ip.h_base, scheme=scheme)
+        exp.append(oracle.rhs(p, ci, fi, geom, soln))
+    for tile, (r, *_rest) in out.items():
+        off = 0
+        for e in exp:
+            assert parity_err(r[off:off + e.size], e) <= RHS_TOL, tile
+            off += e.size
+    for tile, (r, q, (t, dt)) in out.items():
+        er, eq = parity_err(r, ref[0]), parity_err(q, ref[1])
+        log(f"  tile {tile}: residual {er:.2e}, state after 5 steps {eq:.2e}")
+        assert er <= RHS_TOL, tile
+        assert eq <= TRAJ_TOL, tile
+        assert np.allclose(t, ref[2][0], rtol=1e-12, atol=0) and np.allclose(dt, ref[2][1], rtol=1e-12, atol=0), tile
