@@ -464,10 +464,16 @@ struct Tile {
     static constexpr int FLUX_PER_PATCH = NFLD * 2 * N * (N + 1);
     static constexpr int LDS_DOUBLES =
         PPB * (PRIM_PER_PATCH > FLUX_PER_PATCH ? PRIM_PER_PATCH : FLUX_PER_PATCH);
+    // the second-neighbour fluxes of each patch side (4 sides x
+    // 4 fields x N) after the face arrays, and the rounds of extra tasks
+    // (at most 2N edge faces + 4N second fluxes per patch)
+    static constexpr int F2BASE = PPB * FLUX_PER_PATCH;
+    static constexpr int XROUNDS = (PPB * 6 * N + BLK - 1) / BLK;
     static constexpr int NINT = 2 * N * (N - 1);           // interior faces per patch
     static constexpr int NEDGE = 4 * N;                    // patch-edge faces per patch
     static constexpr int NTASK = PPB * (NINT + NEDGE);
     static constexpr int ROUNDS = (NTASK + BLK - 1) / BLK;
+    static_assert(F2BASE + PPB * 4 * NFLD * N <= LDS_DOUBLES, "second-neighbour fluxes fit beside the faces");
 };
 
 template <int N, int PPB>
@@ -743,10 +749,14 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     // ---- face fluxes into registers --------------------------------------
     if constexpr (DESIGN == 1) {
         // paired faces: every cell evaluates its +x and +y faces from one
-        // load of its own record (two independent flux chains per thread);
-        // one more round covers the -x and -y patch edges
-        double FA[4] = {0, 0, 0, 0}, FB[4] = {0, 0, 0, 0}, FC[4] = {0, 0, 0, 0};
-        int dC = -1, fnC = 1;     // FC: normal-frame flux; fnC its normal-momentum field
+        // load of its own record (two independent flux chains per thread).
+        // Every other face evaluation of the workgroup -- the -x and -y
+        // patch edges, and the second fine neighbour of each side with two
+        // -- is a task of one more round, packed onto as few waves as they
+        // fill (a patch's extra fluxes would otherwise cost its wave a round
+        // per kind); the epilogue adds each second-neighbour flux to its
+        // first, in the same order as a single thread would.
+        double FA[4] = {0, 0, 0, 0}, FB[4] = {0, 0, 0, 0};
         if (active) {
             const double h = UNI ? hw : ph[p];
             const int na1 = pnbr[p][2], nb1 = pnbr[p][3], na3 = pnbr[p][6], nb3 = pnbr[p][7];
@@ -759,77 +769,94 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             load_prim<N, PPB>(lds, p, ey ? TL::GHOST + 3 * N + ci : c + N, RY);
             flux_face<true>(L, RX, hx, FA);
             flux_face<false>(L, RY, hy, FB);
-            if (ex && (na1 < 0 || nb1 >= 0)) {
-                if (na1 < 0) {
-                    const double nrm[2] = {h, 0.0};
-                    const double q[4] = {L.r, L.ru, L.rv, L.rE};
-                    flux_boundary(a.m.ip[pinst[p]], 1, nrm, q, a.m.org + 2 * (size_t)quad, (double)N,
-                                  cj + 0.5, h, ptime[p], FA);
-                } else {
-                    Prim F2;
-                    load_prim<N, PPB>(lds, p, TL::SECOND + 1 * N + cj, F2);
-                    double G[4];
-                    flux_face<true>(L, F2, hx, G);
-                    FA[0] += G[0]; FA[1] += G[1]; FA[2] += G[2]; FA[3] += G[3];
-                }
+            if (ex && na1 < 0) {
+                const double nrm[2] = {h, 0.0};
+                const double q[4] = {L.r, L.ru, L.rv, L.rE};
+                flux_boundary(a.m.ip[pinst[p]], 1, nrm, q, a.m.org + 2 * (size_t)quad, (double)N,
+                              cj + 0.5, h, ptime[p], FA);
             }
-            if (ey && (na3 < 0 || nb3 >= 0)) {
-                if (na3 < 0) {
-                    const double nrm[2] = {0.0, h};
-                    const double q[4] = {L.r, L.ru, L.rv, L.rE};
-                    flux_boundary(a.m.ip[pinst[p]], 3, nrm, q, a.m.org + 2 * (size_t)quad, ci + 0.5,
-                                  (double)N, h, ptime[p], FB);
-                } else {
-                    Prim F2;
-                    load_prim<N, PPB>(lds, p, TL::SECOND + 3 * N + ci, F2);
-                    double G[4];
-                    flux_face<false>(L, F2, hy, G);
-                    FB[0] += G[0]; FB[1] += G[1]; FB[2] += G[2]; FB[3] += G[3];
-                }
+            if (ey && na3 < 0) {
+                const double nrm[2] = {0.0, h};
+                const double q[4] = {L.r, L.ru, L.rv, L.rE};
+                flux_boundary(a.m.ip[pinst[p]], 3, nrm, q, a.m.org + 2 * (size_t)quad, ci + 0.5,
+                              (double)N, h, ptime[p], FB);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
-        // the wave that takes the patch-edge round rotates with the block
-        // index, spreading its extra round over the SIMDs
+        // the tasks start on a wave that rotates with the block index,
+        // spreading the extra round over the SIMDs; when the halo is wave
+        // 0's alone (one patch of several waves), on one of the others
         constexpr int NWV = (TL::BLK + 63) / 64;
-        const int et = tid - (PPB * 2 * N <= 64 ? (lb % NWV) * 64 : 0);
-        if (et >= 0 && et < PPB * 2 * N) {
-            const int tp = et / (2 * N), e = et % (2 * N);
-            if (pquad[tp] >= 0) {
-                const double h = ph[tp];
-                const bool xs = e < N;                 // side 0 (x) or side 2 (y)
-                const int k = xs ? e : e - N, side = xs ? 0 : 2;
-                const int na = pnbr[tp][2 * side], nb = pnbr[tp][2 * side + 1];
-                const double hf = nb >= 0 ? 0.5 * h : h;
-                const int tl = TL::GHOST + side * N + k;   // ghost of side 0 or 2
-                const int tr = xs ? k * N : k;             // first cell of row / column k
-                // x and y faces mixed in one wave: the records are read
-                // oriented along each lane's normal by choosing the LDS
-                // arrays (the normal momentum is rho u on x faces, rho v on
-                // y faces), and the flux is stored back the same way, in
-                // place of selecting every value
-                const int offn = xs ? TL::NPOS : 2 * TL::NPOS, offt = 3 * TL::NPOS - offn;
-                OPrim L, R;
-                load_oprim<N, PPB>(lds, tp, tl, offn, offt, L);
-                load_oprim<N, PPB>(lds, tp, tr, offn, offt, R);
-                flux_normal(L, R, hf, FC);
-                if (na < 0) {
-                    const double nrm[2] = {xs ? -h : 0.0, xs ? 0.0 : -h};
-                    const double q[4] = {R.r, xs ? R.rn : R.rt, xs ? R.rt : R.rn, R.rE};
-                    double F[4];
-                    flux_boundary(a.m.ip[pinst[tp]], side, nrm, q, a.m.org + 2 * (size_t)pquad[tp],
-                                  xs ? 0.0 : k + 0.5, xs ? k + 0.5 : 0.0, h, ptime[tp], F);
-                    FC[0] = -F[0]; FC[1] = -(xs ? F[1] : F[2]); FC[2] = -(xs ? F[2] : F[1]); FC[3] = -F[3];
-                } else if (nb >= 0) {
-                    OPrim F2;
-                    load_oprim<N, PPB>(lds, tp, TL::SECOND + side * N + k, offn, offt, F2);
-                    double G[4];
-                    flux_normal(F2, R, hf, G);
-                    FC[0] += G[0]; FC[1] += G[1]; FC[2] += G[2]; FC[3] += G[3];
-                }
-                dC = tp * TL::FLUX_PER_PATCH + (xs ? k * (N + 1) : NFLD * N * (N + 1) + k);
-                fnC = xs ? 1 : 2;
+        constexpr bool HALO0 = (PPB == 1 && NWV > 1 && 4 * N <= 64);
+        const int ew = HALO0 ? 1 + lb % (NWV - 1) : lb % NWV;
+        const int t0 = (tid - ew * 64 + TL::BLK) % TL::BLK;
+        double FC[TL::XROUNDS][4];
+        int dC[TL::XROUNDS], fnC[TL::XROUNDS];
+#pragma unroll
+        for (int r = 0; r < TL::XROUNDS; ++r) {
+            dC[r] = -1;
+            fnC[r] = 1;
+            FC[r][0] = FC[r][1] = FC[r][2] = FC[r][3] = 0.0;
+            int t = t0 + r * TL::BLK;
+            // patch of task t: 2N edge faces, then N per side with two
+            // neighbours, in the order +x, +y, -x, -y
+            int tp = -1;
+            for (int q = 0; q < PPB; ++q) {
+                if (pquad[q] < 0) break;
+                const int m = (pnbr[q][1] >= 0) + (pnbr[q][3] >= 0) + (pnbr[q][5] >= 0) + (pnbr[q][7] >= 0);
+                const int Tq = 2 * N + N * m;
+                if (t < Tq) { tp = q; break; }
+                t -= Tq;
             }
+            if (tp < 0) continue;
+            const int kind = t / N, k = t % N;
+            const double h = ph[tp];
+            bool xs, second, wall = false;
+            int side, tl, tr;
+            if (kind < 2) {                          // patch edge, first (or only) neighbour
+                side = kind == 0 ? 0 : 2;
+                xs = kind == 0;
+                second = false;
+                tl = TL::GHOST + side * N + k;
+                tr = xs ? k * N : k;
+                wall = pnbr[tp][2 * side] < 0;
+            } else {                                 // second fine neighbour of the (kind-2)-th side with two
+                side = -1;
+                int cnt = kind - 2;
+#pragma unroll
+                for (int o = 0; o < 4; ++o) {
+                    const int so = o == 0 ? 1 : o == 1 ? 3 : o == 2 ? 0 : 2;     // +x, +y, -x, -y
+                    if (pnbr[tp][2 * so + 1] >= 0) {
+                        if (cnt == 0 && side < 0) side = so;
+                        --cnt;
+                    }
+                }
+                xs = side < 2;
+                second = true;
+                const int own = side == 0 ? k * N : side == 1 ? k * N + (N - 1) : side == 2 ? k : (N - 1) * N + k;
+                const int sec = TL::SECOND + side * N + k;
+                const bool low = (side == 0 || side == 2);    // the neighbour is on the low side
+                tl = low ? sec : own;
+                tr = low ? own : sec;
+            }
+            const double hf = pnbr[tp][2 * side + 1] >= 0 ? 0.5 * h : h;
+            const int offn = xs ? TL::NPOS : 2 * TL::NPOS, offt = 3 * TL::NPOS - offn;
+            OPrim L, R;
+            load_oprim<N, PPB>(lds, tp, tl, offn, offt, L);
+            load_oprim<N, PPB>(lds, tp, tr, offn, offt, R);
+            flux_normal(L, R, hf, FC[r]);
+            if (wall) {
+                const double nrm[2] = {xs ? -h : 0.0, xs ? 0.0 : -h};
+                const double q[4] = {R.r, xs ? R.rn : R.rt, xs ? R.rt : R.rn, R.rE};
+                double F[4];
+                flux_boundary(a.m.ip[pinst[tp]], side, nrm, q, a.m.org + 2 * (size_t)pquad[tp],
+                              xs ? 0.0 : k + 0.5, xs ? k + 0.5 : 0.0, h, ptime[tp], F);
+                FC[r][0] = -F[0]; FC[r][1] = -(xs ? F[1] : F[2]); FC[r][2] = -(xs ? F[2] : F[1]);
+                FC[r][3] = -F[3];
+            }
+            dC[r] = second ? TL::F2BASE + ((tp * 4 + side) * NFLD) * N + k
+                           : tp * TL::FLUX_PER_PATCH + (xs ? k * (N + 1) : NFLD * N * (N + 1) + k);
+            fnC[r] = xs ? 1 : 2;
         }
         if (active && needB) {   // base state: loaded late, its latency overlaps the flux gather
 #pragma unroll
@@ -846,11 +873,16 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 lds[dB + f * stride] = FB[f];
             }
         }
-        if (dC >= 0) {
-            lds[dC] = FC[0];
-            lds[dC + fnC * stride] = FC[1];
-            lds[dC + (3 - fnC) * stride] = FC[2];
-            lds[dC + 3 * stride] = FC[3];
+#pragma unroll
+        for (int r = 0; r < TL::XROUNDS; ++r) {
+            if (dC[r] < 0) continue;
+            // first fluxes in the face arrays (field stride N (N + 1)), second
+            // ones in the side arrays (field stride N)
+            const int fs = dC[r] >= TL::F2BASE ? N : stride;
+            lds[dC[r]] = FC[r][0];
+            lds[dC[r] + fnC[r] * fs] = FC[r][1];
+            lds[dC[r] + (3 - fnC[r]) * fs] = FC[r][2];
+            lds[dC[r] + 3 * fs] = FC[r][3];
         }
         __syncthreads();
     } else {
@@ -992,10 +1024,19 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         const double *FX = lds + p * TL::FLUX_PER_PATCH;
         const double *FY = FX + NFLD * N * (N + 1);
         const int stride = N * (N + 1);
+        // second fine neighbours' fluxes on the patch edges
+        const double *F2 = lds + TL::F2BASE + p * 4 * NFLD * N;
+        const bool s0 = DESIGN == 1 && ci == 0 && pnbr[p][1] >= 0, s1 = DESIGN == 1 && ci == N - 1 && pnbr[p][3] >= 0;
+        const bool s2 = DESIGN == 1 && cj == 0 && pnbr[p][5] >= 0, s3 = DESIGN == 1 && cj == N - 1 && pnbr[p][7] >= 0;
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
-            double Rf = (FX[f * stride + cj * (N + 1) + ci + 1] - FX[f * stride + cj * (N + 1) + ci]) +
-                        (FY[f * stride + (cj + 1) * N + ci] - FY[f * stride + cj * N + ci]);
+            double fxr = FX[f * stride + cj * (N + 1) + ci + 1], fxl = FX[f * stride + cj * (N + 1) + ci];
+            double fyt = FY[f * stride + (cj + 1) * N + ci], fyb = FY[f * stride + cj * N + ci];
+            if (s1) fxr += F2[(1 * NFLD + f) * N + cj];
+            if (s0) fxl += F2[(0 * NFLD + f) * N + cj];
+            if (s3) fyt += F2[(3 * NFLD + f) * N + ci];
+            if (s2) fyb += F2[(2 * NFLD + f) * N + ci];
+            double Rf = (fxr - fxl) + (fyt - fyb);
             k4[f] = Rf * ninvV;
             sq += k4[f] * k4[f];
         }
@@ -1302,8 +1343,9 @@ __global__ __launch_bounds__(N * N, MINW) void stage_kernel_pp(StageArgs a)
         }
         __builtin_amdgcn_sched_barrier(0);
         {
-            // the patch-edge round on one wave, rotating with the patch
-            const int et = tid - (2 * N <= 64 ? ((lb + q) % NW) * 64 : 0);
+            // the patch-edge round on one wave other than the halo's wave 0,
+            // rotating with the patch
+            const int et = tid - (2 * N <= 64 ? (1 + (lb + q) % (NW - 1)) * 64 : 0);
             if (et >= 0 && et < 2 * N) {
                 const double h = hw;
                 const bool xs = et < N;
