@@ -545,7 +545,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     const int ci = c % N, cj = c / N;
     const int quad = lb * PPB + p;
     const bool active = quad < a.m.nquad;
-    const size_t base = (size_t)quad * NFLD * N2 + c;
+    // threads past the last patch read patch 0 (nquad >= 1 when launched)
+    // and store nothing, so that the loads need no branch
+    const int quadc = active ? quad : 0;
+    const size_t base = (size_t)quadc * NFLD * N2 + c;
 
     // ---- prologue -----------------------------------------------------------
     // The own-state loads and the halo loads go out first; the waits that
@@ -557,13 +560,14 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     // on their own counter, not behind the vector loads.
     constexpr bool UNI = (N2 % 64 == 0);
     constexpr bool ONE_GHOST = (4 * N <= N2);     // at most one halo cell per thread
-    double U[4] = {1.0, 0.0, 0.0, 1.0};
-    double B[4] = {0.0, 0.0, 0.0, 0.0};     // base state U0 of the step
+    // Per-thread values below are left unset where only the paths that
+    // assign them read them (an inactive thread stores and reduces nothing):
+    // initialising them costs a move per register on every path.
+    double U[4];
+    double B[4];     // base state U0 of the step
     const bool needB = (a.mode == MODE_SO && a.U0) || a.mode == MODE_RK4_ACC || a.mode == MODE_RK4_FINAL;
-    if (active) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f) U[f] = a.Us[base + f * N2];
-    }
+    for (int f = 0; f < 4; ++f) U[f] = a.Us[base + f * N2];
     int nbw[8];
     int levw = 0, inw = 0;
     bool actw = false, firstw = false;   // firstw: the wave holds its instance's first quadrant
@@ -585,6 +589,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             inw = 0;
         }
     }
+    // whether the thread's patch exists, as a wave-uniform (scalar) value
+    // when a wave holds one patch: branches on it are scalar, and values
+    // assigned under it need no definition for threads that skip it
+    const bool actu = UNI ? actw : active;
     // first stage of a step: the CFL sub-slots of the patch's instance, read
     // with the halo; the step's dt is formed from them after the records
     unsigned long long slv[(NSLOT + 63) / 64];
@@ -601,18 +609,21 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     // second fine neighbour of a coarse side), loaded here and stored as
     // records after the parameter chains below when there is at most one
     // per thread
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    double g0, g1, g2, g3, s0, s1, s2, s3;
     bool gsec = false;
-    for (int hh = c; hh < 4 * N; hh += N2) {
-        if (!active) break;
+    // with at most one halo cell per thread, threads without one load side
+    // 0's first (unused): one pass, no branch
+    const int hh0 = ONE_GHOST ? (c < 4 * N ? c : 0) : c;
+    for (int hh = hh0, it = 0; ONE_GHOST ? it < 1 : hh < 4 * N; hh += N2, ++it) {
+        if (!ONE_GHOST && !active) break;
         const int sd = hh / N, k = hh % N;
         int na, nb;
         if constexpr (UNI) {
             na = sd == 0 ? nbw[0] : sd == 1 ? nbw[2] : sd == 2 ? nbw[4] : nbw[6];
             nb = sd == 0 ? nbw[1] : sd == 1 ? nbw[3] : sd == 2 ? nbw[5] : nbw[7];
         } else {
-            na = a.m.nbr[quad * 8 + 2 * sd];
-            nb = a.m.nbr[quad * 8 + 2 * sd + 1];
+            na = a.m.nbr[quadc * 8 + 2 * sd];
+            nb = a.m.nbr[quadc * 8 + 2 * sd + 1];
         }
         int ec, oc;   // neighbour's edge cell, own edge cell
         switch (sd) {
@@ -624,7 +635,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         // physical boundary: the ghost holds the own edge cell, so that the
         // generic face evaluation stays finite before the boundary flux
         // replaces it
-        const double *sa = na < 0 ? a.Us + (size_t)quad * NFLD * N2 + oc
+        const double *sa = na < 0 ? a.Us + (size_t)quadc * NFLD * N2 + oc
                                   : a.Us + (size_t)na * NFLD * N2 + ec;
         g0 = sa[0]; g1 = sa[N2]; g2 = sa[2 * N2]; g3 = sa[3 * N2];
         gsec = nb >= 0;
@@ -707,7 +718,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             store_rec<N, PPB>(lds, p, TL::GHOST + c, g0, g1, g2, g3);
         }
     }
-    if (active) {
+    if (actu) {
         store_rec<N, PPB>(lds, p, c, U[0], U[1], U[2], U[3]);
 #pragma unroll
         for (int f = 0; f < 4; ++f) ldsU[(p * NFLD + f) * N2 + c] = U[f];
@@ -756,8 +767,8 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         // fill (a patch's extra fluxes would otherwise cost its wave a round
         // per kind); the epilogue adds each second-neighbour flux to its
         // first, in the same order as a single thread would.
-        double FA[4] = {0, 0, 0, 0}, FB[4] = {0, 0, 0, 0};
-        if (active) {
+        double FA[4], FB[4];
+        if (actu) {
             const double h = UNI ? hw : ph[p];
             const int na1 = pnbr[p][2], nb1 = pnbr[p][3], na3 = pnbr[p][6], nb3 = pnbr[p][7];
             const bool ex = (ci == N - 1), ey = (cj == N - 1);
@@ -790,28 +801,49 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         constexpr bool HALO0 = (PPB == 1 && NWV > 1 && 4 * N <= 64);
         const int ew = HALO0 ? 1 + lb % (NWV - 1) : lb % NWV;
         const int t0 = (tid - ew * 64 + TL::BLK) % TL::BLK;
+        // one patch per workgroup held by whole waves: its neighbour ids are
+        // the scalars nbw, and the task list's shape (length, the order of
+        // the sides with two neighbours) is worked out once, in scalar code
+        constexpr bool SCALAR_TASKS = (PPB == 1 && UNI);
+        int T1 = 0, hs0 = 0, hs1 = 0, hs2 = 0, hs3 = 0;
+        if constexpr (SCALAR_TASKS) {
+            int m = 0;
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                const int so = o == 0 ? 1 : o == 1 ? 3 : o == 2 ? 0 : 2;     // +x, +y, -x, -y
+                if (nbw[2 * so + 1] >= 0) {
+                    if (m == 0) hs0 = so; else if (m == 1) hs1 = so; else if (m == 2) hs2 = so; else hs3 = so;
+                    ++m;
+                }
+            }
+            T1 = actw ? 2 * N + N * m : 0;
+        }
         double FC[TL::XROUNDS][4];
         int dC[TL::XROUNDS], fnC[TL::XROUNDS];
 #pragma unroll
         for (int r = 0; r < TL::XROUNDS; ++r) {
             dC[r] = -1;
             fnC[r] = 1;
-            FC[r][0] = FC[r][1] = FC[r][2] = FC[r][3] = 0.0;
             int t = t0 + r * TL::BLK;
             // patch of task t: 2N edge faces, then N per side with two
             // neighbours, in the order +x, +y, -x, -y
             int tp = -1;
-            for (int q = 0; q < PPB; ++q) {
-                if (pquad[q] < 0) break;
-                const int m = (pnbr[q][1] >= 0) + (pnbr[q][3] >= 0) + (pnbr[q][5] >= 0) + (pnbr[q][7] >= 0);
-                const int Tq = 2 * N + N * m;
-                if (t < Tq) { tp = q; break; }
-                t -= Tq;
+            if constexpr (SCALAR_TASKS) {
+                if (t < T1) tp = 0;
+            } else {
+                for (int q = 0; q < PPB; ++q) {
+                    if (pquad[q] < 0) break;
+                    const int m = (pnbr[q][1] >= 0) + (pnbr[q][3] >= 0) + (pnbr[q][5] >= 0) + (pnbr[q][7] >= 0);
+                    const int Tq = 2 * N + N * m;
+                    if (t < Tq) { tp = q; break; }
+                    t -= Tq;
+                }
             }
             if (tp < 0) continue;
             const int kind = t / N, k = t % N;
-            const double h = ph[tp];
+            const double h = SCALAR_TASKS ? hw : ph[tp];
             bool xs, second, wall = false;
+            bool two;     // the face's side has two (fine) neighbours: fine normal length
             int side, tl, tr;
             if (kind < 2) {                          // patch edge, first (or only) neighbour
                 side = kind == 0 ? 0 : 2;
@@ -819,16 +851,28 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 second = false;
                 tl = TL::GHOST + side * N + k;
                 tr = xs ? k * N : k;
-                wall = pnbr[tp][2 * side] < 0;
+                if constexpr (SCALAR_TASKS) {
+                    wall = kind == 0 ? nbw[0] < 0 : nbw[4] < 0;
+                    two = kind == 0 ? nbw[1] >= 0 : nbw[5] >= 0;
+                } else {
+                    wall = pnbr[tp][2 * side] < 0;
+                    two = pnbr[tp][2 * side + 1] >= 0;
+                }
             } else {                                 // second fine neighbour of the (kind-2)-th side with two
-                side = -1;
-                int cnt = kind - 2;
+                two = true;
+                if constexpr (SCALAR_TASKS) {
+                    const int cnt = kind - 2;
+                    side = cnt == 0 ? hs0 : cnt == 1 ? hs1 : cnt == 2 ? hs2 : hs3;
+                } else {
+                    side = -1;
+                    int cnt = kind - 2;
 #pragma unroll
-                for (int o = 0; o < 4; ++o) {
-                    const int so = o == 0 ? 1 : o == 1 ? 3 : o == 2 ? 0 : 2;     // +x, +y, -x, -y
-                    if (pnbr[tp][2 * so + 1] >= 0) {
-                        if (cnt == 0 && side < 0) side = so;
-                        --cnt;
+                    for (int o = 0; o < 4; ++o) {
+                        const int so = o == 0 ? 1 : o == 1 ? 3 : o == 2 ? 0 : 2;     // +x, +y, -x, -y
+                        if (pnbr[tp][2 * so + 1] >= 0) {
+                            if (cnt == 0 && side < 0) side = so;
+                            --cnt;
+                        }
                     }
                 }
                 xs = side < 2;
@@ -839,7 +883,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 tl = low ? sec : own;
                 tr = low ? own : sec;
             }
-            const double hf = pnbr[tp][2 * side + 1] >= 0 ? 0.5 * h : h;
+            const double hf = two ? 0.5 * h : h;
             const int offn = xs ? TL::NPOS : 2 * TL::NPOS, offt = 3 * TL::NPOS - offn;
             OPrim L, R;
             load_oprim<N, PPB>(lds, tp, tl, offn, offt, L);
@@ -858,13 +902,13 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                            : tp * TL::FLUX_PER_PATCH + (xs ? k * (N + 1) : NFLD * N * (N + 1) + k);
             fnC[r] = xs ? 1 : 2;
         }
-        if (active && needB) {   // base state: loaded late, its latency overlaps the flux gather
+        if (actu && needB) {   // base state: loaded late, its latency overlaps the flux gather
 #pragma unroll
             for (int f = 0; f < 4; ++f) B[f] = a.U0[base + f * N2];
         }
         __syncthreads();   // all reads of the primitive tile are done
         const int stride = N * (N + 1);
-        if (active) {
+        if (actu) {
             const int dA = p * TL::FLUX_PER_PATCH + cj * (N + 1) + ci + 1;
             const int dB = p * TL::FLUX_PER_PATCH + NFLD * N * (N + 1) + (cj + 1) * N + ci;
 #pragma unroll
@@ -983,7 +1027,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             // little ILP and costs the registers that bound occupancy
             __builtin_amdgcn_sched_barrier(0);
         }
-        if (active && needB) {   // base state: loaded late, its latency overlaps the flux gather
+        if (actu && needB) {   // base state: loaded late, its latency overlaps the flux gather
 #pragma unroll
             for (int f = 0; f < 4; ++f) B[f] = a.U0[base + f * N2];
         }
@@ -999,16 +1043,16 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     }
 
     // ---- per-cell residual and stage update ------------------------------
-    if (active) {
+    if (actu) {
 #pragma unroll
         for (int f = 0; f < 4; ++f) U[f] = ldsU[(p * NFLD + f) * N2 + c];
     }
-    double k4[4] = {0.0, 0.0, 0.0, 0.0};
-    double out[4] = {0.0, 0.0, 0.0, 0.0};
+    double k4[4];
+    double out[4];
     double cand = INFINITY;
     double sq = 0.0;
     double cflh = 0.0;      // cfl sqrt(h h) of the patch
-    if (active) {
+    if (actu) {
         // -1 / (h h) and cfl sqrt(h h) from the instance's level-0 values
         double ninvV;
         if constexpr (UNI) {
