@@ -29,6 +29,12 @@
 #include "amr_hip.h"
 #include "amr_internal.h"
 
+// No multiply-add is fused unless the source says fma(): every kernel
+// instantiation (tile variants, the paired and the task rounds) then rounds
+// alike, so the two evaluations of a face that two patches share agree bit
+// for bit and a run's results do not depend on the tile it ran with.
+#pragma clang fp contract(off)
+
 namespace {
 
 constexpr int NFLD = 4;
@@ -172,11 +178,12 @@ __device__ __forceinline__ void flux_normal(const OPrim &L, const OPrim &R, doub
 {
     double inv_rf = frcp(L.sr + R.sr);
     double rhobar = L.sr * R.sr;
-    double unbar = (L.un * L.sr + R.un * R.sr) * inv_rf;
-    double utbar = (L.ut * L.sr + R.ut * R.sr) * inv_rf;
-    double Hbar = (L.H * L.sr + R.H * R.sr) * inv_rf;
+    // the low side's products stand alone: a cell's two paired faces share them
+    double unbar = fma(R.un, R.sr, L.un * L.sr) * inv_rf;
+    double utbar = fma(R.ut, R.sr, L.ut * L.sr) * inv_rf;
+    double Hbar = fma(R.H, R.sr, L.H * L.sr) * inv_rf;
     double abar, inva;
-    fsqrt_rsq(0.4 * (Hbar - 0.5 * (unbar * unbar + utbar * utbar)), abar, inva);
+    fsqrt_rsq(0.4 * fma(-0.5, fma(unbar, unbar, utbar * utbar), Hbar), abar, inva);
     double Mn = unbar * inva;
     double eigp = unbar + abar, eigm = unbar - abar;
     double eigp_max = R.un + abar, eigm_min = L.un - abar;
@@ -191,22 +198,22 @@ __device__ __forceinline__ void flux_normal(const OPrim &L, const OPrim &R, doub
     double invb = rr * d2;
     double Mc = rr * d1;
     double bcoeff = b1 * b2 * invb;
-    double delta1 = (R.r - L.r) - (R.p - L.p) * (inva * inva);
+    double delta1 = fma(-(R.p - L.p), inva * inva, R.r - L.r);
     double dH = R.H - L.H;
     // the face length folded into the coefficients
     const double invbh = invb * h;
     const double c1 = b1 * invbh, c2 = b2 * invbh;
     const double bch = bcoeff * h, bMh = bcoeff * Mc * h;
-    double fl1 = L.rn * L.un + L.p, fr1 = R.rn * R.un + R.p;
+    double fl1 = fma(L.rn, L.un, L.p), fr1 = fma(R.rn, R.un, R.p);
     double fl2 = L.rn * L.ut, fr2 = R.rn * R.ut;
     double dn = delta1 * unbar;                           // the shear term vanishes along n
-    double dt = delta1 * utbar + rhobar * (R.ut - L.ut);
+    double dt = fma(delta1, utbar, rhobar * (R.ut - L.ut));
     double fl3 = (L.rE + L.p) * L.un, fr3 = (R.rE + R.p) * R.un;
     F[0] = fma(-bMh, delta1, fma(bch, R.r - L.r, fma(-c2, R.rn, L.rn * c1)));
     F[1] = fma(-bMh, dn, fma(bch, R.rn - L.rn, fma(-c2, fr1, fl1 * c1)));
     F[2] = fma(-bMh, dt, fma(bch, R.rt - L.rt, fma(-c2, fr2, fl2 * c1)));
-    F[3] = fma(-bMh, delta1 * Hbar + rhobar * dH,
-               fma(bch, R.r * R.H - L.r * L.H, fma(-c2, fr3, fl3 * c1)));
+    F[3] = fma(-bMh, fma(delta1, Hbar, rhobar * dH),
+               fma(bch, fma(R.r, R.H, -(L.r * L.H)), fma(-c2, fr3, fl3 * c1)));
 }
 
 // Flux through an x face (XDIR) or y face in the (x, y) frame.
@@ -488,7 +495,7 @@ __device__ __forceinline__ void store_rec(double *lds, int p, int t, double r, d
     const double ir = rsr * rsr;
     const double u = ru * ir, v = rv * ir;
     // flux_2d_module.F90:98 pressure
-    const double pr = (1.4 - 1.0) * (rE - 0.5 * r * (u * u + v * v));
+    const double pr = (1.4 - 1.0) * fma(-(0.5 * r), fma(u, u, v * v), rE);
     double *b = lds + p * TL::PRIM_PER_PATCH + t;
     b[0 * TL::NPOS] = r;  b[1 * TL::NPOS] = ru; b[2 * TL::NPOS] = rv; b[3 * TL::NPOS] = rE;
     b[4 * TL::NPOS] = pr; b[5 * TL::NPOS] = ir; b[6 * TL::NPOS] = sr;
@@ -522,6 +529,51 @@ __device__ __forceinline__ void load_oprim(const double *lds, int p, int t, int 
     P.un = P.rn * ir;
     P.ut = P.rt * ir;
     P.H = (P.rE + P.p) * ir;
+}
+
+// The stage update of one cell from its residual k4 = R(U) / V (the schemes
+// of rk_time_integrators.c: Shu-Osher stages for SSP-RK2/3, the RK4
+// accumulator, the 2N-storage LSERK45 register), shared by the stage
+// kernels; B is the step's base state where the mode reads one.
+__device__ __forceinline__ void stage_update(const StageArgs &a, size_t base, int N2, double dt, const double U[4],
+                                             const double B[4], const double k4[4], double out[4])
+{
+    switch (a.mode) {
+    case MODE_SO:
+        if (a.U0) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) out[f] = fma(a.c2 * dt, k4[f], fma(a.c0, B[f], a.c1 * U[f]));
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) out[f] = fma(a.c2 * dt, k4[f], a.c1 * U[f]);
+        }
+        break;
+    case MODE_RK4_ACC:
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            double acc = a.first ? k4[f] : fma(2.0, k4[f], a.Acc[base + f * N2]);
+            a.Acc[base + f * N2] = acc;
+            out[f] = fma(a.c2 * dt, k4[f], B[f]);
+        }
+        break;
+    case MODE_RK4_FINAL:
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+            out[f] = fma(dt / 6.0, a.Acc[base + f * N2] + k4[f], B[f]);
+        break;
+    case MODE_LSERK:
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            double dq = a.first ? dt * k4[f] : fma(a.Acc[base + f * N2], a.c1, dt * k4[f]);
+            a.Acc[base + f * N2] = dq;
+            out[f] = fma(a.c2, dq, U[f]);
+        }
+        break;
+    default:   // MODE_RHS
+#pragma unroll
+        for (int f = 0; f < 4; ++f) out[f] = k4[f];
+        break;
+    }
 }
 
 template <int N, int PPB, int MINW, int DESIGN>
@@ -1082,46 +1134,11 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             if (s2) fyb += F2[(2 * NFLD + f) * N + ci];
             double Rf = (fxr - fxl) + (fyt - fyb);
             k4[f] = Rf * ninvV;
-            sq += k4[f] * k4[f];
+            sq = fma(k4[f], k4[f], sq);
         }
         double dt = 0.0;
         if (a.mode != MODE_RHS) dt = UNI ? dtw : pdt[p];
-        switch (a.mode) {
-        case MODE_SO:
-            if (a.U0) {
-#pragma unroll
-                for (int f = 0; f < 4; ++f) out[f] = a.c0 * B[f] + a.c1 * U[f] + (a.c2 * dt) * k4[f];
-            } else {
-#pragma unroll
-                for (int f = 0; f < 4; ++f) out[f] = a.c1 * U[f] + (a.c2 * dt) * k4[f];
-            }
-            break;
-        case MODE_RK4_ACC:
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                double acc = a.first ? k4[f] : a.Acc[base + f * N2] + 2.0 * k4[f];
-                a.Acc[base + f * N2] = acc;
-                out[f] = B[f] + (a.c2 * dt) * k4[f];
-            }
-            break;
-        case MODE_RK4_FINAL:
-#pragma unroll
-            for (int f = 0; f < 4; ++f)
-                out[f] = B[f] + (dt / 6.0) * (a.Acc[base + f * N2] + k4[f]);
-            break;
-        case MODE_LSERK:
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                double dq = a.first ? dt * k4[f] : a.Acc[base + f * N2] * a.c1 + dt * k4[f];
-                a.Acc[base + f * N2] = dq;
-                out[f] = U[f] + a.c2 * dq;
-            }
-            break;
-        default:   // MODE_RHS
-#pragma unroll
-            for (int f = 0; f < 4; ++f) out[f] = k4[f];
-            break;
-        }
+        stage_update(a, base, N2, dt, U, B, k4, out);
         // an instance past its physical stop (dt = 0) keeps its state bit for
         // bit: every stage state equals U^n then, while a convex combination
         // such as 0.75 U + 0.25 U can round away from U
@@ -1455,45 +1472,11 @@ __global__ __launch_bounds__(N * N, MINW) void stage_kernel_pp(StageArgs a)
                 double Rf = (FX[f * stride + cj * (N + 1) + ci + 1] - FX[f * stride + cj * (N + 1) + ci]) +
                             (FY[f * stride + (cj + 1) * N + ci] - FY[f * stride + cj * N + ci]);
                 k4[f] = Rf * ninvV;
-                sq += k4[f] * k4[f];
+                sq = fma(k4[f], k4[f], sq);
             }
         }
         const double dt = a.mode != MODE_RHS ? dtw : 0.0;
-        switch (a.mode) {
-        case MODE_SO:
-            if (a.U0) {
-#pragma unroll
-                for (int f = 0; f < 4; ++f) out[f] = a.c0 * B[f] + a.c1 * U[f] + (a.c2 * dt) * k4[f];
-            } else {
-#pragma unroll
-                for (int f = 0; f < 4; ++f) out[f] = a.c1 * U[f] + (a.c2 * dt) * k4[f];
-            }
-            break;
-        case MODE_RK4_ACC:
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                double acc = a.first ? k4[f] : a.Acc[base + f * N2] + 2.0 * k4[f];
-                a.Acc[base + f * N2] = acc;
-                out[f] = B[f] + (a.c2 * dt) * k4[f];
-            }
-            break;
-        case MODE_RK4_FINAL:
-#pragma unroll
-            for (int f = 0; f < 4; ++f) out[f] = B[f] + (dt / 6.0) * (a.Acc[base + f * N2] + k4[f]);
-            break;
-        case MODE_LSERK:
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                double dq = a.first ? dt * k4[f] : a.Acc[base + f * N2] * a.c1 + dt * k4[f];
-                a.Acc[base + f * N2] = dq;
-                out[f] = U[f] + a.c2 * dq;
-            }
-            break;
-        default:   // MODE_RHS
-#pragma unroll
-            for (int f = 0; f < 4; ++f) out[f] = k4[f];
-            break;
-        }
+        stage_update(a, base, N2, dt, U, B, k4, out);
         if (a.mode != MODE_RHS && dt == 0.0) {
 #pragma unroll
             for (int f = 0; f < 4; ++f) out[f] = U[f];
