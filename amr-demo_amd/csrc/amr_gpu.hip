@@ -332,6 +332,14 @@ __device__ __forceinline__ double dpp_f64(double v)
     hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+// A wave-uniform double moved to scalar registers (its value in lane 0):
+// it then holds no vector registers while it waits for its uses.
+__device__ __forceinline__ double uniform_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 __device__ __forceinline__ double lane_f64(double v, int lane)
 {
     const long long b = __double_as_longlong(v);
@@ -596,7 +604,8 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     const int p = tid / N2, c = tid % N2;
     const int ci = c % N, cj = c / N;
     const int quad = lb * PPB + p;
-    const bool active = quad < a.m.nquad;
+    // one patch per workgroup: the grid has exactly one workgroup per patch
+    const bool active = PPB == 1 || quad < a.m.nquad;
     // threads past the last patch read patch 0 (nquad >= 1 when launched)
     // and store nothing, so that the loads need no branch
     const int quadc = active ? quad : 0;
@@ -626,7 +635,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     if constexpr (UNI) {
         typedef const int __attribute__((address_space(4))) cint4;
         const int qw = __builtin_amdgcn_readfirstlane(quad);
-        actw = qw < a.m.nquad;
+        actw = PPB == 1 || qw < a.m.nquad;
         const int qc = actw ? qw : 0;      // a valid index (nquad >= 1 when launched); discarded if inactive
         const cint4 *nbc = (const cint4 *)a.m.nbr + (size_t)qc * 8;
 #pragma unroll
@@ -797,7 +806,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 if (rem < 0.0) rem = 0.0;
                 if (rem < dt) dt = rem;
             }
-            dtw = dt;
+            dtw = uniform_f64(dt);     // used by the epilogue only
             if (c == 0) {
                 ptime[p] = ti;
                 if (firstw) {                 // one writer per instance (a hot line otherwise)
@@ -824,6 +833,9 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             const double h = UNI ? hw : ph[p];
             const int na1 = pnbr[p][2], nb1 = pnbr[p][3], na3 = pnbr[p][6], nb3 = pnbr[p][7];
             const bool ex = (ci == N - 1), ey = (cj == N - 1);
+            // wall parameters: wave-uniform (scalar) when a wave holds one patch
+            const int wq = UNI ? __builtin_amdgcn_readfirstlane(quad) : quad;
+            const int wi = UNI ? inw : pinst[p];
             const double hx = (ex && nb1 >= 0) ? 0.5 * h : h;
             const double hy = (ey && nb3 >= 0) ? 0.5 * h : h;
             Prim L, RX, RY;
@@ -832,16 +844,24 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             load_prim<N, PPB>(lds, p, ey ? TL::GHOST + 3 * N + ci : c + N, RY);
             flux_face<true>(L, RX, hx, FA);
             flux_face<false>(L, RY, hy, FB);
+            // both fluxes complete here, before the wall branches: a wall
+            // replaces the value, and the compiler would otherwise sink the
+            // second flux past the first branch and lose the two chains'
+            // interleaving (an empty statement, no instruction)
+            asm volatile("" : "+v"(FA[0]), "+v"(FA[1]), "+v"(FA[2]), "+v"(FA[3]), "+v"(FB[0]), "+v"(FB[1]),
+                         "+v"(FB[2]), "+v"(FB[3]));
             if (ex && na1 < 0) {
                 const double nrm[2] = {h, 0.0};
-                const double q[4] = {L.r, L.ru, L.rv, L.rE};
-                flux_boundary(a.m.ip[pinst[p]], 1, nrm, q, a.m.org + 2 * (size_t)quad, (double)N,
+                const double q[4] = {ldsU[(p * NFLD + 0) * N2 + c], ldsU[(p * NFLD + 1) * N2 + c],
+                                     ldsU[(p * NFLD + 2) * N2 + c], ldsU[(p * NFLD + 3) * N2 + c]};
+                flux_boundary(a.m.ip[wi], 1, nrm, q, a.m.org + 2 * (size_t)wq, (double)N,
                               cj + 0.5, h, ptime[p], FA);
             }
             if (ey && na3 < 0) {
                 const double nrm[2] = {0.0, h};
-                const double q[4] = {L.r, L.ru, L.rv, L.rE};
-                flux_boundary(a.m.ip[pinst[p]], 3, nrm, q, a.m.org + 2 * (size_t)quad, ci + 0.5,
+                const double q[4] = {ldsU[(p * NFLD + 0) * N2 + c], ldsU[(p * NFLD + 1) * N2 + c],
+                                     ldsU[(p * NFLD + 2) * N2 + c], ldsU[(p * NFLD + 3) * N2 + c]};
+                flux_boundary(a.m.ip[wi], 3, nrm, q, a.m.org + 2 * (size_t)wq, ci + 0.5,
                               (double)N, h, ptime[p], FB);
             }
         }
@@ -1809,6 +1829,7 @@ const TileVariant kVariants[] = {
     {16, 1, 1, 0}, {16, 1, 4, 0}, {16, 1, 1, 1}, {16, 1, 4, 1},
     {8, 1, 1, 1}, {8, 1, 4, 1}, {8, 2, 4, 1}, {8, 1, 5, 1},
     {16, 1, 3, 2},
+    {16, 1, 5, 1},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -1838,16 +1859,17 @@ int grid_of(const amrgpu_ctx *c, int v, int nquad)
 }
 
 
-// Stage-kernel tile used until amrgpu_set_tile / amrgpu_autotune picks one;
-// for n = 8 the paired-faces tile with two patches per workgroup, the
-// fastest or within 2 % of it on every mesh measured (207k to 16.8M cells).
+// Stage-kernel tile used until amrgpu_set_tile / amrgpu_autotune picks one
+// (every tile computes the same bits): for n = 8 and 16 the paired-faces
+// tile with one patch per workgroup, four and five waves per SIMD, the
+// fastest or within 2 % of it on the meshes measured (207k to 16.8M cells).
 int default_variant(int n)
 {
     switch (n) {
     case 2: return 0;
     case 4: return 2;
-    case 8: return 16;
-    case 16: return 10;
+    case 8: return 15;
+    case 16: return 19;
     default: return -1;
     }
 }
@@ -1908,6 +1930,7 @@ int launch_stage(amrgpu_ctx *c, const StageArgs &a)
     case 16: return launch_v<8, 2, 4, 1>(c, a);
     case 17: return launch_v<8, 1, 5, 1>(c, a);
     case 18: return launch_pp<16, 3>(c, a);
+    case 19: return launch_v<16, 1, 5, 1>(c, a);
     default: set_err("bad tile variant %d", c->variant); return AMRGPU_ERR_ARG;
     }
 }
