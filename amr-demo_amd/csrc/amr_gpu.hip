@@ -457,15 +457,23 @@ struct Tile {
     static constexpr int N2 = N * N;
     static constexpr int BLK = N2 * PPB;
     // LDS record per tile position: r, ru, rv, rE, p, 1/r, sqrt(r), one
-    // array per quantity.  Positions [0, N2) are the patch cells row by row
-    // (pitch N: the lanes of a wave read consecutive addresses, free of bank
-    // conflicts), [N2, N2 + 4N) the ghost cell of each side s = 0..3 (-x, +x,
-    // -y, +y) at N2 + s N + k, [N2 + 4N, N2 + 8N) the second fine neighbour
-    // of each coarse side of a hanging face at the same offsets.
+    // array per quantity (array stride NPOS).  Positions [0, N2) are the
+    // patch cells row by row (pitch N: the lanes of a wave read consecutive
+    // addresses, free of bank conflicts), then GHOST + s N + k the ghost
+    // cell of each side s = 0..3 (-x, +x, -y, +y) and SECOND + s N + k the
+    // second fine neighbour of each coarse side of a hanging face.
+    // 8x8 patches (SPLIT): the halo's 8N positions equal the N2 cells, so
+    // the halo gets a block of its own after the cells' seven arrays, at the
+    // same stride N2.  The face and second-neighbour arrays written after
+    // the flux rounds then overlay everything but the cells' r, ru, rv, rE,
+    // which the epilogue reads back instead of keeping its own copy: 7.5 KB
+    // of LDS per patch instead of 9, which is what lets a CU hold 20
+    // one-patch workgroups (five waves per SIMD).
     static constexpr int NP = 7;
-    static constexpr int GHOST = N2;
-    static constexpr int SECOND = N2 + 4 * N;
-    static constexpr int NPOS = N2 + 8 * N;
+    static constexpr bool SPLIT = (8 * N == N2);
+    static constexpr int NPOS = SPLIT ? N2 : N2 + 8 * N;
+    static constexpr int GHOST = SPLIT ? NP * N2 : N2;
+    static constexpr int SECOND = GHOST + 4 * N;
     // tile position of cell (i, j), -1 <= i, j <= N, not a corner
     static __device__ __forceinline__ int pos(int i, int j)
     {
@@ -477,18 +485,33 @@ struct Tile {
     }
     static constexpr int PRIM_PER_PATCH = NP * NPOS;
     static constexpr int FLUX_PER_PATCH = NFLD * 2 * N * (N + 1);
-    static constexpr int LDS_DOUBLES =
-        PPB * (PRIM_PER_PATCH > FLUX_PER_PATCH ? PRIM_PER_PATCH : FLUX_PER_PATCH);
     // the second-neighbour fluxes of each patch side (4 sides x
-    // 4 fields x N) after the face arrays, and the rounds of extra tasks
+    // 4 fields x N) beside the face arrays, and the rounds of extra tasks
     // (at most 2N edge faces + 4N second fluxes per patch)
-    static constexpr int F2BASE = PPB * FLUX_PER_PATCH;
+    static constexpr int F2_PER_PATCH = 4 * NFLD * N;
+    // SPLIT: one block per patch -- cells' r, ru, rv, rE (4 N2), then the
+    // faces and second-neighbour fluxes over the rest of the records
+    static constexpr int PB = 4 * N2 + FLUX_PER_PATCH + F2_PER_PATCH;
+    static constexpr int LDS_DOUBLES =
+        SPLIT ? PPB * PB : PPB * (PRIM_PER_PATCH > FLUX_PER_PATCH ? PRIM_PER_PATCH : FLUX_PER_PATCH);
+    static constexpr int F2BASE = PPB * FLUX_PER_PATCH;     // (not SPLIT)
+    // where patch p's records, face arrays and second-neighbour arrays start
+    static __device__ __forceinline__ constexpr int rec(int p) { return p * (SPLIT ? PB : PRIM_PER_PATCH); }
+    static __device__ __forceinline__ constexpr int face(int p)
+    {
+        return SPLIT ? p * PB + 4 * N2 : p * FLUX_PER_PATCH;
+    }
+    static __device__ __forceinline__ constexpr int f2(int p)
+    {
+        return SPLIT ? p * PB + 4 * N2 + FLUX_PER_PATCH : F2BASE + p * F2_PER_PATCH;
+    }
     static constexpr int XROUNDS = (PPB * 6 * N + BLK - 1) / BLK;
     static constexpr int NINT = 2 * N * (N - 1);           // interior faces per patch
     static constexpr int NEDGE = 4 * N;                    // patch-edge faces per patch
     static constexpr int NTASK = PPB * (NINT + NEDGE);
     static constexpr int ROUNDS = (NTASK + BLK - 1) / BLK;
-    static_assert(F2BASE + PPB * 4 * NFLD * N <= LDS_DOUBLES, "second-neighbour fluxes fit beside the faces");
+    static_assert(SPLIT || F2BASE + PPB * F2_PER_PATCH <= LDS_DOUBLES, "second-neighbour fluxes fit beside the faces");
+    static_assert(!SPLIT || PB >= 2 * NP * N2, "a patch's block holds its records");
 };
 
 template <int N, int PPB>
@@ -504,7 +527,7 @@ __device__ __forceinline__ void store_rec(double *lds, int p, int t, double r, d
     const double u = ru * ir, v = rv * ir;
     // flux_2d_module.F90:98 pressure
     const double pr = (1.4 - 1.0) * fma(-(0.5 * r), fma(u, u, v * v), rE);
-    double *b = lds + p * TL::PRIM_PER_PATCH + t;
+    double *b = lds + TL::rec(p) + t;
     b[0 * TL::NPOS] = r;  b[1 * TL::NPOS] = ru; b[2 * TL::NPOS] = rv; b[3 * TL::NPOS] = rE;
     b[4 * TL::NPOS] = pr; b[5 * TL::NPOS] = ir; b[6 * TL::NPOS] = sr;
 }
@@ -513,7 +536,7 @@ template <int N, int PPB>
 __device__ __forceinline__ void load_prim(const double *lds, int p, int t, Prim &P)
 {
     using TL = Tile<N, PPB>;
-    const double *b = lds + p * TL::PRIM_PER_PATCH + t;
+    const double *b = lds + TL::rec(p) + t;
     P.r = b[0 * TL::NPOS]; P.ru = b[1 * TL::NPOS]; P.rv = b[2 * TL::NPOS]; P.rE = b[3 * TL::NPOS];
     P.p = b[4 * TL::NPOS];
     const double ir = b[5 * TL::NPOS];
@@ -529,7 +552,7 @@ template <int N, int PPB>
 __device__ __forceinline__ void load_oprim(const double *lds, int p, int t, int offn, int offt, OPrim &P)
 {
     using TL = Tile<N, PPB>;
-    const double *b = lds + p * TL::PRIM_PER_PATCH + t;
+    const double *b = lds + TL::rec(p) + t;
     P.r = b[0]; P.rn = b[offn]; P.rt = b[offt]; P.rE = b[3 * TL::NPOS];
     P.p = b[4 * TL::NPOS];
     const double ir = b[5 * TL::NPOS];
@@ -590,7 +613,8 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     using TL = Tile<N, PPB>;
     constexpr int N2 = TL::N2;
     __shared__ double lds[TL::LDS_DOUBLES];
-    __shared__ double ldsU[PPB * NFLD * N2];     // own state, kept out of registers
+    // own state, kept out of registers (SPLIT: read back from the records)
+    __shared__ double ldsU[TL::SPLIT ? 1 : PPB * NFLD * N2];
     __shared__ double red[(TL::BLK + 63) / 64];
     __shared__ int pquad[PPB], plevel[PPB], pinst[PPB];
     __shared__ int pnbr[PPB][8];
@@ -610,6 +634,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     // and store nothing, so that the loads need no branch
     const int quadc = active ? quad : 0;
     const size_t base = (size_t)quadc * NFLD * N2 + c;
+    // field f of the cell's own state, from LDS
+    auto own_u = [=](int f) {
+        return TL::SPLIT ? lds[TL::rec(p) + f * TL::NPOS + c] : ldsU[(p * NFLD + f) * N2 + c];
+    };
 
     // ---- prologue -----------------------------------------------------------
     // The own-state loads and the halo loads go out first; the waits that
@@ -782,7 +810,8 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     if (actu) {
         store_rec<N, PPB>(lds, p, c, U[0], U[1], U[2], U[3]);
 #pragma unroll
-        for (int f = 0; f < 4; ++f) ldsU[(p * NFLD + f) * N2 + c] = U[f];
+        for (int f = 0; f < 4; ++f)
+            if constexpr (!TL::SPLIT) ldsU[(p * NFLD + f) * N2 + c] = U[f];
     }
     if constexpr (UNI) {
         if (a.role & ROLE_DT) {
@@ -852,15 +881,13 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                          "+v"(FB[2]), "+v"(FB[3]));
             if (ex && na1 < 0) {
                 const double nrm[2] = {h, 0.0};
-                const double q[4] = {ldsU[(p * NFLD + 0) * N2 + c], ldsU[(p * NFLD + 1) * N2 + c],
-                                     ldsU[(p * NFLD + 2) * N2 + c], ldsU[(p * NFLD + 3) * N2 + c]};
+                const double q[4] = {own_u(0), own_u(1), own_u(2), own_u(3)};
                 flux_boundary(a.m.ip[wi], 1, nrm, q, a.m.org + 2 * (size_t)wq, (double)N,
                               cj + 0.5, h, ptime[p], FA);
             }
             if (ey && na3 < 0) {
                 const double nrm[2] = {0.0, h};
-                const double q[4] = {ldsU[(p * NFLD + 0) * N2 + c], ldsU[(p * NFLD + 1) * N2 + c],
-                                     ldsU[(p * NFLD + 2) * N2 + c], ldsU[(p * NFLD + 3) * N2 + c]};
+                const double q[4] = {own_u(0), own_u(1), own_u(2), own_u(3)};
                 flux_boundary(a.m.ip[wi], 3, nrm, q, a.m.org + 2 * (size_t)wq, ci + 0.5,
                               (double)N, h, ptime[p], FB);
             }
@@ -892,6 +919,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         }
         double FC[TL::XROUNDS][4];
         int dC[TL::XROUNDS], fnC[TL::XROUNDS];
+        bool sC[TL::XROUNDS];      // a second neighbour's flux (side arrays, field stride N)
 #pragma unroll
         for (int r = 0; r < TL::XROUNDS; ++r) {
             dC[r] = -1;
@@ -970,9 +998,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 FC[r][0] = -F[0]; FC[r][1] = -(xs ? F[1] : F[2]); FC[r][2] = -(xs ? F[2] : F[1]);
                 FC[r][3] = -F[3];
             }
-            dC[r] = second ? TL::F2BASE + ((tp * 4 + side) * NFLD) * N + k
-                           : tp * TL::FLUX_PER_PATCH + (xs ? k * (N + 1) : NFLD * N * (N + 1) + k);
+            dC[r] = second ? TL::f2(tp) + (side * NFLD) * N + k
+                           : TL::face(tp) + (xs ? k * (N + 1) : NFLD * N * (N + 1) + k);
             fnC[r] = xs ? 1 : 2;
+            sC[r] = second;
         }
         if (actu && needB) {   // base state: loaded late, its latency overlaps the flux gather
 #pragma unroll
@@ -981,8 +1010,8 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         __syncthreads();   // all reads of the primitive tile are done
         const int stride = N * (N + 1);
         if (actu) {
-            const int dA = p * TL::FLUX_PER_PATCH + cj * (N + 1) + ci + 1;
-            const int dB = p * TL::FLUX_PER_PATCH + NFLD * N * (N + 1) + (cj + 1) * N + ci;
+            const int dA = TL::face(p) + cj * (N + 1) + ci + 1;
+            const int dB = TL::face(p) + NFLD * N * (N + 1) + (cj + 1) * N + ci;
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
                 lds[dA + f * stride] = FA[f];
@@ -994,7 +1023,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             if (dC[r] < 0) continue;
             // first fluxes in the face arrays (field stride N (N + 1)), second
             // ones in the side arrays (field stride N)
-            const int fs = dC[r] >= TL::F2BASE ? N : stride;
+            const int fs = sC[r] ? N : stride;
             lds[dC[r]] = FC[r][0];
             lds[dC[r] + fnC[r] * fs] = FC[r][1];
             lds[dC[r] + (3 - fnC[r]) * fs] = FC[r][2];
@@ -1054,7 +1083,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                     double nrm[2] = {0.0, 0.0};
                     if (side == 0) nrm[0] = -h; else if (side == 1) nrm[0] = h;
                     else if (side == 2) nrm[1] = -h; else nrm[1] = h;
-                    const double *b = lds + tp * TL::PRIM_PER_PATCH + town;
+                    const double *b = lds + TL::rec(tp) + town;
                     double q[4] = {b[0], b[TL::NPOS], b[2 * TL::NPOS], b[3 * TL::NPOS]};
                     const double fxo = side == 0 ? 0.0 : (side == 1 ? (double)N : fi + 0.5);
                     const double fyo = side == 2 ? 0.0 : (side == 3 ? (double)N : fj + 0.5);
@@ -1093,7 +1122,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     #pragma unroll
             for (int f = 0; f < 4; ++f) Fr[r][f] = F[f];
             // FX[p][f][fj][fi] (fi in 0..N) ; FY[p][f][fj][fi] (fj in 0..N)
-            Fdst[r] = tp * TL::FLUX_PER_PATCH + (isx ? (fj * (N + 1) + fi)
+            Fdst[r] = TL::face(tp) + (isx ? (fj * (N + 1) + fi)
                                                       : (NFLD * N * (N + 1) + fj * N + fi));
             // keep the rounds apart: interleaving three flux evaluations buys
             // little ILP and costs the registers that bound occupancy
@@ -1117,7 +1146,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     // ---- per-cell residual and stage update ------------------------------
     if (actu) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f) U[f] = ldsU[(p * NFLD + f) * N2 + c];
+        for (int f = 0; f < 4; ++f) U[f] = own_u(f);
     }
     double k4[4];
     double out[4];
@@ -1137,11 +1166,11 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             ninvV = ldexp(ip.ninv_vol0, 2 * plevel[p]);
             cflh = ldexp(ip.cfl_h0, -plevel[p]);
         }
-        const double *FX = lds + p * TL::FLUX_PER_PATCH;
+        const double *FX = lds + TL::face(p);
         const double *FY = FX + NFLD * N * (N + 1);
         const int stride = N * (N + 1);
         // second fine neighbours' fluxes on the patch edges
-        const double *F2 = lds + TL::F2BASE + p * 4 * NFLD * N;
+        const double *F2 = lds + TL::f2(p);
         const bool s0 = DESIGN == 1 && ci == 0 && pnbr[p][1] >= 0, s1 = DESIGN == 1 && ci == N - 1 && pnbr[p][3] >= 0;
         const bool s2 = DESIGN == 1 && cj == 0 && pnbr[p][5] >= 0, s3 = DESIGN == 1 && cj == N - 1 && pnbr[p][7] >= 0;
 #pragma unroll
