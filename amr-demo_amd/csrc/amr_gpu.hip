@@ -660,6 +660,11 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     int nbw[8];
     int levw = 0, inw = 0;
     bool actw = false, firstw = false;   // firstw: the wave holds its instance's first quadrant
+    // one paired-face patch per workgroup: every reader of the patch's ids
+    // and parameters takes them from the wave's scalars, and the LDS copies
+    // the other tiles share between waves are not written
+    constexpr bool SMETA = (PPB == 1 && UNI && DESIGN == 1);
+    double tpw = 0.0;     // the step's start time of the patch's instance (wave-uniform)
     if constexpr (UNI) {
         typedef const int __attribute__((address_space(4))) cint4;
         const int qw = __builtin_amdgcn_readfirstlane(quad);
@@ -749,8 +754,9 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             // written by an earlier launch, constant during this one
             dtw = actw && a.mode != MODE_RHS ? ((const cdbl4 *)a.dt_cur)[inw] : 0.0;
             tw = actw ? ((const cdbl4 *)a.m.t_step)[inw] : 0.0;
+            tpw = tw;
         }
-        if (c == 0) {
+        if (!SMETA && c == 0) {
             pquad[p] = actw ? lb * PPB + p : -1;
             plevel[p] = levw;
             pinst[p] = inw;
@@ -836,8 +842,9 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 if (rem < dt) dt = rem;
             }
             dtw = uniform_f64(dt);     // used by the epilogue only
+            tpw = ti;
             if (c == 0) {
-                ptime[p] = ti;
+                if (!SMETA) ptime[p] = ti;
                 if (firstw) {                 // one writer per instance (a hot line otherwise)
                     a.dt_cur[inw] = dt;
                     a.t_step[inw] = ti;
@@ -860,7 +867,9 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         double FA[4], FB[4];
         if (actu) {
             const double h = UNI ? hw : ph[p];
-            const int na1 = pnbr[p][2], nb1 = pnbr[p][3], na3 = pnbr[p][6], nb3 = pnbr[p][7];
+            const int na1 = UNI ? nbw[2] : pnbr[p][2], nb1 = UNI ? nbw[3] : pnbr[p][3];
+            const int na3 = UNI ? nbw[6] : pnbr[p][6], nb3 = UNI ? nbw[7] : pnbr[p][7];
+            const double tp_ = UNI ? tpw : ptime[p];
             const bool ex = (ci == N - 1), ey = (cj == N - 1);
             // wall parameters: wave-uniform (scalar) when a wave holds one patch
             const int wq = UNI ? __builtin_amdgcn_readfirstlane(quad) : quad;
@@ -883,13 +892,13 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 const double nrm[2] = {h, 0.0};
                 const double q[4] = {own_u(0), own_u(1), own_u(2), own_u(3)};
                 flux_boundary(a.m.ip[wi], 1, nrm, q, a.m.org + 2 * (size_t)wq, (double)N,
-                              cj + 0.5, h, ptime[p], FA);
+                              cj + 0.5, h, tp_, FA);
             }
             if (ey && na3 < 0) {
                 const double nrm[2] = {0.0, h};
                 const double q[4] = {own_u(0), own_u(1), own_u(2), own_u(3)};
                 flux_boundary(a.m.ip[wi], 3, nrm, q, a.m.org + 2 * (size_t)wq, ci + 0.5,
-                              (double)N, h, ptime[p], FB);
+                              (double)N, h, tp_, FB);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -993,8 +1002,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
                 const double nrm[2] = {xs ? -h : 0.0, xs ? 0.0 : -h};
                 const double q[4] = {R.r, xs ? R.rn : R.rt, xs ? R.rt : R.rn, R.rE};
                 double F[4];
-                flux_boundary(a.m.ip[pinst[tp]], side, nrm, q, a.m.org + 2 * (size_t)pquad[tp],
-                              xs ? 0.0 : k + 0.5, xs ? k + 0.5 : 0.0, h, ptime[tp], F);
+                const int wi = SCALAR_TASKS ? inw : pinst[tp];
+                const int wq = SCALAR_TASKS ? __builtin_amdgcn_readfirstlane(quad) : pquad[tp];
+                flux_boundary(a.m.ip[wi], side, nrm, q, a.m.org + 2 * (size_t)wq,
+                              xs ? 0.0 : k + 0.5, xs ? k + 0.5 : 0.0, h, SCALAR_TASKS ? tpw : ptime[tp], F);
                 FC[r][0] = -F[0]; FC[r][1] = -(xs ? F[1] : F[2]); FC[r][2] = -(xs ? F[2] : F[1]);
                 FC[r][3] = -F[3];
             }
@@ -1171,8 +1182,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         const int stride = N * (N + 1);
         // second fine neighbours' fluxes on the patch edges
         const double *F2 = lds + TL::f2(p);
-        const bool s0 = DESIGN == 1 && ci == 0 && pnbr[p][1] >= 0, s1 = DESIGN == 1 && ci == N - 1 && pnbr[p][3] >= 0;
-        const bool s2 = DESIGN == 1 && cj == 0 && pnbr[p][5] >= 0, s3 = DESIGN == 1 && cj == N - 1 && pnbr[p][7] >= 0;
+        const int n1 = UNI ? nbw[1] : pnbr[p][1], n3 = UNI ? nbw[3] : pnbr[p][3];
+        const int n5 = UNI ? nbw[5] : pnbr[p][5], n7 = UNI ? nbw[7] : pnbr[p][7];
+        const bool s0 = DESIGN == 1 && ci == 0 && n1 >= 0, s1 = DESIGN == 1 && ci == N - 1 && n3 >= 0;
+        const bool s2 = DESIGN == 1 && cj == 0 && n5 >= 0, s3 = DESIGN == 1 && cj == N - 1 && n7 >= 0;
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
             double fxr = FX[f * stride + cj * (N + 1) + ci + 1], fxl = FX[f * stride + cj * (N + 1) + ci];
@@ -1207,11 +1220,11 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         for (int i = tid; i < a.m.ninst * NSLOT; i += TL::BLK) a.slot[i] = DBL_INF_BITS;
     }
     if (a.role & ROLE_CFL) {
-        int uinst = pinst[0];
+        int uinst = SMETA ? inw : pinst[0];
 #pragma unroll
         for (int k = 1; k < PPB; ++k)
             if (pquad[k] >= 0 && pinst[k] != uinst) uinst = -1;
-        cfl_reduce<TL::BLK>(cand, active ? pinst[p] : 0, active, uinst, a.slot, red);
+        cfl_reduce<TL::BLK>(cand, active ? (SMETA ? inw : pinst[p]) : 0, active, uinst, a.slot, red);
     }
     if (a.norm_part) sum_reduce_store<TL::BLK>(sq, &a.norm_part[blockIdx.x], red);
 }
