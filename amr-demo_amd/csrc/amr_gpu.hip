@@ -711,21 +711,22 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     for (int hh = hh0, it = 0; ONE_GHOST ? it < 1 : hh < 4 * N; hh += N2, ++it) {
         if (!ONE_GHOST && !active) break;
         const int sd = hh / N, k = hh % N;
+        const bool sx = sd < 2, shi = sd & 1;    // an x side; the high (+) side
         int na, nb;
         if constexpr (UNI) {
-            na = sd == 0 ? nbw[0] : sd == 1 ? nbw[2] : sd == 2 ? nbw[4] : nbw[6];
-            nb = sd == 0 ? nbw[1] : sd == 1 ? nbw[3] : sd == 2 ? nbw[5] : nbw[7];
+            // selects of wave-uniform ids by two bits of the side: no branches
+            const int na_x = shi ? nbw[2] : nbw[0], na_y = shi ? nbw[6] : nbw[4];
+            const int nb_x = shi ? nbw[3] : nbw[1], nb_y = shi ? nbw[7] : nbw[5];
+            na = sx ? na_x : na_y;
+            nb = sx ? nb_x : nb_y;
         } else {
             na = a.m.nbr[quadc * 8 + 2 * sd];
             nb = a.m.nbr[quadc * 8 + 2 * sd + 1];
         }
-        int ec, oc;   // neighbour's edge cell, own edge cell
-        switch (sd) {
-        case 0: ec = k * N + (N - 1); oc = k * N;           break;
-        case 1: ec = k * N + 0;       oc = k * N + (N - 1); break;
-        case 2: ec = (N - 1) * N + k; oc = k;               break;
-        default: ec = 0 * N + k;      oc = (N - 1) * N + k; break;
-        }
+        // neighbour's edge cell ec and own edge cell oc: row / column k of
+        // the side, at the far (N - 1) or near (0) end of the normal
+        const int kb = sx ? k * N : k, far = sx ? N - 1 : (N - 1) * N;
+        const int ec = kb + (shi ? 0 : far), oc = kb + (shi ? far : 0);
         // physical boundary: the ghost holds the own edge cell, so that the
         // generic face evaluation stays finite before the boundary flux
         // replaces it
@@ -1204,7 +1205,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         // an instance past its physical stop (dt = 0) keeps its state bit for
         // bit: every stage state equals U^n then, while a convex combination
         // such as 0.75 U + 0.25 U can round away from U
-        if (a.mode != MODE_RHS && dt == 0.0) {
+        // (a wave-uniform test when a wave holds one patch: a scalar branch
+        // around the rare case instead of selects on every path)
+        const bool frozen = a.mode != MODE_RHS && dt == 0.0;
+        if (UNI ? (bool)__builtin_amdgcn_readfirstlane((int)frozen) : frozen) {
 #pragma unroll
             for (int f = 0; f < 4; ++f) out[f] = U[f];
         }
