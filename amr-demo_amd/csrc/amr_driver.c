@@ -774,15 +774,16 @@ static int batch_build_tab(const tab_t *tab, const int *dropped, int ninst, int 
 }
 
 /* Per-instance regrid work of one batch regrid, shared by the host threads:
- * instance k (taken in turn from a counter) is regridded by its tags and its
- * mesh table rebuilt when it has not reached its stop; one that has leaves
- * the device mesh at this regrid and needs neither. */
+ * instance k (taken in turn from a counter, largest first) is regridded by
+ * its tags and its mesh table rebuilt when it has not reached its stop; one
+ * that has leaves the device mesh at this regrid and needs neither. */
 typedef struct {
     amr_forest_t **f;
     const int *done, *tags, *obase;
     int allow, ninst;
     tab_t *tab;
     int *ok;
+    int *order;      /* instances by decreasing quadrant count */
     int next;
     pthread_mutex_t lock;
 } regrid_job_t;
@@ -795,9 +796,10 @@ static void *regrid_worker(void *arg)
     const int was_parallel = amr_forest_host_parallel(0);
     for (;;) {
         pthread_mutex_lock(&j->lock);
-        const int k = j->next++;
+        const int i = j->next++;
         pthread_mutex_unlock(&j->lock);
-        if (k >= j->ninst) break;
+        if (i >= j->ninst) break;
+        const int k = j->order ? j->order[i] : i;
         j->ok[k] = 1;
         if (j->done[k]) continue;      /* leaves the device mesh at this regrid */
         amr_forest_track_begin(j->f[k], 0);
@@ -825,6 +827,22 @@ static void regrid_all(regrid_job_t *j)
 {
     const int nt = host_threads(j->ninst);
     j->next = 0;
+    /* the largest instances first: the threads' last instances are then
+     * small ones, and the regrid ends close to the total work / threads
+     * (an instance's regrid is serial, and one large instance taken last
+     * would hold the others) */
+    j->order = (int *)malloc(sizeof(int) * (size_t)j->ninst);
+    if (j->order) {     /* insertion sort by old quadrant count, stable */
+        for (int k = 0; k < j->ninst; ++k) {
+            const int sk = j->obase[k + 1] - j->obase[k];
+            int i = k;
+            while (i > 0 && j->obase[j->order[i - 1] + 1] - j->obase[j->order[i - 1]] < sk) {
+                j->order[i] = j->order[i - 1];
+                --i;
+            }
+            j->order[i] = k;
+        }
+    }
     pthread_mutex_init(&j->lock, NULL);
     pthread_t th[16];
     int started = 0;
@@ -833,6 +851,8 @@ static void regrid_all(regrid_job_t *j)
     regrid_worker(j);
     for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
     pthread_mutex_destroy(&j->lock);
+    free(j->order);
+    j->order = NULL;
 }
 
 static void *regrid_all_thread(void *arg)
