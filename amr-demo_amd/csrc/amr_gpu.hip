@@ -687,18 +687,6 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     // when a wave holds one patch: branches on it are scalar, and values
     // assigned under it need no definition for threads that skip it
     const bool actu = UNI ? actw : active;
-    // first stage of a step: the CFL sub-slots of the patch's instance, read
-    // with the halo; the step's dt is formed from them after the records
-    unsigned long long slv[(NSLOT + 63) / 64];
-    if constexpr (UNI) {
-        if ((a.role & ROLE_DT) && a.use_cfl) {
-            const unsigned long long *sp = a.slot + (size_t)inw * NSLOT;
-            const int lane = threadIdx.x & 63;
-#pragma unroll
-            for (int k = 0; k < (NSLOT + 63) / 64; ++k)
-                slv[k] = lane + 64 * k < NSLOT ? sp[lane + 64 * k] : DBL_INF_BITS;
-        }
-    }
     // halo: the ghost cell of side s = hh / N, position k = hh % N (and the
     // second fine neighbour of a coarse side), loaded here and stored as
     // records after the parameter chains below when there is at most one
@@ -741,6 +729,20 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         if constexpr (!ONE_GHOST) {
             if (gsec) store_rec<N, PPB>(lds, p, TL::SECOND + hh, s0, s1, s2, s3);
             store_rec<N, PPB>(lds, p, TL::GHOST + hh, g0, g1, g2, g3);
+        }
+    }
+    // first stage of a step: the CFL sub-slots of the patch's instance, read
+    // after the halo loads are issued (a wait for them, wherever the compiler
+    // places the reduction, then holds back no halo load); the step's dt is
+    // formed from them after the records
+    unsigned long long slv[(NSLOT + 63) / 64];
+    if constexpr (UNI) {
+        if ((a.role & ROLE_DT) && a.use_cfl) {
+            const unsigned long long *sp = a.slot + (size_t)inw * NSLOT;
+            const int lane = threadIdx.x & 63;
+#pragma unroll
+            for (int k = 0; k < (NSLOT + 63) / 64; ++k)
+                slv[k] = lane + 64 * k < NSLOT ? sp[lane + 64 * k] : DBL_INF_BITS;
         }
     }
     // patch parameters that depend on the instance
@@ -830,6 +832,11 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             double dt = ipc[inw].dt_cap;
             if (a.use_cfl) {
                 double m = INFINITY;
+                // the slots pass through an empty asm here, so that their
+                // reduction (and the wait for their loads) is not moved up
+                // to the loads, ahead of the records' arithmetic
+#pragma unroll
+                for (int k = 0; k < (NSLOT + 63) / 64; ++k) asm volatile("" : "+v"(slv[k]));
 #pragma unroll
                 for (int k = 0; k < (NSLOT + 63) / 64; ++k) m = fmin(m, bits2d(slv[k]));
                 m = wave_min(m);
