@@ -102,6 +102,14 @@ struct StageArgs {
 
 __device__ __forceinline__ double bits2d(unsigned long long b) { return __longlong_as_double((long long)b); }
 __device__ __forceinline__ unsigned long long d2bits(double d) { return (unsigned long long)__double_as_longlong(d); }
+// A device pointer taken as a global-memory one: loads through it are
+// global_load (vector memory counter only) instead of flat loads, which
+// also count against the LDS counter and so make every later LDS wait wait
+// for them too.
+__device__ __forceinline__ const double __attribute__((address_space(1))) *gptr(const double *p)
+{
+    return (const double __attribute__((address_space(1))) *)p;
+}
 
 // Logical block index with each XCD receiving a contiguous range, so that
 // Morton-adjacent patches (which share halo cells) are processed by the same
@@ -1024,7 +1032,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         }
         if (actu && needB) {   // base state: loaded late, its latency overlaps the flux gather
 #pragma unroll
-            for (int f = 0; f < 4; ++f) B[f] = a.U0[base + f * N2];
+            for (int f = 0; f < 4; ++f) B[f] = gptr(a.U0)[base + f * N2];
         }
         __syncthreads();   // all reads of the primitive tile are done
         const int stride = N * (N + 1);
@@ -1149,7 +1157,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         }
         if (actu && needB) {   // base state: loaded late, its latency overlaps the flux gather
 #pragma unroll
-            for (int f = 0; f < 4; ++f) B[f] = a.U0[base + f * N2];
+            for (int f = 0; f < 4; ++f) B[f] = gptr(a.U0)[base + f * N2];
         }
         __syncthreads();   // all reads of the primitive tile are done
     #pragma unroll
@@ -1368,7 +1376,7 @@ __global__ __launch_bounds__(N * N, MINW) void stage_kernel_pp(StageArgs a)
         const size_t base = (size_t)q * NFLD * N2 + c;
         if (needB) {
 #pragma unroll
-            for (int f = 0; f < 4; ++f) B[f] = a.U0[base + f * N2];
+            for (int f = 0; f < 4; ++f) B[f] = gptr(a.U0)[base + f * N2];
         }
         unsigned long long slv[(NSLOT + 63) / 64];
         if ((a.role & ROLE_DT) && a.use_cfl) {
