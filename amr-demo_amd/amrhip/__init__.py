@@ -174,7 +174,7 @@ class EvolveOpts(C.Structure):
     """amr_evolve_opts_t"""
     _fields_ = [("tol", _d), ("allow_coarsening", _i), ("regrid_interval", _i), ("tagger", _vp),
                 ("tagger_ctx", _vp), ("tile_ppb", _i), ("tile_minw", _i), ("tile_design", _i),
-                ("inst", C.POINTER(InstParams)), ("regrid_lag", _i)]
+                ("inst", C.POINTER(InstParams)), ("regrid_lag", _i), ("groups", _i)]
 
 
 class PhysicsParams(C.Structure):
@@ -445,9 +445,11 @@ class Forest:
 
 
 def evolve_batch(forests, insts, tol, regrid_interval, allow_coarsening=True, tile=None, tagger=None,
-                 regrid_lag=0):
-    """amr_evolve_batch: every forest advanced in one device context;
-    insts[k] = make_inst(...) of forest k.  Returns (per-instance stats, total)"""
+                 regrid_lag=0, groups=1):
+    """amr_evolve_batch: every forest advanced in one device context
+    (groups > 1: in that many contexts on their own host threads, one group's
+    host regrid overlapping the others' steps); insts[k] = make_inst(...) of
+    forest k.  Returns (per-instance stats, total)"""
     L = lib()
     k = len(forests)
     fa = (_vp * k)(*[f.h for f in forests])
@@ -462,6 +464,7 @@ def evolve_batch(forests, insts, tol, regrid_interval, allow_coarsening=True, ti
     elif tile:
         o.tile_ppb, o.tile_minw, o.tile_design = tile
     o.regrid_lag = regrid_lag
+    o.groups = groups
     per = (EvolveStats * k)()
     tot = EvolveStats()
     check(L.amr_evolve_batch(fa, ia, k, C.byref(o), per, C.byref(tot)), "amr_evolve_batch")

@@ -861,10 +861,156 @@ static void *regrid_all_thread(void *arg)
     return NULL;
 }
 
+/* ---- instance groups: one device context and host thread per group ----- */
+
+typedef struct {               /* a tag source shared by the groups, one call at a time */
+    amr_tagger_fn fn;
+    void *ctx;
+    pthread_mutex_t lock;
+} shared_tagger_t;
+
+static int shared_tagger(void *user, struct amrgpu_ctx *ctx, amr_forest_t *f, int *tags)
+{
+    shared_tagger_t *s = (shared_tagger_t *)user;
+    pthread_mutex_lock(&s->lock);
+    const int rc = s->fn(s->ctx, ctx, f, tags);
+    pthread_mutex_unlock(&s->lock);
+    return rc;
+}
+
+typedef struct {
+    amr_forest_t **f;
+    amrgpu_inst_t *inst;
+    int n;
+    amr_evolve_opts_t o;
+    amr_evolve_stats_t *per, tot;
+    int rc;
+} batch_group_t;
+
+static void *batch_group_thread(void *arg)
+{
+    batch_group_t *g = (batch_group_t *)arg;
+    g->rc = amr_evolve_batch(g->f, g->inst, g->n, &g->o, g->per, &g->tot);
+    return NULL;
+}
+
+/* Estimated work of an instance: quadrants x 2^(finest level), the steps to a
+ * given time growing with the finest level's 1/h. */
+static double inst_work(const amr_forest_t *f)
+{
+    const amr_quad_t *q = amr_forest_quads(f);
+    const int nq = amr_forest_num_quads(f);
+    int lmax = 0;
+    for (int i = 0; i < nq; ++i)
+        if (q[i].level > lmax) lmax = q[i].level;
+    return (double)nq * ldexp(1.0, lmax);
+}
+
+static int evolve_groups(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, const amr_evolve_opts_t *o,
+                         amr_evolve_stats_t *per_inst, amr_evolve_stats_t *total)
+{
+    int ng = o->groups < ninst ? o->groups : ninst;
+    if (ng > AMR_MAX_GROUPS) ng = AMR_MAX_GROUPS;
+    batch_group_t grp[AMR_MAX_GROUPS];
+    memset(grp, 0, sizeof(grp));
+    int *order = (int *)malloc(sizeof(int) * (size_t)ninst);
+    int *of = (int *)malloc(sizeof(int) * (size_t)ninst);           /* group of each instance */
+    double *w = (double *)malloc(sizeof(double) * (size_t)ninst);
+    amr_forest_t **fs = (amr_forest_t **)malloc(sizeof(amr_forest_t *) * (size_t)ninst);
+    amrgpu_inst_t *is = (amrgpu_inst_t *)malloc(sizeof(amrgpu_inst_t) * (size_t)ninst);
+    amr_evolve_stats_t *ps = (amr_evolve_stats_t *)calloc((size_t)ninst, sizeof(amr_evolve_stats_t));
+    int rc = 0;
+    if (!order || !of || !w || !fs || !is || !ps) rc = AMRGPU_ERR_ARG;
+    shared_tagger_t tg;
+    tg.fn = o->tagger;
+    tg.ctx = o->tagger_ctx;
+    pthread_mutex_init(&tg.lock, NULL);
+    if (!rc) {
+        /* largest estimated work first, each to the group with the least so far */
+        double load[AMR_MAX_GROUPS] = {0};
+        for (int k = 0; k < ninst; ++k) {
+            w[k] = inst_work(f[k]);
+            int i = k;
+            while (i > 0 && w[order[i - 1]] < w[k]) {
+                order[i] = order[i - 1];
+                --i;
+            }
+            order[i] = k;
+        }
+        for (int i = 0; i < ninst; ++i) {
+            int best = 0;
+            for (int gi = 1; gi < ng; ++gi)
+                if (load[gi] < load[best]) best = gi;
+            of[order[i]] = best;
+            load[best] += w[order[i]];
+        }
+        /* each group's instances in their input order, contiguous in fs / is / ps */
+        int pos = 0;
+        for (int gi = 0; gi < ng; ++gi) {
+            batch_group_t *g = &grp[gi];
+            g->f = fs + pos;
+            g->inst = is + pos;
+            g->per = ps + pos;
+            for (int k = 0; k < ninst; ++k)
+                if (of[k] == gi) {
+                    fs[pos] = f[k];
+                    is[pos] = inst[k];
+                    ++pos;
+                    ++g->n;
+                }
+            g->o = *o;
+            g->o.groups = 1;
+            if (o->tagger) {
+                g->o.tagger = shared_tagger;
+                g->o.tagger_ctx = &tg;
+            }
+        }
+        pthread_t th[AMR_MAX_GROUPS];
+        int started[AMR_MAX_GROUPS] = {0};
+        for (int gi = 1; gi < ng; ++gi)
+            started[gi] = pthread_create(&th[gi], NULL, batch_group_thread, &grp[gi]) == 0;
+        for (int gi = 1; gi < ng; ++gi)      /* a thread that could not start runs here */
+            if (!started[gi]) batch_group_thread(&grp[gi]);
+        batch_group_thread(&grp[0]);
+        for (int gi = 1; gi < ng; ++gi)
+            if (started[gi]) pthread_join(th[gi], NULL);
+        for (int gi = 0; gi < ng && !rc; ++gi) rc = grp[gi].rc;
+    }
+    if (order && of && w && fs && is && ps) {
+        amr_evolve_stats_t tot;
+        memset(&tot, 0, sizeof(tot));
+        for (int gi = 0; gi < ng; ++gi) {
+            const amr_evolve_stats_t *t = &grp[gi].tot;
+            if (t->steps > tot.steps) tot.steps = t->steps;
+            tot.regrids += t->regrids;
+            tot.peak_quads += t->peak_quads;
+            tot.cell_stages += t->cell_stages;
+            if (t->t > tot.t) tot.t = t->t;
+            /* the groups overlap: each phase's time is the longest group's */
+            if (t->wall_gpu > tot.wall_gpu) tot.wall_gpu = t->wall_gpu;
+            if (t->wall_regrid > tot.wall_regrid) tot.wall_regrid = t->wall_regrid;
+            if (t->wall_tags > tot.wall_tags) tot.wall_tags = t->wall_tags;
+            if (t->wall_forest > tot.wall_forest) tot.wall_forest = t->wall_forest;
+            if (t->wall_remesh > tot.wall_remesh) tot.wall_remesh = t->wall_remesh;
+        }
+        if (total) *total = tot;
+        if (per_inst) {
+            int pos = 0;
+            for (int gi = 0; gi < ng; ++gi)
+                for (int k = 0; k < ninst; ++k)
+                    if (of[k] == gi) per_inst[k] = ps[pos++];
+        }
+    }
+    pthread_mutex_destroy(&tg.lock);
+    free(order); free(of); free(w); free(fs); free(is); free(ps);
+    return rc;
+}
+
 int amr_evolve_batch(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst, const amr_evolve_opts_t *o,
                      amr_evolve_stats_t *per_inst, amr_evolve_stats_t *total)
 {
     if (!f || !inst || ninst <= 0) return AMRGPU_ERR_ARG;
+    if (o && o->groups > 1 && ninst > 1) return evolve_groups(f, inst, ninst, o, per_inst, total);
     const int n = amr_forest_nelem_subgrid(f[0]);
     for (int k = 1; k < ninst; ++k)
         if (amr_forest_nelem_subgrid(f[k]) != n) return AMRGPU_ERR_ARG;

@@ -94,9 +94,10 @@ def batch_forests(L, seeds, lmaxs, lmin, tol, n):
     return [L.riemann_forest(s, lmin - d, lm - d, tol, n=n) for s, lm in zip(seeds, lmaxs)]
 
 
-def run_batch(amrhip, forests, insts, tol, interval, n, tagger=None, lag=0):
+def run_batch(amrhip, forests, insts, tol, interval, n, tagger=None, lag=0, groups=1):
     t0 = time.perf_counter()
-    per, tot = amrhip.evolve_batch(forests, insts, tol, interval, tile="auto", tagger=tagger, regrid_lag=lag)
+    per, tot = amrhip.evolve_batch(forests, insts, tol, interval, tile="auto", tagger=tagger, regrid_lag=lag,
+                                   groups=groups)
     wall = time.perf_counter() - t0
     useful = sum(p.cell_stages for p in per)
     r = {"patch": n, "wall_s": round(wall, 3), "steps": tot.steps, "regrids": tot.regrids,
@@ -110,6 +111,9 @@ def run_batch(amrhip, forests, insts, tol, interval, n, tagger=None, lag=0):
          "end_to_end_Mcell_upd_s": useful / wall / 1e6,
          "gpu_Mcell_upd_s": useful / tot.wall_gpu / 1e6,
          "gpu_hbm_frac": useful / tot.wall_gpu * B_ALG_STAGE / (HBM_PEAK_GBS * 1e9)}
+    if groups > 1:
+        # the groups' GPU phases overlap one another: no single GPU time
+        r.update({"groups": groups, "gpu_s": None, "gpu_Mcell_upd_s": None, "gpu_hbm_frac": None})
     return r, per
 
 
@@ -485,6 +489,20 @@ def batched_experiment(amrhip, args, model, plain_run, plain_forests, problem):
     levels = (5, 6, 7)
     cache = {}
     r["l1_density_mean_levels_5_7"], nl1 = batch_l1(L, plain_forests, seeds, lmaxs, t_end, levels, cache)
+    ng = args.batch_groups
+    if ng > 1:
+        # the same batch split into groups on their own contexts and host
+        # threads: one group's host regrid overlaps the others' steps
+        fg = batch_forests(L, seeds, lmaxs, args.lmin, args.tol, 8)
+        rg, _ = run_batch(amrhip, fg, insts, args.tol, 10, 8, groups=ng)
+        rg["instances_bitwise_equal_to_one_context"] = int(sum(
+            bool(a.quads() == b.quads() and np.array_equal(a.soln(), b.soln())) for a, b in zip(fg, plain_forests)))
+        rg["speedup_vs_sequential_plain"] = rg["end_to_end_Mcell_upd_s"] / out["sequential_plain_every_10"][
+            "end_to_end_Mcell_upd_s"]
+        out[f"batched_plain_every_10_groups_{ng}"] = rg
+        log(f"batched plain, {ng} groups: {rg['wall_s']:.2f} s, {rg['end_to_end_Mcell_upd_s']:.0f} Mcell-upd/s "
+            f"end to end, {rg['instances_bitwise_equal_to_one_context']} of {len(seeds)} instances bitwise equal")
+        del fg
     if model is not None:
         iv, lag = args.learned_interval, args.learned_lag
         fb = batch_forests(L, seeds, lmaxs, args.lmin, args.tol, 8)
@@ -498,6 +516,16 @@ def batched_experiment(amrhip, args, model, plain_run, plain_forests, problem):
         rl["speedup_vs_sequential_plain"] = rl["end_to_end_Mcell_upd_s"] / out["sequential_plain_every_10"][
             "end_to_end_Mcell_upd_s"]
         rl["wall_time_ratio_sequential_plain_over_batched_learned"] = wall_alone / rl["wall_s"]
+        if ng > 1:
+            fb = batch_forests(L, seeds, lmaxs, args.lmin, args.tol, 8)
+            rlg, _ = run_batch(amrhip, fb, insts, args.tol, iv, 8, tagger=amrhip.MlpTagger(model, args.tol, True),
+                               lag=lag, groups=ng)
+            rlg["l1_density_mean_levels_5_7"], _ = batch_l1(L, fb, seeds, lmaxs, t_end, levels, cache)
+            rlg["speedup_vs_sequential_plain"] = rlg["end_to_end_Mcell_upd_s"] / out["sequential_plain_every_10"][
+                "end_to_end_Mcell_upd_s"]
+            rlg["wall_time_ratio_sequential_plain_over_batched_learned"] = wall_alone / rlg["wall_s"]
+            out[f"batched_learned_every_{iv}_lag_{lag}_groups_{ng}"] = rlg
+            log(f"batched learned, {ng} groups: {rlg['wall_s']:.2f} s, L1 {rlg['l1_density_mean_levels_5_7']:.4e}")
     out["l1_instances"] = nl1
     r["speedup_vs_sequential_plain"] = r["end_to_end_Mcell_upd_s"] / out["sequential_plain_every_10"][
         "end_to_end_Mcell_upd_s"]
@@ -518,6 +546,9 @@ def main():
     ap.add_argument("--t-end", type=float, default=0.25)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-groups", type=int, default=2,
+                    help="also run the 64-instance batch split into this many groups on their own contexts "
+                         "and host threads (<= 1: not)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile", type=str, default="", help="ppb,minw,design (skips the autotuner)")
     ap.add_argument("--workload", choices=["batch", "riemann4", "vortex"], default="batch",

@@ -126,7 +126,15 @@ typedef struct {
                                     k in (0, regrid_interval): the host regrids the topology on
                                     a thread while the GPU advances k more steps on the old mesh,
                                     then the state k steps later is remapped (tags k steps old) */
+    int    groups;               /* amr_evolve_batch only.  0 or 1: one device context for every
+                                    instance.  G > 1 (at most AMR_MAX_GROUPS): the instances are
+                                    split into G groups of about equal estimated work, each
+                                    advanced by its own context, stream and host thread, so that
+                                    one group's host regrid overlaps the others' time steps; every
+                                    instance's steps, regrids and results are unchanged */
 } amr_evolve_opts_t;
+
+#define AMR_MAX_GROUPS 8
 
 /* The reference's own tag source, as a tagger for amr_evolve_device /
  * amr_evolve_batch: solver_tag_feature's density thresholds
@@ -164,7 +172,9 @@ int amr_evolve_device(amr_forest_t *f, const amr_evolve_opts_t *opts, amr_evolve
  * per_inst[ninst] (may be NULL) receives each instance's statistics
  * (cell_stages counted while it has not reached its stop at the start of a
  * poll of at most 16 steps); *total the whole run's (cell_stages of every
- * launched cell).  Final states are downloaded into the forests. */
+ * launched cell; with opts->groups > 1 the sums over the groups, steps and
+ * the wall_* times those of the longest group).  Final states are downloaded
+ * into the forests. */
 int amr_evolve_batch(amr_forest_t **forests, const amrgpu_inst_t *inst, int ninst,
                      const amr_evolve_opts_t *opts, amr_evolve_stats_t *per_inst, amr_evolve_stats_t *total);
 

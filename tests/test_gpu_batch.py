@@ -59,6 +59,27 @@ def test_batch_equals_each_instance_alone(allow_coarsening, lag):
     assert tot.steps == max(steps) or tot.steps - max(steps) < 16
 
 
+@pytest.mark.parametrize("groups", [2, 3])
+def test_grouped_batch_equals_one_context(groups):
+    """groups > 1: the instances split over contexts advanced on their own
+    host threads give every instance the steps, regrids, mesh and state of
+    the single-context batch, bit for bit"""
+    params()
+    one = [forest(s, lm) for s, lm in CASES]
+    grouped = [forest(s, lm) for s, lm in CASES]
+    insts = [amrhip.make_inst(1.0, (0, 0, 0, 0), 0.3, 1.0, T_END) for _ in CASES]
+    print(f"groups {groups}, sizes {[f.nquad for f in one]}", flush=True)
+    per1, tot1 = amrhip.evolve_batch(one, insts, 0.1, 10)
+    perg, totg = amrhip.evolve_batch(grouped, insts, 0.1, 10, groups=groups)
+    for fa, fb, sa, sb in zip(one, grouped, per1, perg):
+        assert (sb.steps, sb.regrids, sb.t) == (sa.steps, sa.regrids, sa.t)
+        assert sb.cell_stages == sa.cell_stages
+        assert fb.quads() == fa.quads()
+        assert np.array_equal(fb.soln(), fa.soln())
+    assert totg.steps == tot1.steps or tot1.steps - totg.steps < 16
+    assert totg.regrids >= tot1.regrids
+
+
 def test_batch_with_learned_tagger_runs():
     params()
     batch = [forest(s, lm) for s, lm in CASES[:3]]
@@ -67,6 +88,13 @@ def test_batch_with_learned_tagger_runs():
     tg = amrhip.MlpTagger(m, 0.1, union_plain=True)
     per, tot = amrhip.evolve_batch(batch, insts, 0.1, 20, tagger=tg)
     s = tg.stats()
+    # the same tagger shared by two groups (one call at a time)
+    batch2 = [forest(s_, lm) for s_, lm in CASES[:3]]
+    per2, tot2 = amrhip.evolve_batch(batch2, insts, 0.1, 20, tagger=tg, groups=2)
+    assert tot2.regrids > 0
+    for f, st in zip(batch2, per2):
+        assert st.t == pytest.approx(T_END, abs=0)
+        assert f.is_balanced()
     assert tot.regrids > 0 and sum(s["predicted"]) > 0
     for f, st in zip(batch, per):
         assert st.t == pytest.approx(T_END, abs=0)
