@@ -884,12 +884,14 @@ typedef struct {
     int n;
     amr_evolve_opts_t o;
     amr_evolve_stats_t *per, tot;
+    int device;                /* the caller's HIP device */
     int rc;
 } batch_group_t;
 
 static void *batch_group_thread(void *arg)
 {
     batch_group_t *g = (batch_group_t *)arg;
+    amrgpu_use_device(g->device);
     g->rc = amr_evolve_batch(g->f, g->inst, g->n, &g->o, g->per, &g->tot);
     return NULL;
 }
@@ -925,6 +927,7 @@ static int evolve_groups(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst,
     tg.fn = o->tagger;
     tg.ctx = o->tagger_ctx;
     pthread_mutex_init(&tg.lock, NULL);
+    const int dev = amrgpu_current_device();
     if (!rc) {
         /* largest estimated work first, each to the group with the least so far */
         double load[AMR_MAX_GROUPS] = {0};
@@ -960,6 +963,7 @@ static int evolve_groups(amr_forest_t **f, const amrgpu_inst_t *inst, int ninst,
                 }
             g->o = *o;
             g->o.groups = 1;
+            g->device = dev;
             if (o->tagger) {
                 g->o.tagger = shared_tagger;
                 g->o.tagger_ctx = &tg;

@@ -2203,6 +2203,17 @@ extern "C" {
 
 const char *amrgpu_last_error(void) { return g_err; }
 
+int amrgpu_current_device(void)
+{
+    int dev = 0;
+    return hipGetDevice(&dev) == hipSuccess ? dev : 0;
+}
+
+void amrgpu_use_device(int dev)
+{
+    (void)hipSetDevice(dev);
+}
+
 amrgpu_ctx *amrgpu_create(void)
 {
     amrgpu_ctx *c = new amrgpu_ctx();

@@ -39,6 +39,10 @@ AMR_HIDDEN int amrgpu_staging(struct amrgpu_ctx *c, double **buf, size_t *ndoubl
  * off for the calling thread (the batch's per-instance regrids do).  Returns
  * the thread's previous setting, which the caller restores. */
 AMR_HIDDEN int amr_forest_host_parallel(int on);
+/* The calling thread's current HIP device, and making a device current on
+ * the calling thread (threads the library starts use the caller's device). */
+AMR_HIDDEN int amrgpu_current_device(void);
+AMR_HIDDEN void amrgpu_use_device(int dev);
 
 #ifdef __cplusplus
 }
