@@ -1,6 +1,8 @@
 // amr_mlp.hip -- learned refinement tagger on gfx950: patch features, and
 // the forward / backward passes of a 384 -> 64 -> 64 -> 3 ReLU perceptron on
-// the MFMA units (v_mfma_f32_16x16x16_bf16: BF16 operands, FP32 accumulate),
+// the MFMA units (gfx950's v_mfma_f32_16x16x32_bf16: BF16 operands, FP32
+// accumulate, K = 32 per instruction; the one K = 16 product, by the padded
+// logit gradients, uses v_mfma_f32_16x16x16_bf16),
 // with a deterministic two-pass gradient reduction and Adam.
 //
 // The reference has no learned tagger (it refines on density thresholds,
@@ -16,10 +18,11 @@
 // per chunk.  Each workgroup writes one partial gradient; a second kernel sums
 // the partials in a fixed order (bitwise reproducible) and applies Adam.
 //
-// MFMA 16x16x16 operand layout (lane l, r = l % 16, g = l / 16, t = 0..3):
-//   A[i][k]: lane (i, k/4) holds A[r][4g + t]
-//   B[k][j]: lane holds B[4g + t][r]
-//   D[i][j]: lane holds D[4g + t][r]
+// MFMA operand layouts (lane l, r = l % 16, g = l / 16):
+//   16x16x32: A[i][k]: lane holds A[r][8g + t], B[k][j]: lane holds
+//             B[8g + t][r], t = 0..7 (eight consecutive k per lane)
+//   16x16x16: the same with 4g + t, t = 0..3
+//   D[i][j] (both): lane holds D[4g + t][r], t = 0..3
 
 #include <hip/hip_runtime.h>
 
@@ -47,6 +50,8 @@ constexpr int ROWS = 64;                         // samples per chunk
 constexpr int BLK = 256;
 
 typedef short s4v __attribute__((ext_vector_type(4)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 b8v __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline uint16_t f2bf(float x)
@@ -61,6 +66,11 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 __device__ __forceinline__ f4v mfma(s4v a, s4v b, f4v c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4v mfma32(s8v a, s8v b, f4v c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8v, a), __builtin_bit_cast(b8v, b), c, 0,
+                                                   0, 0);
 }
 
 __device__ __forceinline__ unsigned long long d2bits(double v) { return (unsigned long long)__double_as_longlong(v); }
@@ -142,7 +152,7 @@ __global__ __launch_bounds__(256) void pool16_kernel(const double *__restrict__ 
 // ---------------------------------------------------------------------------
 // shared pieces of the network kernels
 // ---------------------------------------------------------------------------
-struct NetLds {
+struct __attribute__((aligned(16))) NetLds {     // rows read as 16-byte vectors
     uint16_t W1[NH * SX];
     uint16_t X[ROWS * SX];
     uint16_t W2[NH * SH];
@@ -153,7 +163,7 @@ struct NetLds {
     float Z[ROWS * 4];
 };
 
-struct TrainLds {
+struct __attribute__((aligned(16))) TrainLds {
     NetLds net;
     uint16_t dA2[ROWS * SH];
     uint16_t dA1[ROWS * SH];
@@ -218,12 +228,12 @@ __device__ void forward_rows(NetLds &s)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[c] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 4
-        for (int kk = 0; kk < NIN / 16; ++kk) {
-            const s4v a = *(const s4v *)&s.X[row * SX + 16 * kk + 4 * g];
+        for (int kk = 0; kk < NIN / 32; ++kk) {
+            const s8v a = *(const s8v *)&s.X[row * SX + 32 * kk + 8 * g];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const s4v bv = *(const s4v *)&s.W1[(16 * c + r) * SX + 16 * kk + 4 * g];
-                acc[c] = mfma(a, bv, acc[c]);
+                const s8v bv = *(const s8v *)&s.W1[(16 * c + r) * SX + 32 * kk + 8 * g];
+                acc[c] = mfma32(a, bv, acc[c]);
             }
         }
 #pragma unroll
@@ -240,12 +250,12 @@ __device__ void forward_rows(NetLds &s)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[c] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kk = 0; kk < NH / 16; ++kk) {
-            const s4v a = *(const s4v *)&s.H1[row * SH + 16 * kk + 4 * g];
+        for (int kk = 0; kk < NH / 32; ++kk) {
+            const s8v a = *(const s8v *)&s.H1[row * SH + 32 * kk + 8 * g];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const s4v bv = *(const s4v *)&s.W2[(16 * c + r) * SH + 16 * kk + 4 * g];
-                acc[c] = mfma(a, bv, acc[c]);
+                const s8v bv = *(const s8v *)&s.W2[(16 * c + r) * SH + 32 * kk + 8 * g];
+                acc[c] = mfma32(a, bv, acc[c]);
             }
         }
 #pragma unroll
@@ -260,10 +270,10 @@ __device__ void forward_rows(NetLds &s)
     {
         f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kk = 0; kk < NH / 16; ++kk) {
-            const s4v a = *(const s4v *)&s.H2[row * SH + 16 * kk + 4 * g];
-            const s4v bv = *(const s4v *)&s.W3[r * SH + 16 * kk + 4 * g];
-            acc = mfma(a, bv, acc);
+        for (int kk = 0; kk < NH / 32; ++kk) {
+            const s8v a = *(const s8v *)&s.H2[row * SH + 32 * kk + 8 * g];
+            const s8v bv = *(const s8v *)&s.W3[r * SH + 32 * kk + 8 * g];
+            acc = mfma32(a, bv, acc);
         }
         if (r < NOUT)
 #pragma unroll
@@ -364,15 +374,15 @@ __global__ __launch_bounds__(BLK) void grad_kernel(Weights wt, GradArgs ga)
 
         // dW3 += dZ^T H2 (wave w: hidden tile w)
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            s4v a, bv;
+        for (int kk = 0; kk < ROWS / 32; ++kk) {
+            s8v a, bv;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int smp = 16 * kk + 4 * g + t;
+            for (int t = 0; t < 8; ++t) {
+                const int smp = 32 * kk + 8 * g + t;
                 a[t] = (short)s.dZ[smp * SZ + r];
                 bv[t] = (short)n.H2[smp * SH + 16 * w + r];
             }
-            g3 = mfma(a, bv, g3);
+            g3 = mfma32(a, bv, g3);
         }
         // dA2 = (dZ W3) * [H2 > 0] for wave w's samples
         {
@@ -397,14 +407,14 @@ __global__ __launch_bounds__(BLK) void grad_kernel(Weights wt, GradArgs ga)
 #pragma unroll
             for (int c = 0; c < 4; ++c) acc[c] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const s4v a = *(const s4v *)&s.dA2[(16 * w + r) * SH + 16 * kk + 4 * g];
+            for (int kk = 0; kk < NH / 32; ++kk) {
+                const s8v a = *(const s8v *)&s.dA2[(16 * w + r) * SH + 32 * kk + 8 * g];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    s4v bv;
+                    s8v bv;
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) bv[t] = (short)n.W2[(16 * kk + 4 * g + t) * SH + 16 * c + r];
-                    acc[c] = mfma(a, bv, acc[c]);
+                    for (int t = 0; t < 8; ++t) bv[t] = (short)n.W2[(32 * kk + 8 * g + t) * SH + 16 * c + r];
+                    acc[c] = mfma32(a, bv, acc[c]);
                 }
             }
 #pragma unroll
@@ -417,35 +427,35 @@ __global__ __launch_bounds__(BLK) void grad_kernel(Weights wt, GradArgs ga)
         }
         // dW2 += dA2^T H1 (wave w: output-row tile w)
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            s4v a;
+        for (int kk = 0; kk < ROWS / 32; ++kk) {
+            s8v a;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) a[t] = (short)s.dA2[(16 * kk + 4 * g + t) * SH + 16 * w + r];
+            for (int t = 0; t < 8; ++t) a[t] = (short)s.dA2[(32 * kk + 8 * g + t) * SH + 16 * w + r];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                s4v bv;
+                s8v bv;
 #pragma unroll
-                for (int t = 0; t < 4; ++t) bv[t] = (short)n.H1[(16 * kk + 4 * g + t) * SH + 16 * c + r];
-                g2[c] = mfma(a, bv, g2[c]);
+                for (int t = 0; t < 8; ++t) bv[t] = (short)n.H1[(32 * kk + 8 * g + t) * SH + 16 * c + r];
+                g2[c] = mfma32(a, bv, g2[c]);
             }
         }
         __syncthreads();
         // dW1 += dA1^T X (wave w: feature tiles 6w..6w+5)
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            s4v a[4];
+        for (int kk = 0; kk < ROWS / 32; ++kk) {
+            s8v a[4];
 #pragma unroll
             for (int hc = 0; hc < 4; ++hc)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) a[hc][t] = (short)s.dA1[(16 * kk + 4 * g + t) * SH + 16 * hc + r];
+                for (int t = 0; t < 8; ++t) a[hc][t] = (short)s.dA1[(32 * kk + 8 * g + t) * SH + 16 * hc + r];
 #pragma unroll
             for (int fi = 0; fi < 6; ++fi) {
-                s4v bv;
+                s8v bv;
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    bv[t] = (short)n.X[(16 * kk + 4 * g + t) * SX + 16 * (6 * w + fi) + r];
+                for (int t = 0; t < 8; ++t)
+                    bv[t] = (short)n.X[(32 * kk + 8 * g + t) * SX + 16 * (6 * w + fi) + r];
 #pragma unroll
-                for (int hc = 0; hc < 4; ++hc) g1[hc][fi] = mfma(a[hc], bv, g1[hc][fi]);
+                for (int hc = 0; hc < 4; ++hc) g1[hc][fi] = mfma32(a[hc], bv, g1[hc][fi]);
             }
         }
         // bias gradients and loss: fixed summation order over the chunk's samples
