@@ -3,7 +3,7 @@ Riemann problems (seeds 1000..1063, levels 3..lmax with lmax drawn from
 {5, 6, 7, 8}, 8x8 FP64 patches, density-gradient regrid every 10 steps,
 SSP-RK3, input CFL 0.4) advanced together to t = 0.25 in one device context,
 each instance equal to its own run alone bit for bit: steps, regrids, final
-mesh and state."""
+mesh and state; and the same batch split into two groups equal to it."""
 import time
 
 import numpy as np
@@ -60,3 +60,13 @@ def test_64_instances_to_t025_each_equal_to_alone():
     print(f"peak cells per instance {min(sizes)}..{max(sizes)}", flush=True)
     # uneven sizes: peaks from about 18k to 0.5M cells
     assert min(sizes) < 30_000 and max(sizes) > 400_000
+    # the same 64 instances split into two groups (own contexts and host
+    # threads, regrids overlapping steps): each instance as in one context
+    grouped = [forest(s, lm) for s, lm in zip(seeds, lmaxs)]
+    t0 = time.perf_counter()
+    perg, totg = amrhip.evolve_batch(grouped, insts, TOL, 10, groups=2)
+    print(f"two groups: {time.perf_counter() - t0:.1f} s", flush=True)
+    for k in range(K):
+        assert (perg[k].steps, perg[k].regrids, perg[k].t) == (per[k].steps, per[k].regrids, per[k].t), k
+        assert grouped[k].quads() == batch[k].quads(), f"instance {k}"
+        assert np.array_equal(grouped[k].soln(), batch[k].soln()), f"instance {k}"
