@@ -1625,21 +1625,23 @@ __global__ void slot_reset_kernel(unsigned long long *slot, int ninst)
     for (int i = threadIdx.x; i < ninst * NSLOT; i += blockDim.x) slot[i] = DBL_INF_BITS;
 }
 
-// RK1 has a single stage: the step's dt and time update run in this tiny
-// kernel ahead of it.
-__global__ void dt_prep_kernel(MeshDev m, int use_cfl, double *dt_cur, double *t_inst, double *t_step,
-                               unsigned long long *slot)
+// The step's dt and time update, ahead of its first stage: one wave per
+// instance forms min(input dt, CFL minimum over the instance's sub-slots,
+// time left), publishes it with the step's start time, advances the time and
+// re-arms the sub-slots that the step's last stage fills.  The stages then
+// read the published dt instead of every wave reducing the sub-slots.
+__global__ __launch_bounds__(64) void dt_prep_kernel(MeshDev m, int use_cfl, double *dt_cur, double *t_inst,
+                                                     double *t_step, unsigned long long *slot)
 {
-    // one wave
-    for (int i = 0; i < m.ninst; ++i) {
-        double dt = step_dt(m.ip[i], slot, t_inst, i, use_cfl);
-        if (threadIdx.x == 0) {
-            dt_cur[i] = dt;
-            t_step[i] = t_inst[i];
-            t_inst[i] += dt;
-        }
-        for (int k = threadIdx.x; k < NSLOT; k += 64) slot[(size_t)i * NSLOT + k] = DBL_INF_BITS;
+    const int i = blockIdx.x;
+    if (i >= m.ninst) return;
+    const double dt = step_dt(m.ip[i], slot, t_inst, i, use_cfl);
+    if (threadIdx.x == 0) {
+        dt_cur[i] = dt;
+        t_step[i] = t_inst[i];
+        t_inst[i] += dt;
     }
+    for (int k = threadIdx.x; k < NSLOT; k += 64) slot[(size_t)i * NSLOT + k] = DBL_INF_BITS;
 }
 
 // reference patch layout Q(4,n,n) (AoS) <-> U[quad][field][cell] (SoA)
@@ -2060,23 +2062,23 @@ int build_step(amrgpu_ctx *c, int cur, std::vector<StageArgs> &st, bool with_nor
         return (cur + 1) % 3;
     case 2:
         a.Us = A; a.U0 = nullptr; a.Uout = B; a.mode = MODE_SO; a.c0 = 0; a.c1 = 1; a.c2 = 1;
-        a.role = ROLE_DT; st.push_back(a);
+        a.role = 0; st.push_back(a);
         a.Us = B; a.U0 = A; a.Uout = A; a.c0 = 0.5; a.c1 = 0.5; a.c2 = 0.5;
-        a.role = ROLE_TIME; if (with_norm) a.norm_part = c->d_norm; st.push_back(a);
+        a.role = 0; if (with_norm) a.norm_part = c->d_norm; st.push_back(a);
         return cur;
     case 3:
         a.Us = A; a.U0 = nullptr; a.Uout = B; a.mode = MODE_SO; a.c0 = 0; a.c1 = 1; a.c2 = 1;
-        a.role = ROLE_DT; st.push_back(a);
+        a.role = 0; st.push_back(a);
         a.Us = B; a.U0 = A; a.Uout = C; a.c0 = 0.75; a.c1 = 0.25; a.c2 = 0.25;
-        a.role = ROLE_TIME; st.push_back(a);
+        a.role = 0; st.push_back(a);
         a.Us = C; a.U0 = A; a.Uout = A; a.c0 = 1.0 / 3.0; a.c1 = 2.0 / 3.0; a.c2 = 2.0 / 3.0;
         a.role = cfl; if (with_norm) a.norm_part = c->d_norm; st.push_back(a);
         return cur;
     case 4:
         a.Acc = c->Acc;
         a.Us = A; a.U0 = A; a.Uout = B; a.mode = MODE_RK4_ACC; a.first = 1; a.c2 = 0.5;
-        a.role = ROLE_DT; st.push_back(a);
-        a.Us = B; a.Uout = C; a.first = 0; a.c2 = 0.5; a.role = ROLE_TIME; st.push_back(a);
+        a.role = 0; st.push_back(a);
+        a.Us = B; a.Uout = C; a.first = 0; a.c2 = 0.5; a.role = 0; st.push_back(a);
         a.Us = C; a.Uout = B; a.c2 = 1.0; a.role = 0; st.push_back(a);
         a.Us = B; a.Uout = A; a.mode = MODE_RK4_FINAL; a.role = cfl;
         if (with_norm) a.norm_part = c->d_norm; st.push_back(a);
@@ -2093,7 +2095,7 @@ int build_step(amrgpu_ctx *c, int cur, std::vector<StageArgs> &st, bool with_nor
         double *X = A, *Y = B;
         for (int s = 0; s < 5; ++s) {
             a.Us = X; a.U0 = nullptr; a.Uout = Y; a.first = (s == 0); a.c1 = ra[s]; a.c2 = rb[s];
-            a.role = (s == 0 ? ROLE_DT : 0) | (s == 1 ? ROLE_TIME : 0) | (s == 4 ? cfl : 0);
+            a.role = s == 4 ? cfl : 0;
             a.norm_part = (s == 4 && with_norm) ? c->d_norm : nullptr;
             st.push_back(a);
             double *tmp = X; X = Y; Y = tmp;
@@ -2110,11 +2112,9 @@ int enqueue_step(amrgpu_ctx *c, int cur, bool with_norm, int *next)
     std::vector<StageArgs> st;
     int nx = build_step(c, cur, st, with_norm);
     if (nx < 0) { set_err("bad time scheme %d", c->scheme); return AMRGPU_ERR_ARG; }
-    if (c->scheme == 1) {
-        hipLaunchKernelGGL(dt_prep_kernel, dim3(1), dim3(64), 0, c->stream, mesh_of(c), 1,
-                           c->d_dt_cur, c->d_t, c->d_tstep, c->d_slot);
-        HIPCHK(hipGetLastError());
-    }
+    hipLaunchKernelGGL(dt_prep_kernel, dim3(std::max(1, c->ninst)), dim3(64), 0, c->stream, mesh_of(c),
+                       c->scheme != 2, c->d_dt_cur, c->d_t, c->d_tstep, c->d_slot);
+    HIPCHK(hipGetLastError());
     for (auto &a : st) {
         int rc = launch_stage(c, a);
         if (rc) return rc;
@@ -2895,10 +2895,8 @@ int amrgpu_bench(amrgpu_ctx *c, int nsteps, double *ms_total, double *avg_stage_
     // per-stage kernel durations: direct launches bracketed by events
     for (int s = 0; s < nsteps; ++s) {
         int nx = build_step(c, c->cur, st, false);
-        if (c->scheme == 1) {
-            hipLaunchKernelGGL(dt_prep_kernel, dim3(1), dim3(64), 0, c->stream, mesh_of(c), 1,
-                               c->d_dt_cur, c->d_t, c->d_tstep, c->d_slot);
-        }
+        hipLaunchKernelGGL(dt_prep_kernel, dim3(std::max(1, c->ninst)), dim3(64), 0, c->stream, mesh_of(c),
+                           c->scheme != 2, c->d_dt_cur, c->d_t, c->d_tstep, c->d_slot);
         HIPCHK(hipEventRecord(ev[(size_t)s * (ns + 1)], c->stream));
         for (int k = 0; k < ns; ++k) {
             rc = launch_stage(c, st[k]);
