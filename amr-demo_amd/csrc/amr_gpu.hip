@@ -43,8 +43,8 @@ constexpr unsigned long long DBL_INF_BITS = 0x7ff0000000000000ull;
 // folds into sub-slot b % NSLOT): atomics on one address serialise, and on a
 // 16.8M-cell mesh with one patch per workgroup 64 sub-slots still left the
 // last stage ~17 % slower than the others (4096 atomics per address); 256
-// recover most of it while the first stage's wave still reduces the
-// sub-slots with four loads per lane.
+// recover most of it, and the one wave per instance of the next step's
+// dt_prep_kernel reduces them with four loads per lane.
 constexpr int NSLOT = 256;
 
 struct InstDev {
@@ -79,7 +79,7 @@ struct MeshDev {
 };
 
 enum { MODE_SO = 0, MODE_RK4_ACC = 1, MODE_RK4_FINAL = 2, MODE_LSERK = 3, MODE_RHS = 4 };
-enum { ROLE_DT = 1, ROLE_TIME = 2, ROLE_CFL = 4 };
+enum { ROLE_CFL = 4 };
 
 struct StageArgs {
     MeshDev m;
@@ -90,11 +90,8 @@ struct StageArgs {
     double c0, c1, c2;
     int mode;
     int first;           // first stage of an accumulating scheme
-    int role;
-    int use_cfl;         // stage-1 dt includes the CFL slot
-    double *dt_cur;
-    double *t_inst;
-    double *t_step;      // written at the step's first stage (equal values)
+    int role;            // ROLE_CFL: the stage folds the next step's CFL minimum
+    const double *dt_cur;// ninst: the step's dt (dt_prep_kernel)
     unsigned long long *slot;
     double *norm_part;
     int per_block;       // persistent design: patches per workgroup (a contiguous range)
@@ -667,7 +664,7 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
     for (int f = 0; f < 4; ++f) U[f] = a.Us[base + f * N2];
     int nbw[8];
     int levw = 0, inw = 0;
-    bool actw = false, firstw = false;   // firstw: the wave holds its instance's first quadrant
+    bool actw = false;
     // one paired-face patch per workgroup: every reader of the patch's ids
     // and parameters takes them from the wave's scalars, and the LDS copies
     // the other tiles share between waves are not written
@@ -683,7 +680,6 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         for (int k = 0; k < 8; ++k) nbw[k] = nbc[k];
         levw = ((const cint4 *)a.m.level)[qc];
         inw = ((const cint4 *)a.m.inst)[qc];
-        firstw = actw && (qc == 0 || ((const cint4 *)a.m.inst)[qc - 1] != inw);
         if (!actw) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) nbw[k] = -1;
@@ -739,20 +735,6 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             store_rec<N, PPB>(lds, p, TL::GHOST + hh, g0, g1, g2, g3);
         }
     }
-    // first stage of a step: the CFL sub-slots of the patch's instance, read
-    // after the halo loads are issued (a wait for them, wherever the compiler
-    // places the reduction, then holds back no halo load); the step's dt is
-    // formed from them after the records
-    unsigned long long slv[(NSLOT + 63) / 64];
-    if constexpr (UNI) {
-        if ((a.role & ROLE_DT) && a.use_cfl) {
-            const unsigned long long *sp = a.slot + (size_t)inw * NSLOT;
-            const int lane = threadIdx.x & 63;
-#pragma unroll
-            for (int k = 0; k < (NSLOT + 63) / 64; ++k)
-                slv[k] = lane + 64 * k < NSLOT ? sp[lane + 64 * k] : DBL_INF_BITS;
-        }
-    }
     // patch parameters that depend on the instance
     double hw = 0.0, dtw = 0.0;
     if constexpr (UNI) {
@@ -760,13 +742,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
         typedef const double __attribute__((address_space(4))) cdbl4;
         const cinst4 *ipc = (const cinst4 *)a.m.ip;
         hw = actw ? ldexp(ipc[inw].h_base, -levw) / (double)N : 0.0;
-        double tw = 0.0;
-        if (!(a.role & ROLE_DT)) {
-            // written by an earlier launch, constant during this one
-            dtw = actw && a.mode != MODE_RHS ? ((const cdbl4 *)a.dt_cur)[inw] : 0.0;
-            tw = actw ? ((const cdbl4 *)a.m.t_step)[inw] : 0.0;
-            tpw = tw;
-        }
+        // the step's dt and start time, written by the step's prep kernel
+        dtw = actw && a.mode != MODE_RHS ? ((const cdbl4 *)a.dt_cur)[inw] : 0.0;
+        const double tw = actw ? ((const cdbl4 *)a.m.t_step)[inw] : 0.0;
+        tpw = tw;
         if (!SMETA && c == 0) {
             pquad[p] = actw ? lb * PPB + p : -1;
             plevel[p] = levw;
@@ -774,10 +753,8 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
 #pragma unroll
             for (int k = 0; k < 8; ++k) pnbr[p][k] = nbw[k];
             ph[p] = hw;
-            if (!(a.role & ROLE_DT)) {
-                pdt[p] = dtw;
-                ptime[p] = tw;
-            }
+            pdt[p] = dtw;
+            ptime[p] = tw;
         }
     } else if (tid < PPB) {
         int qq = lb * PPB + tid;
@@ -789,34 +766,9 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
 #pragma unroll
         for (int k = 0; k < 8; ++k) pnbr[tid][k] = act ? a.m.nbr[qq * 8 + k] : -1;
         ph[tid] = act ? ldexp(a.m.ip[in].h_base, -lev) / (double)N : 0.0;
-        if (!(a.role & ROLE_DT)) {
-            pdt[tid] = act && a.mode != MODE_RHS ? a.dt_cur[in] : 0.0;
-            // the step's start time, stable while the time advances
-            ptime[tid] = act ? a.m.t_step[in] : 0.0;
-        }
-    }
-    if (!UNI && (a.role & ROLE_DT) && tid < 64) {
-        // first stage of a step: wave 0 derives the dt of the block's
-        // instances from the CFL sub-slots and publishes it
-        int last_inst = -1;
-        double last_dt = 0.0;
-        for (int q = 0; q < PPB; ++q) {
-            int qq = lb * PPB + q;
-            if (qq >= a.m.nquad) break;
-            int in = a.m.inst[qq];
-            if (in != last_inst) {
-                last_dt = step_dt(a.m.ip[in], a.slot, a.t_inst, in, a.use_cfl);
-                last_inst = in;
-                if ((tid & 63) == 0) {                          // identical values from every writer
-                    a.dt_cur[in] = last_dt;
-                    a.t_step[in] = a.t_inst[in];
-                }
-            }
-            if ((tid & 63) == 0) {
-                pdt[q] = last_dt;
-                ptime[q] = a.t_inst[in];
-            }
-        }
+        pdt[tid] = act && a.mode != MODE_RHS ? a.dt_cur[in] : 0.0;
+        // the step's start time, stable while the time advances
+        ptime[tid] = act ? a.m.t_step[in] : 0.0;
     }
     if constexpr (ONE_GHOST) {
         if (active && c < 4 * N) {
@@ -829,44 +781,6 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
 #pragma unroll
         for (int f = 0; f < 4; ++f)
             if constexpr (!TL::SPLIT) ldsU[(p * NFLD + f) * N2 + c] = U[f];
-    }
-    if constexpr (UNI) {
-        if (a.role & ROLE_DT) {
-            // the step's dt of the patch's instance (step_dt with the slots
-            // loaded above; every wave of the instance forms the same value)
-            typedef const InstDev __attribute__((address_space(4))) cinst4;
-            typedef const double __attribute__((address_space(4))) cdbl4;
-            const cinst4 *ipc = (const cinst4 *)a.m.ip;
-            double dt = ipc[inw].dt_cap;
-            if (a.use_cfl) {
-                double m = INFINITY;
-                // the slots pass through an empty asm here, so that their
-                // reduction (and the wait for their loads) is not moved up
-                // to the loads, ahead of the records' arithmetic
-#pragma unroll
-                for (int k = 0; k < (NSLOT + 63) / 64; ++k) asm volatile("" : "+v"(slv[k]));
-#pragma unroll
-                for (int k = 0; k < (NSLOT + 63) / 64; ++k) m = fmin(m, bits2d(slv[k]));
-                m = wave_min(m);
-                if (m < dt) dt = m;
-            }
-            const double ti = ((const cdbl4 *)a.t_inst)[inw];   // advanced by a later stage only
-            const double t_end = ipc[inw].t_end;
-            if (t_end > 0.0) {
-                double rem = t_end - ti;
-                if (rem < 0.0) rem = 0.0;
-                if (rem < dt) dt = rem;
-            }
-            dtw = uniform_f64(dt);     // used by the epilogue only
-            tpw = ti;
-            if (c == 0) {
-                if (!SMETA) ptime[p] = ti;
-                if (firstw) {                 // one writer per instance (a hot line otherwise)
-                    a.dt_cur[inw] = dt;
-                    a.t_step[inw] = ti;
-                }
-            }
-        }
     }
     __syncthreads();
 
@@ -1232,12 +1146,6 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             __builtin_nontemporal_store(out[f], &a.Uout[base + f * N2]);
         if (a.role & ROLE_CFL) cand = cfl_candidate_h(out, cflh);
     }
-    // time bookkeeping: stage 2 advances t by this step's dt and re-arms the
-    // CFL slot that the last stage fills
-    if ((a.role & ROLE_TIME) && blockIdx.x == 0) {
-        for (int i = tid; i < a.m.ninst; i += TL::BLK) a.t_inst[i] += a.dt_cur[i];
-        for (int i = tid; i < a.m.ninst * NSLOT; i += TL::BLK) a.slot[i] = DBL_INF_BITS;
-    }
     if (a.role & ROLE_CFL) {
         int uinst = SMETA ? inw : pinst[0];
 #pragma unroll
@@ -1346,12 +1254,6 @@ __global__ __launch_bounds__(N * N, MINW) void stage_kernel_pp(StageArgs a)
     const cint4 *nbrc = (const cint4 *)a.m.nbr;
     const cinst4 *ipc = (const cinst4 *)a.m.ip;
 
-    // time bookkeeping of the step's second stage (the t_inst / slot roles)
-    if ((a.role & ROLE_TIME) && blockIdx.x == 0) {
-        for (int i = tid; i < a.m.ninst; i += N2) a.t_inst[i] += a.dt_cur[i];
-        for (int i = tid; i < a.m.ninst * NSLOT; i += N2) a.slot[i] = DBL_INF_BITS;
-    }
-
     int nbw[8];
     bool hangw = false;
     if (q0 < q1) {
@@ -1371,20 +1273,11 @@ __global__ __launch_bounds__(N * N, MINW) void stage_kernel_pp(StageArgs a)
         __syncthreads();
         const int levw = ((const cint4 *)a.m.level)[q];
         const int inw = ((const cint4 *)a.m.inst)[q];
-        const bool firstw = q == 0 || ((const cint4 *)a.m.inst)[q - 1] != inw;
         double B[4] = {0.0, 0.0, 0.0, 0.0};
         const size_t base = (size_t)q * NFLD * N2 + c;
         if (needB) {
 #pragma unroll
             for (int f = 0; f < 4; ++f) B[f] = gptr(a.U0)[base + f * N2];
-        }
-        unsigned long long slv[(NSLOT + 63) / 64];
-        if ((a.role & ROLE_DT) && a.use_cfl) {
-            const unsigned long long *sp = a.slot + (size_t)inw * NSLOT;
-            const int lane = tid & 63;
-#pragma unroll
-            for (int k = 0; k < (NSLOT + 63) / 64; ++k)
-                slv[k] = lane + 64 * k < NSLOT ? sp[lane + 64 * k] : DBL_INF_BITS;
         }
         // the next patch's neighbour ids (scalar loads, used below)
         int nbn[8];
@@ -1406,33 +1299,9 @@ __global__ __launch_bounds__(N * N, MINW) void stage_kernel_pp(StageArgs a)
         }
         // patch parameters behind the instance id
         const double hw = ldexp(ipc[inw].h_base, -levw) / (double)N;
-        double dtw = 0.0, tw = 0.0;
-        if (a.role & ROLE_DT) {
-            double dt = ipc[inw].dt_cap;
-            if (a.use_cfl) {
-                double m = INFINITY;
-#pragma unroll
-                for (int k = 0; k < (NSLOT + 63) / 64; ++k) m = fmin(m, bits2d(slv[k]));
-                m = wave_min(m);
-                if (m < dt) dt = m;
-            }
-            const double ti = ((const cdbl4 *)a.t_inst)[inw];
-            const double t_end = ipc[inw].t_end;
-            if (t_end > 0.0) {
-                double rem = t_end - ti;
-                if (rem < 0.0) rem = 0.0;
-                if (rem < dt) dt = rem;
-            }
-            dtw = dt;
-            tw = ti;
-            if (c == 0 && firstw) {          // one writer per instance
-                a.dt_cur[inw] = dt;
-                a.t_step[inw] = ti;
-            }
-        } else {
-            dtw = a.mode != MODE_RHS ? ((const cdbl4 *)a.dt_cur)[inw] : 0.0;
-            tw = ((const cdbl4 *)a.m.t_step)[inw];
-        }
+        // the step's dt and start time, written by the step's prep kernel
+        const double dtw = a.mode != MODE_RHS ? ((const cdbl4 *)a.dt_cur)[inw] : 0.0;
+        const double tw = ((const cdbl4 *)a.m.t_step)[inw];
         // the next patch's loads go out now, into the other buffer (read
         // last by the patch before this one, which every wave has finished)
         if (more) {
@@ -2037,10 +1906,7 @@ StageArgs base_args(amrgpu_ctx *c)
     memset(&a, 0, sizeof(a));
     a.m = mesh_of(c);
     a.dt_cur = c->d_dt_cur;
-    a.t_inst = c->d_t;
-    a.t_step = c->d_tstep;
     a.slot = c->d_slot;
-    a.use_cfl = (c->scheme != 2);
     return a;
 }
 
