@@ -205,7 +205,8 @@ def roofline_of(cells, ms, steps, n, tile, launch, stage_ms):
         "kernel": f"stage_kernel<{n},{tile[0]},{tile[1]},{tile[2]}>",
         "bytes_per_launch": int(bytes_launch),
         "avg_launch_us": round(avg_stage * 1e3, 3),
-        "avg_launch_from": f"timed region / launches (HIP events on the launch stream, {launch})",
+        "avg_launch_from": f"timed region / stage launches, 3 per step, the step's dt prep launch counted in "
+                           f"the time (HIP events on the launch stream, {launch})",
         "stage_launch_us_bracketed": [round(s * 1e3, 3) for s in stage_ms],
     }
 
