@@ -1126,7 +1126,10 @@ __global__ __launch_bounds__((Tile<N, PPB>::BLK), (MINW)) void stage_kernel(Stag
             if (s2) fyb += F2[(2 * NFLD + f) * N + ci];
             double Rf = (fxr - fxl) + (fyt - fyb);
             k4[f] = Rf * ninvV;
-            sq = fma(k4[f], k4[f], sq);
+        }
+        if (a.norm_part) {        // the residual norm, only on the launches that report it
+#pragma unroll
+            for (int f = 0; f < 4; ++f) sq = fma(k4[f], k4[f], sq);
         }
         double dt = 0.0;
         if (a.mode != MODE_RHS) dt = UNI ? dtw : pdt[p];
