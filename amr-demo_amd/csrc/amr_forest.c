@@ -247,6 +247,31 @@ void amr_forest_quad_geom(const amr_forest_t *f, int k, double xyz[12])
     }
 }
 
+/* The lower-left corner of quadrant k: corner 0 of amr_forest_quad_geom,
+ * operation for operation (so bit for bit), without the other corners. */
+static void quad_origin(const amr_forest_t *f, int k, double *ox, double *oy)
+{
+    const amr_quad_t *q = &f->q[k];
+    int tx = q->tree % f->ntx, ty = q->tree / f->ntx;
+    double v[4][2];
+    for (int c = 0; c < 4; ++c) {
+        int ix = tx + (c & 1), iy = ty + (c >> 1);
+        v[c][0] = (double)ix * f->scale + f->xlo;
+        v[c][1] = (double)iy * f->scale + f->ylo;
+    }
+    const double intsize = 1.0 / (double)AMR_ROOT_LEN;
+    double h2 = 0.5 * intsize * (double)qlen(q->level);
+    double eta_z = 0.0;
+    double eta_y = intsize * (double)q->y + h2 * (1.0 + (2 * 0 - 1));
+    double eta_x = intsize * (double)q->x + h2 * (1.0 + (2 * 0 - 1));
+    double o[2];
+    for (int j = 0; j < 2; ++j)
+        o[j] = ((1. - eta_z) * ((1. - eta_y) * ((1. - eta_x) * v[0][j] + eta_x * v[1][j]) +
+                                eta_y * ((1. - eta_x) * v[2][j] + eta_x * v[3][j])));
+    *ox = o[0];
+    *oy = o[1];
+}
+
 static int cmp_pos(const amr_forest_t *f, int k, int tree, uint64_t key)
 {
     if (f->q[k].tree != tree) return f->q[k].tree < tree ? -1 : 1;
@@ -423,11 +448,8 @@ int amr_forest_build_nbr(amr_forest_t *f, int *nbr, int *level, double *org)
     int err = 0;
 #pragma omp parallel for schedule(static) reduction(| : err) if (nq >= PAR_MIN && t_parallel)
     for (int k = 0; k < nq; ++k) {
-        double g[12];
-        amr_forest_quad_geom(f, k, g);
         level[k] = f->q[k].level;
-        org[2 * (size_t)k] = g[0];
-        org[2 * (size_t)k + 1] = g[1];
+        quad_origin(f, k, &org[2 * (size_t)k], &org[2 * (size_t)k + 1]);
         for (int s = 0; s < 4; ++s) {
             int nb[2] = {-1, -1};
             const int c = classify_side(f, k, s, nb);
